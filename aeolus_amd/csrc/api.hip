@@ -1,0 +1,531 @@
+// C entry points: argument validation, plan/model construction and the
+// orchestration of the per-stage kernels.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace aeo {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* get_error() { return g_last_error.c_str(); }
+
+int fft_factor(int n, int* radices) {
+    int k = 0;
+    while (n % 4 == 0 && k < kMaxRadices) { radices[k++] = 4; n /= 4; }
+    while (n % 2 == 0 && k < kMaxRadices) { radices[k++] = 2; n /= 2; }
+    while (n % 3 == 0 && k < kMaxRadices) { radices[k++] = 3; n /= 3; }
+    while (n % 5 == 0 && k < kMaxRadices) { radices[k++] = 5; n /= 5; }
+    for (int p = 7; n > 1 && k < kMaxRadices; p += 2) {
+        while (n % p == 0 && k < kMaxRadices) { radices[k++] = p; n /= p; }
+    }
+    return n == 1 ? k : -1;
+}
+
+template <class T>
+static int dev_alloc(T** ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) return AEO_OK;
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                  " bytes failed: " + hipGetErrorString(e));
+        *ptr = nullptr;
+        return AEO_ERR_HIP;
+    }
+    return AEO_OK;
+}
+
+template <class T>
+static int upload(T** ptr, const T* host, size_t count) {
+    int rc = dev_alloc(ptr, count);
+    if (rc) return rc;
+    AEO_CHECK_HIP(hipMemcpy(*ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+    return AEO_OK;
+}
+
+static void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+static void free_plan(aeo_plan_s* p) {
+    if (!p) return;
+    for (void* q : {(void*)p->d_Ps, (void*)p->d_Pa, (void*)p->d_offs, (void*)p->d_w_rows,
+                    (void*)p->d_coslat, (void*)p->d_tw, (void*)p->d_E, (void*)p->d_O,
+                    (void*)p->d_S, (void*)p->d_D, (void*)p->d_Yk, (void*)p->d_Yp, (void*)p->d_w,
+                    (void*)p->d_pcoslat, (void*)p->d_gw, (void*)p->d_part, (void*)p->d_Cp,
+                    (void*)p->d_Cm, (void*)p->d_spec})
+        dev_free(q);
+    delete p;
+}
+
+static void free_model(aeo_model_s* m) {
+    if (!m) return;
+    for (void* q : {(void*)m->d_nulap, (void*)m->d_lap, (void*)m->d_nu4lap2,
+                    (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
+                    (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
+                    (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ksum})
+        dev_free(q);
+    delete m;
+}
+
+// C+/C- coupling coefficients of sin(theta) d/dtheta
+// (num/spectral_operators.py:135-176), row-major (L+1)^2.
+static void coupling_tables(int L, std::vector<double>& Cp, std::vector<double>& Cm) {
+    const int n = L + 1;
+    Cp.assign((size_t)n * n, 0.0);
+    Cm.assign((size_t)n * n, 0.0);
+    for (int l = 0; l <= L; ++l) {
+        for (int m = 0; m <= l; ++m) {
+            const double Lb = l, Mb = m;
+            const double np_ = std::max((Lb + 1.0) * (Lb + 1.0) - Mb * Mb, 0.0);
+            const double dp = std::max((2.0 * Lb + 1.0) * (2.0 * Lb + 3.0), 1.0);
+            const double nm = std::max(Lb * Lb - Mb * Mb, 0.0);
+            const double dm = std::max((2.0 * Lb - 1.0) * (2.0 * Lb + 1.0), 1.0);
+            Cp[(size_t)l * n + m] = (l == L) ? 0.0 : Lb * std::sqrt(np_ / dp);
+            Cm[(size_t)l * n + m] = (l == 0) ? 0.0 : -(Lb + 1.0) * std::sqrt(nm / dm);
+        }
+    }
+}
+
+static int common_tables(aeo_plan_s* p) {
+    std::vector<double> Cp, Cm;
+    coupling_tables(p->L, Cp, Cm);
+    int rc = upload(&p->d_Cp, Cp.data(), Cp.size());
+    if (rc) return rc;
+    rc = upload(&p->d_Cm, Cm.data(), Cm.size());
+    if (rc) return rc;
+    const int nw = std::max(p->max_fields, 4);
+    return dev_alloc(&p->d_spec, (size_t)nw * (p->L + 1) * (p->L + 1));
+}
+
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace aeo
+
+using namespace aeo;
+
+extern "C" {
+
+int aeo_version(void) { return AEO_ABI_VERSION; }
+
+const char* aeo_last_error(void) { return get_error(); }
+
+int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
+                          const double* w_rows, const double* coslat_rows, int max_fields,
+                          aeo_plan_t* out) {
+    AEO_REQUIRE(out != nullptr, AEO_ERR_INVALID, "aeo_plan_create_gauss: out is NULL");
+    *out = nullptr;
+    AEO_REQUIRE(x_nodes && w_rows && coslat_rows, AEO_ERR_INVALID,
+                "aeo_plan_create_gauss: x_nodes, w_rows and coslat_rows must be non-NULL");
+    AEO_REQUIRE(nlat >= 2 && nlon >= 4, AEO_ERR_INVALID,
+                "aeo_plan_create_gauss: needs nlat >= 2 and nlon >= 4");
+    AEO_REQUIRE(l_max >= 0, AEO_ERR_INVALID, "aeo_plan_create_gauss: l_max must be >= 0");
+    AEO_REQUIRE(max_fields >= 1, AEO_ERR_INVALID, "aeo_plan_create_gauss: max_fields must be >= 1");
+    AEO_REQUIRE(nlon <= kChunkElems, AEO_ERR_UNSUPPORTED,
+                "aeo_plan_create_gauss: nlon > 2048 is not supported by the LDS FFT");
+    int radices[kMaxRadices];
+    const int nst = fft_factor(nlon, radices);
+    AEO_REQUIRE(nst > 0, AEO_ERR_UNSUPPORTED,
+                "aeo_plan_create_gauss: nlon has too many prime factors for the LDS FFT");
+    for (int j = 0; j < nlat; ++j) {
+        const int js = nlat - 1 - j;
+        AEO_REQUIRE(std::fabs(x_nodes[j] + x_nodes[js]) <= 1e-15 &&
+                        std::fabs(w_rows[j] - w_rows[js]) <= 1e-15 * std::fabs(w_rows[j]) &&
+                        (j == 0 || x_nodes[j] < x_nodes[j - 1]),
+                    AEO_ERR_INVALID,
+                    "aeo_plan_create_gauss: nodes must be descending and symmetric about the "
+                    "equator (x[j] = -x[nlat-1-j], equal weights)");
+    }
+    aeo_plan_s* p = new (std::nothrow) aeo_plan_s();
+    AEO_REQUIRE(p != nullptr, AEO_ERR_HIP, "aeo_plan_create_gauss: out of host memory");
+    p->kind = kGauss;
+    p->L = l_max;
+    p->B = n_basis(l_max);
+    p->nlat = nlat;
+    p->nlon = nlon;
+    p->nh = (nlat + 1) / 2;
+    p->max_fields = max_fields;
+    const int L = l_max, nh = p->nh;
+    std::vector<long long> offs(L + 2, 0);
+    for (int m = 0; m <= L; ++m) offs[m + 1] = offs[m] + (long long)nh * (L - m + 1);
+    std::vector<double2> tw(nlon);
+    for (int t = 0; t < nlon; ++t) {
+        const double ang = 2.0 * M_PI * (double)t / (double)nlon;
+        tw[t] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    std::vector<double> xn(x_nodes, x_nodes + nh);
+    double* d_x = nullptr;
+    int rc = AEO_OK;
+    const size_t plane = (size_t)nh * (L + 1);
+    const int nwork = ((std::max(max_fields, 12) + 3) / 4) * 4;
+    do {
+        if ((rc = upload(&p->d_offs, offs.data(), offs.size()))) break;
+        if ((rc = dev_alloc(&p->d_Ps, (size_t)offs[L + 1]))) break;
+        if ((rc = dev_alloc(&p->d_Pa, (size_t)offs[L + 1]))) break;
+        if ((rc = upload(&p->d_w_rows, w_rows, (size_t)nlat))) break;
+        if ((rc = upload(&p->d_coslat, coslat_rows, (size_t)nlat))) break;
+        if ((rc = upload(&p->d_tw, tw.data(), tw.size()))) break;
+        if ((rc = dev_alloc(&p->d_E, plane * nwork))) break;
+        if ((rc = dev_alloc(&p->d_O, plane * nwork))) break;
+        if ((rc = dev_alloc(&p->d_S, plane * nwork))) break;
+        if ((rc = dev_alloc(&p->d_D, plane * nwork))) break;
+        if ((rc = upload(&d_x, xn.data(), xn.size()))) break;
+        if ((rc = common_tables(p))) break;
+        p->fft.n = nlon;
+        p->fft.nstages = nst;
+        for (int i = 0; i < nst; ++i) p->fft.radix[i] = radices[i];
+        p->fft.tw = p->d_tw;
+        if ((rc = gauss_build_tables(p, d_x, nullptr))) break;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            set_error(std::string("aeo_plan_create_gauss: ") + hipGetErrorString(e));
+            rc = AEO_ERR_HIP;
+        }
+    } while (0);
+    dev_free(d_x);
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return AEO_OK;
+}
+
+int aeo_plan_create_points(int npoints, int l_max, const double* lat, const double* lon,
+                           const double* weights, const double* coslat, int max_fields,
+                           aeo_plan_t* out) {
+    AEO_REQUIRE(out != nullptr, AEO_ERR_INVALID, "aeo_plan_create_points: out is NULL");
+    *out = nullptr;
+    AEO_REQUIRE(lat && lon && coslat, AEO_ERR_INVALID,
+                "aeo_plan_create_points: lat, lon and coslat must be non-NULL");
+    AEO_REQUIRE(npoints >= 1 && l_max >= 0 && max_fields >= 1, AEO_ERR_INVALID,
+                "aeo_plan_create_points: needs npoints >= 1, l_max >= 0, max_fields >= 1");
+    aeo_plan_s* p = new (std::nothrow) aeo_plan_s();
+    AEO_REQUIRE(p != nullptr, AEO_ERR_HIP, "aeo_plan_create_points: out of host memory");
+    p->kind = kPoints;
+    p->L = l_max;
+    p->B = n_basis(l_max);
+    p->npts = npoints;
+    p->max_fields = max_fields;
+    p->nchunks = (npoints + 1023) / 1024;
+    std::vector<double> w(npoints);
+    for (int i = 0; i < npoints; ++i) w[i] = weights ? weights[i] : 4.0 * M_PI / npoints;
+    double *d_lat = nullptr, *d_lon = nullptr;
+    const int nwork = std::max(max_fields, 4);
+    int rc = AEO_OK;
+    do {
+        if ((rc = upload(&d_lat, lat, (size_t)npoints))) break;
+        if ((rc = upload(&d_lon, lon, (size_t)npoints))) break;
+        if ((rc = upload(&p->d_w, w.data(), w.size()))) break;
+        if ((rc = upload(&p->d_pcoslat, coslat, (size_t)npoints))) break;
+        if ((rc = dev_alloc(&p->d_Yk, (size_t)2 * p->B * npoints))) break;
+        if ((rc = dev_alloc(&p->d_Yp, (size_t)p->B * npoints))) break;
+        if ((rc = dev_alloc(&p->d_gw, (size_t)nwork * npoints))) break;
+        if ((rc = dev_alloc(&p->d_part, (size_t)p->nchunks * nwork * p->B))) break;
+        if ((rc = common_tables(p))) break;
+        if ((rc = points_build_tables(p, d_lat, d_lon, nullptr))) break;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            set_error(std::string("aeo_plan_create_points: ") + hipGetErrorString(e));
+            rc = AEO_ERR_HIP;
+        }
+    } while (0);
+    dev_free(d_lat);
+    dev_free(d_lon);
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return AEO_OK;
+}
+
+int aeo_plan_destroy(aeo_plan_t plan) {
+    free_plan(plan);
+    return AEO_OK;
+}
+
+int aeo_plan_info(aeo_plan_t plan, int* l_max, int* npoints, int* nbasis) {
+    AEO_REQUIRE(plan != nullptr, AEO_ERR_INVALID, "aeo_plan_info: plan is NULL");
+    if (l_max) *l_max = plan->L;
+    if (npoints) *npoints = plan->kind == kGauss ? plan->nlat * plan->nlon : plan->npts;
+    if (nbasis) *nbasis = plan->B;
+    return AEO_OK;
+}
+
+int aeo_plan_basis(aeo_plan_t plan, double* Y_out, void* stream) {
+    AEO_REQUIRE(plan && Y_out, AEO_ERR_INVALID, "aeo_plan_basis: NULL argument");
+    AEO_REQUIRE(plan->kind == kPoints, AEO_ERR_UNSUPPORTED,
+                "aeo_plan_basis: Gauss plans do not materialise the dense basis");
+    AEO_CHECK_HIP(hipMemcpyAsync(Y_out, plan->d_Yp, sizeof(double2) * plan->B * plan->npts,
+                                 hipMemcpyDeviceToDevice, as_stream(stream)));
+    return AEO_OK;
+}
+
+int aeo_analysis(aeo_plan_t plan, const double* f, int nfields, double* a_out, void* stream) {
+    AEO_REQUIRE(plan && f && a_out, AEO_ERR_INVALID, "aeo_analysis: NULL argument");
+    AEO_REQUIRE(nfields >= 1, AEO_ERR_INVALID, "aeo_analysis: nfields must be >= 1");
+    if (plan->kind == kGauss)
+        return gauss_analysis(plan, f, nfields, reinterpret_cast<double2*>(a_out),
+                              as_stream(stream));
+    return points_analysis(plan, f, nfields, reinterpret_cast<double2*>(a_out), as_stream(stream));
+}
+
+int aeo_synthesis(aeo_plan_t plan, const double* a, int nfields, double* f_out, void* stream) {
+    AEO_REQUIRE(plan && a && f_out, AEO_ERR_INVALID, "aeo_synthesis: NULL argument");
+    AEO_REQUIRE(nfields >= 1, AEO_ERR_INVALID, "aeo_synthesis: nfields must be >= 1");
+    if (plan->kind == kGauss)
+        return gauss_synthesis(plan, reinterpret_cast<const double2*>(a), nfields, f_out,
+                               as_stream(stream));
+    return points_synthesis(plan, reinterpret_cast<const double2*>(a), nfields, f_out,
+                            as_stream(stream));
+}
+
+int aeo_spec_op(int op, int l_max, double radius, const double* in, double* out, int nfields,
+                void* stream) {
+    AEO_REQUIRE(in && out, AEO_ERR_INVALID, "aeo_spec_op: NULL argument");
+    AEO_REQUIRE(l_max >= 0 && nfields >= 1, AEO_ERR_INVALID, "aeo_spec_op: bad sizes");
+    AEO_REQUIRE(op >= AEO_OP_LAP && op <= AEO_OP_INVLAP_ZERO, AEO_ERR_UNSUPPORTED,
+                "aeo_spec_op: unknown operator code");
+    AEO_REQUIRE(radius > 0.0 && std::isfinite(radius), AEO_ERR_INVALID,
+                "aeo_spec_op: radius must be finite and > 0");
+    AEO_REQUIRE(!((op == AEO_OP_SINTH_DTH || op == AEO_OP_SINTH_DTH_T) && in == out),
+                AEO_ERR_INVALID, "aeo_spec_op: the l+-1 coupling cannot run in place");
+    return launch_spec_op(op, l_max, radius, reinterpret_cast<const double2*>(in),
+                          reinterpret_cast<double2*>(out), nfields, as_stream(stream));
+}
+
+int aeo_jacobian(aeo_plan_t prod, double radius, const double* a, const double* b, double* out,
+                 int dealias, void* stream) {
+    AEO_REQUIRE(prod && a && b && out, AEO_ERR_INVALID, "aeo_jacobian: NULL argument");
+    AEO_REQUIRE(radius > 0.0, AEO_ERR_INVALID, "aeo_jacobian: radius must be > 0");
+    auto A = reinterpret_cast<const double2*>(a);
+    auto Bp = reinterpret_cast<const double2*>(b);
+    auto O = reinterpret_cast<double2*>(out);
+    if (prod->kind == kGauss) return gauss_jacobian(prod, radius, A, Bp, O, dealias, as_stream(stream));
+    return points_jacobian(prod, radius, A, Bp, O, dealias, as_stream(stream));
+}
+
+static int velocity_common(aeo_plan_t plan, double radius, const double* q, double* o1,
+                           double* o2, int mode, void* stream) {
+    AEO_REQUIRE(plan && q && o1 && o2, AEO_ERR_INVALID, "velocity/gradient: NULL argument");
+    AEO_REQUIRE(radius > 0.0, AEO_ERR_INVALID, "velocity/gradient: radius must be > 0");
+    hipStream_t s = as_stream(stream);
+    const int L = plan->L;
+    const size_t fs = (size_t)(L + 1) * (L + 1);
+    // spec[0] = d/dlambda q, spec[1] = S q; the synthesised fields land in
+    // o1 (S q) and o2 (q_lambda) before the pointwise stage rewrites them.
+    double2* spec = plan->d_spec;
+    int rc = launch_spec_op(AEO_OP_DLAMBDA, L, radius, reinterpret_cast<const double2*>(q), spec,
+                            1, s);
+    if (rc) return rc;
+    rc = launch_spec_op(AEO_OP_SINTH_DTH, L, radius, reinterpret_cast<const double2*>(q),
+                        spec + fs, 1, s);
+    if (rc) return rc;
+    const int npts = plan->kind == kGauss ? plan->nlat * plan->nlon : plan->npts;
+    rc = aeo_synthesis(plan, reinterpret_cast<const double*>(spec + fs), 1, o1, stream);
+    if (rc) return rc;
+    rc = aeo_synthesis(plan, reinterpret_cast<const double*>(spec), 1, o2, stream);
+    if (rc) return rc;
+    const double* c = plan->kind == kGauss ? plan->d_coslat : plan->d_pcoslat;
+    const int row_len = plan->kind == kGauss ? plan->nlon : 0;
+    return launch_velocity_points(npts, row_len, radius, c, o1, o2, o1, o2, mode, s);
+}
+
+int aeo_velocity(aeo_plan_t plan, double radius, const double* psi, double* u, double* v,
+                 void* stream) {
+    return velocity_common(plan, radius, psi, u, v, 0, stream);
+}
+
+int aeo_gradient(aeo_plan_t plan, double radius, const double* q, double* dqdx, double* dqdy,
+                 void* stream) {
+    // velocity_common writes (S q, q_lambda) into (o1, o2) and mode 1 maps
+    // them to (q_lambda / c, -S q / (R c)).
+    return velocity_common(plan, radius, q, dqdx, dqdy, 1, stream);
+}
+
+static int model_common(aeo_plan_t prod, double radius, aeo_model_s* m) {
+    const int L = prod->L;
+    m->plan = prod;
+    m->L = L;
+    m->R = radius;
+    std::vector<double> lap(L + 1), nulap(L + 1), nu4lap2(L + 1);
+    for (int l = 0; l <= L; ++l) {
+        lap[l] = -(double)l * (l + 1.0) / (radius * radius);
+        const double le = (l == 0) ? 1.0 / (radius * radius) : lap[l];
+        nulap[l] = m->nu * le;
+        nu4lap2[l] = m->nu4 * (lap[l] * lap[l]);
+    }
+    int rc;
+    if ((rc = upload(&m->d_lap, lap.data(), lap.size()))) return rc;
+    if ((rc = upload(&m->d_nulap, nulap.data(), nulap.size()))) return rc;
+    if ((rc = upload(&m->d_nu4lap2, nu4lap2.data(), nu4lap2.size()))) return rc;
+    const int nsyn = (m->kind == kBVE) ? 4 : 12;
+    const int nana = (m->kind == kBVE) ? 1 : 4;
+    const size_t fs = (size_t)(L + 1) * (L + 1);
+    if (prod->kind == kGauss) {
+        const size_t plane = (size_t)prod->nh * (L + 1);
+        if ((rc = dev_alloc(&m->d_E, plane * nsyn))) return rc;
+        if ((rc = dev_alloc(&m->d_O, plane * nsyn))) return rc;
+        if ((rc = dev_alloc(&m->d_S, plane * nana))) return rc;
+        if ((rc = dev_alloc(&m->d_D, plane * nana))) return rc;
+    } else {
+        if ((rc = dev_alloc(&m->d_gw, (size_t)prod->npts * nana))) return rc;
+        if ((rc = dev_alloc(&m->d_part, (size_t)prod->nchunks * nana * prod->B))) return rc;
+    }
+    if ((rc = dev_alloc(&m->d_ystage, fs * m->nstate))) return rc;
+    if ((rc = dev_alloc(&m->d_ksum, fs * m->nstate))) return rc;
+    return AEO_OK;
+}
+
+int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosity,
+                   aeo_model_t* out) {
+    AEO_REQUIRE(out != nullptr, AEO_ERR_INVALID, "aeo_bve_create: out is NULL");
+    *out = nullptr;
+    AEO_REQUIRE(prod != nullptr, AEO_ERR_INVALID, "aeo_bve_create: plan is NULL");
+    AEO_REQUIRE(radius > 0.0 && std::isfinite(radius), AEO_ERR_INVALID,
+                "aeo_bve_create: radius must be finite and > 0");
+    AEO_REQUIRE(std::isfinite(omega) && std::isfinite(viscosity), AEO_ERR_INVALID,
+                "aeo_bve_create: omega and viscosity must be finite");
+    aeo_model_s* m = new (std::nothrow) aeo_model_s();
+    AEO_REQUIRE(m != nullptr, AEO_ERR_HIP, "aeo_bve_create: out of host memory");
+    m->kind = kBVE;
+    m->nstate = 1;
+    m->omega = omega;
+    m->nu = viscosity;
+    int rc = model_common(prod, radius, m);
+    if (rc) {
+        free_model(m);
+        return rc;
+    }
+    *out = m;
+    return AEO_OK;
+}
+
+int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
+                   double mean_depth, double hyperdiffusion_nu4, const double* phi_s_lm,
+                   aeo_model_t* out) {
+    AEO_REQUIRE(out != nullptr, AEO_ERR_INVALID, "aeo_swe_create: out is NULL");
+    *out = nullptr;
+    AEO_REQUIRE(prod != nullptr, AEO_ERR_INVALID, "aeo_swe_create: plan is NULL");
+    AEO_REQUIRE(std::isfinite(gravity) && gravity > 0, AEO_ERR_INVALID,
+                "gravity must be finite and > 0");
+    AEO_REQUIRE(std::isfinite(mean_depth) && mean_depth > 0, AEO_ERR_INVALID,
+                "mean_depth must be finite and > 0");
+    AEO_REQUIRE(std::isfinite(hyperdiffusion_nu4) && hyperdiffusion_nu4 >= 0, AEO_ERR_INVALID,
+                "hyperdiffusion_nu4 must be finite and >= 0");
+    AEO_REQUIRE(radius > 0.0 && std::isfinite(radius) && std::isfinite(omega), AEO_ERR_INVALID,
+                "aeo_swe_create: radius must be finite and > 0, omega finite");
+    aeo_model_s* m = new (std::nothrow) aeo_model_s();
+    AEO_REQUIRE(m != nullptr, AEO_ERR_HIP, "aeo_swe_create: out of host memory");
+    m->kind = kSWE;
+    m->nstate = 3;
+    m->omega = omega;
+    m->gravity = gravity;
+    m->mean_depth = mean_depth;
+    m->phi0 = gravity * mean_depth;
+    m->nu4 = hyperdiffusion_nu4;
+    int rc = model_common(prod, radius, m);
+    if (!rc && phi_s_lm) {
+        // lap * phi_s and nu4 * lap^2 * (phi_s with the (0,0) entry zeroed)
+        // (phy/shallow_water.py:240-253), formed once on the host.
+        const int L = prod->L;
+        const size_t fs = (size_t)(L + 1) * (L + 1);
+        std::vector<double2> ps(fs), lps(fs), nps(fs);
+        if (hipMemcpy(ps.data(), phi_s_lm, fs * sizeof(double2), hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+            set_error("aeo_swe_create: cannot read phi_s_lm");
+            rc = AEO_ERR_HIP;
+        } else {
+            bool any = false;
+            for (size_t i = 0; i < fs; ++i) any = any || ps[i].x != 0.0 || ps[i].y != 0.0;
+            m->has_topo = any;
+            for (int l = 0; l <= L; ++l) {
+                const double lap = -(double)l * (l + 1.0) / (radius * radius);
+                for (int mm = 0; mm <= L; ++mm) {
+                    const size_t i = (size_t)l * (L + 1) + mm;
+                    lps[i] = make_double2(lap * ps[i].x, lap * ps[i].y);
+                    const double2 an = (i == 0) ? make_double2(0.0, 0.0) : ps[i];
+                    const double q = hyperdiffusion_nu4 * (lap * lap);
+                    nps[i] = make_double2(q * an.x, q * an.y);
+                }
+            }
+            if (any) {
+                rc = upload(&m->d_lap_phi_s, lps.data(), fs);
+                if (!rc) rc = upload(&m->d_nu4_lap2_phi_s, nps.data(), fs);
+            }
+        }
+    }
+    if (rc) {
+        free_model(m);
+        return rc;
+    }
+    *out = m;
+    return AEO_OK;
+}
+
+int aeo_model_destroy(aeo_model_t model) {
+    free_model(model);
+    return AEO_OK;
+}
+
+static int run_stage(aeo_model_s* m, const double2* ys, const double2* y0, const double2* forcing,
+                     double2* kout, int stage, double dt, double2* ynext, hipStream_t s) {
+    if (m->plan->kind == kGauss)
+        return gauss_model_stage(m, ys, y0, forcing, kout, stage, dt, ynext, s);
+    return points_model_stage(m, ys, y0, forcing, kout, stage, dt, ynext, s);
+}
+
+int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing, double* out,
+                       void* stream) {
+    AEO_REQUIRE(model && y && out, AEO_ERR_INVALID, "aeo_model_tendency: NULL argument");
+    AEO_REQUIRE(!(forcing && model->kind == kSWE), AEO_ERR_UNSUPPORTED,
+                "aeo_model_tendency: forcing is only defined for the BVE model");
+    auto Y = reinterpret_cast<const double2*>(y);
+    return run_stage(model, Y, Y, reinterpret_cast<const double2*>(forcing),
+                     reinterpret_cast<double2*>(out), -1, 0.0, nullptr, as_stream(stream));
+}
+
+int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
+                  const double* forcing, double* y_out, void* stream) {
+    AEO_REQUIRE(model && y && y_out, AEO_ERR_INVALID, "aeo_model_rk4: NULL argument");
+    AEO_REQUIRE(nsteps >= 0, AEO_ERR_INVALID, "aeo_model_rk4: nsteps must be >= 0");
+    AEO_REQUIRE(std::isfinite(dt), AEO_ERR_INVALID, "aeo_model_rk4: dt must be finite");
+    AEO_REQUIRE(!(forcing && model->kind == kSWE), AEO_ERR_UNSUPPORTED,
+                "aeo_model_rk4: forcing is only defined for the BVE model");
+    hipStream_t s = as_stream(stream);
+    const size_t fs = (size_t)(model->L + 1) * (model->L + 1) * model->nstate;
+    auto F = reinterpret_cast<const double2*>(forcing);
+    auto Yout = reinterpret_cast<double2*>(y_out);
+    const double2* cur = reinterpret_cast<const double2*>(y);
+    if (nsteps == 0) {
+        if (y != y_out)
+            AEO_CHECK_HIP(hipMemcpyAsync(y_out, y, fs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        return AEO_OK;
+    }
+    for (int step = 0; step < nsteps; ++step) {
+        for (int st = 0; st < 4; ++st) {
+            const double2* ys = (st == 0) ? cur : model->d_ystage;
+            double2* ynext = (st == 3) ? Yout : model->d_ystage;
+            int rc = run_stage(model, ys, cur, F, nullptr, st, dt, ynext, s);
+            if (rc) return rc;
+        }
+        cur = Yout;
+    }
+    return AEO_OK;
+}
+
+int aeo_spectral_diag(int l_max, double radius, double omega, const double* zeta, int nfields,
+                      double* out, void* stream) {
+    AEO_REQUIRE(zeta && out, AEO_ERR_INVALID, "aeo_spectral_diag: NULL argument");
+    AEO_REQUIRE(l_max >= 0 && nfields >= 1, AEO_ERR_INVALID, "aeo_spectral_diag: bad sizes");
+    return launch_spectral_diag(l_max, radius, omega, reinterpret_cast<const double2*>(zeta),
+                                nfields, out, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Shared definitions for the aeolus_hip library (gfx950 / CDNA4).
+//
+// Complex numbers are stored as interleaved FP64 pairs (double2: x = real,
+// y = imaginary), which is the memory layout of a complex128 tensor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/aeolus_hip.h"
+
+namespace aeo {
+
+// ---------------------------------------------------------------------------
+// Error handling: every C entry point returns an int status and leaves a
+// thread-local message readable through aeo_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* get_error();
+
+struct Status {
+    int code;
+    std::string msg;
+};
+
+#define AEO_CHECK_HIP(expr)                                                    \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::aeo::set_error(std::string("HIP error at " __FILE__ ":") +       \
+                             std::to_string(__LINE__) + ": " +                 \
+                             hipGetErrorString(e_) + " (" #expr ")");          \
+            return AEO_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+#define AEO_REQUIRE(cond, code, msg)                                           \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::aeo::set_error(msg);                                             \
+            return (code);                                                     \
+        }                                                                      \
+    } while (0)
+
+#define AEO_LAUNCH_CHECK()                                                     \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess) {                                                \
+            ::aeo::set_error(std::string("kernel launch failed at " __FILE__  \
+                                         ":") +                                \
+                             std::to_string(__LINE__) + ": " +                 \
+                             hipGetErrorString(e_));                           \
+            return AEO_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device complex helpers.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double2 c2(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return c2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return c2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return c2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cscale(double2 a, double s) { return c2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cconj(double2 a) { return c2(a.x, -a.y); }
+
+inline int n_basis(int L) { return (L + 1) * (L + 2) / 2; }
+
+// Block size used by the grid (FFT) kernels and the maximum number of
+// complex values one thread keeps in registers across an in-place FFT stage.
+constexpr int kGridThreads = 256;
+constexpr int kMaxPerThread = 24;
+constexpr int kMaxRadices = 16;
+
+// Mixed-radix factorisation for the LDS FFT (radix 4 first, then 2, 3, 5,
+// then any remaining prime handled by a direct-DFT stage).
+int fft_factor(int n, int* radices);
+
+}  // namespace aeo

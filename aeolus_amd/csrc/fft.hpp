@@ -1,0 +1,192 @@
+// Batched in-place Stockham FFT over rows held in LDS (gfx950).
+//
+// A workgroup transforms `nrows` contiguous rows of length n (complex FP64)
+// in LDS.  Each stage of radix R reads the R inputs of every butterfly into
+// registers, synchronises, and writes the R outputs to their Stockham
+// positions, so no second LDS buffer is needed:
+//
+//   j in [0, n/R),  k = j mod Ns,
+//   v[r] = x[j + r n/R] * W^(r k n/(Ns R)),        (W = e^{dir 2 pi i / n})
+//   V    = R-point DFT of v,
+//   x[(j / Ns) Ns R + k + q Ns] = V[q],            Ns *= R after the stage.
+//
+// Radices 2, 3, 4 and 5 use hand-written butterflies; any other prime factor
+// uses a direct R-point DFT computed one output per thread.  Rows are
+// processed in chunks of at most kChunkElems complex values so each thread
+// keeps at most 8 complex values in registers per stage.
+#pragma once
+
+#include "common.hpp"
+
+namespace aeo {
+
+constexpr int kChunkElems = 2048;  // complex values per chunk (256 threads x 8)
+
+struct FftDev {
+    int n;
+    int nstages;
+    int radix[kMaxRadices];
+    const double2* tw;  // tw[t] = exp(-2 pi i t / n), t in [0, n)
+};
+
+template <int DIR>
+__device__ inline double2 twiddle(const double2* tw, int t) {
+    double2 w = tw[t];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+// Multiply by DIR * i.
+template <int DIR>
+__device__ inline double2 mul_i(double2 a) {
+    return DIR > 0 ? c2(-a.y, a.x) : c2(a.y, -a.x);
+}
+
+template <int DIR, int R>
+__device__ inline void butterfly(double2* v) {
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        double2 s13 = cadd(v[1], v[3]), d13 = mul_i<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const double h = 0.86602540378443864676;  // sin(2 pi / 3)
+        double2 s = cadd(v[1], v[2]);
+        double2 d = csub(v[1], v[2]);
+        double2 m = c2(v[0].x - 0.5 * s.x, v[0].y - 0.5 * s.y);
+        double2 t = mul_i<DIR>(cscale(d, h));
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, t);
+        v[2] = csub(m, t);
+    } else if constexpr (R == 5) {
+        const double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
+        const double c2_ = -0.80901699437494742410; // cos(4 pi / 5)
+        const double s1 = 0.95105651629515357212;   // sin(2 pi / 5)
+        const double s2 = 0.58778525229247312917;   // sin(4 pi / 5)
+        double2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        double2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        double2 x0 = v[0];
+        double2 r1 = c2(x0.x + c1 * a1.x + c2_ * a2.x, x0.y + c1 * a1.y + c2_ * a2.y);
+        double2 r2 = c2(x0.x + c2_ * a1.x + c1 * a2.x, x0.y + c2_ * a1.y + c1 * a2.y);
+        double2 i1 = mul_i<DIR>(c2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+        double2 i2 = mul_i<DIR>(c2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+        v[0] = cadd(x0, cadd(a1, a2));
+        v[1] = cadd(r1, i1);
+        v[4] = csub(r1, i1);
+        v[2] = cadd(r2, i2);
+        v[3] = csub(r2, i2);
+    }
+}
+
+// One radix-R stage over `nrows` rows of length n starting at buf.
+template <int DIR, int R>
+__device__ inline void fft_stage(double2* buf, int nrows, int n, int Ns, const double2* tw) {
+    constexpr int MAXB = (kChunkElems / R + kGridThreads - 1) / kGridThreads;
+    const int nb = n / R;
+    const int total = nrows * nb;
+    const int span = n / (Ns * R);
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < total) {
+            const int row = t / nb;
+            const int j = t - row * nb;
+            const int k = j % Ns;
+            const double2* rb = buf + row * n;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = rb[j + r * nb];
+                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
+                v[b][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < total) {
+            const int row = t / nb;
+            const int j = t - row * nb;
+            const int k = j % Ns;
+            butterfly<DIR, R>(v[b]);
+            double2* rb = buf + row * n;
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) rb[base + q * Ns] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Direct-DFT stage for a prime radix R > 5: one output per thread slot.
+template <int DIR>
+__device__ inline void fft_stage_generic(double2* buf, int nrows, int n, int Ns, int R,
+                                         const double2* tw) {
+    constexpr int MAXO = kChunkElems / kGridThreads;
+    const int total = nrows * n;
+    const int nb = n / R;
+    const int span = n / (Ns * R);
+    double2 out[MAXO];
+#pragma unroll
+    for (int b = 0; b < MAXO; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        double2 acc = c2(0.0, 0.0);
+        if (t < total) {
+            const int row = t / n;
+            const int p = t - row * n;           // output position
+            const int k = p % Ns;
+            const int q = (p / Ns) % R;
+            const int j = (p / (Ns * R)) * Ns + k;
+            const double2* rb = buf + row * n;
+            for (int r = 0; r < R; ++r) {
+                double2 x = rb[j + r * nb];
+                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
+                const int e = ((r * q) % R) * nb;  // W_R^{rq} = W_n^{(rq mod R) n/R}
+                acc = cadd(acc, e ? cmul(x, twiddle<DIR>(tw, e)) : x);
+            }
+        }
+        out[b] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXO; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < total) buf[t] = out[b];
+    }
+    __syncthreads();
+}
+
+// Transform `nrows` contiguous rows of length fp.n in LDS (in place).
+// DIR = -1: X_m = sum_k x_k e^{-2 pi i m k / n};  DIR = +1: unnormalised inverse.
+// Must be called by all threads of the (kGridThreads-wide) workgroup.
+template <int DIR>
+__device__ void fft_rows(double2* buf, int nrows, const FftDev& fp) {
+    const int n = fp.n;
+    const int cb = (n <= kChunkElems) ? (kChunkElems / n) : 1;
+    for (int r0 = 0; r0 < nrows; r0 += cb) {
+        const int nr = (nrows - r0 < cb) ? (nrows - r0) : cb;
+        double2* b = buf + r0 * n;
+        int Ns = 1;
+        for (int s = 0; s < fp.nstages; ++s) {
+            const int R = fp.radix[s];
+            switch (R) {
+                case 2: fft_stage<DIR, 2>(b, nr, n, Ns, fp.tw); break;
+                case 3: fft_stage<DIR, 3>(b, nr, n, Ns, fp.tw); break;
+                case 4: fft_stage<DIR, 4>(b, nr, n, Ns, fp.tw); break;
+                case 5: fft_stage<DIR, 5>(b, nr, n, Ns, fp.tw); break;
+                default: fft_stage_generic<DIR>(b, nr, n, Ns, R, fp.tw); break;
+            }
+            Ns *= R;
+        }
+    }
+}
+
+}  // namespace aeo

@@ -1,0 +1,137 @@
+// Host-side plan and model objects plus the device views passed to kernels.
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace aeo {
+
+enum PlanKind { kGauss = 0, kPoints = 1 };
+
+// Device view of a Gauss plan (passed by value to kernels).
+struct GaussDev {
+    int L, nlat, nlon, nh;
+    const double* Ps;         // synthesis table: block m, [l-m][jp]   (jp fastest)
+    const double* Pa;         // analysis table:  block m, [jp][l-m]   (l fastest)
+    const long long* offs;    // element offset of block m (L+2 entries)
+    const double* w_rows;     // nlat solid-angle weights per point of a row
+    const double* coslat;     // nlat cos(latitude), unclamped
+    FftDev fft;
+};
+
+// Device view of a point-set plan.
+struct PointsDev {
+    int L, B, npts;
+    const double* Yk;         // [2][B][npts]: real plane then imaginary plane
+    const double2* Yp;        // [npts][B]
+    const double* w;          // npts quadrature weights
+    const double* coslat;     // npts cos(latitude), unclamped
+};
+
+}  // namespace aeo
+
+struct aeo_plan_s {
+    int kind = aeo::kGauss;
+    int L = 0, B = 0, max_fields = 0;
+    // Gauss
+    int nlat = 0, nlon = 0, nh = 0;
+    double* d_Ps = nullptr;
+    double* d_Pa = nullptr;
+    long long* d_offs = nullptr;
+    double* d_w_rows = nullptr;
+    double* d_coslat = nullptr;
+    double2* d_tw = nullptr;
+    aeo::FftDev fft{};
+    // Gauss workspaces: [fields][nh][L+1] complex
+    double2* d_E = nullptr;
+    double2* d_O = nullptr;
+    double2* d_S = nullptr;
+    double2* d_D = nullptr;
+    // Points
+    int npts = 0;
+    double* d_Yk = nullptr;
+    double2* d_Yp = nullptr;
+    double* d_w = nullptr;
+    double* d_pcoslat = nullptr;
+    int nchunks = 0;
+    double* d_gw = nullptr;        // [fields][npts]
+    double2* d_part = nullptr;     // [nchunks][fields][B]
+    // Spectral constants ((L+1)^2 real each; lap has L+1 entries)
+    double* d_Cp = nullptr;
+    double* d_Cm = nullptr;
+    // generic scratch for composed operations: [fields][(L+1)^2] complex
+    double2* d_spec = nullptr;
+
+    aeo::GaussDev gdev() const {
+        aeo::GaussDev g;
+        g.L = L; g.nlat = nlat; g.nlon = nlon; g.nh = nh;
+        g.Ps = d_Ps; g.Pa = d_Pa; g.offs = d_offs;
+        g.w_rows = d_w_rows; g.coslat = d_coslat; g.fft = fft;
+        return g;
+    }
+    aeo::PointsDev pdev() const {
+        aeo::PointsDev p;
+        p.L = L; p.B = B; p.npts = npts;
+        p.Yk = d_Yk; p.Yp = d_Yp; p.w = d_w; p.coslat = d_pcoslat;
+        return p;
+    }
+};
+
+enum ModelKind { kBVE = 0, kSWE = 1 };
+
+struct aeo_model_s {
+    int kind = kBVE;
+    aeo_plan_s* plan = nullptr;
+    int L = 0;
+    int nstate = 1;                 // fields in the state (1 BVE, 3 SWE)
+    double R = 1.0, omega = 0.0;
+    double nu = 0.0;                // BVE viscosity
+    double gravity = 0.0, mean_depth = 0.0, phi0 = 0.0, nu4 = 0.0;
+    bool has_topo = false;
+    // constants
+    double* d_nulap = nullptr;      // BVE: nu * laplacian_eig (l = 0 entry nu / R^2)
+    double* d_lap = nullptr;        // -l(l+1)/R^2 (l = 0 entry 0)
+    double* d_nu4lap2 = nullptr;    // SWE: nu4 * lap^2
+    double2* d_lap_phi_s = nullptr; // SWE topography: lap * phi_s
+    double2* d_nu4_lap2_phi_s = nullptr;
+    // workspaces
+    double2* d_E = nullptr;
+    double2* d_O = nullptr;
+    double2* d_S = nullptr;
+    double2* d_D = nullptr;
+    double* d_gw = nullptr;
+    double2* d_part = nullptr;
+    double2* d_ystage = nullptr;
+    double2* d_ksum = nullptr;
+};
+
+namespace aeo {
+
+// Launchers implemented in gauss.hip / points.hip / spectral.hip.
+int gauss_build_tables(aeo_plan_s* p, const double* h_x, hipStream_t s);
+int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
+int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
+int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,
+                   int dealias, hipStream_t s);
+int gauss_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
+                      const double2* forcing, double2* kout, int stage, double dt,
+                      double2* ynext, hipStream_t s);
+
+int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s);
+int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
+int points_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
+int points_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,
+                    int dealias, hipStream_t s);
+int points_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
+                       const double2* forcing, double2* kout, int stage, double dt,
+                       double2* ynext, hipStream_t s);
+
+int launch_spec_op(int op, int L, double R, const double2* in, double2* out, int nf,
+                   hipStream_t s);
+int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf, double* out,
+                         hipStream_t s);
+int launch_velocity_points(int n, int row_len, double R, const double* coslat,
+                           const double* g_s, const double* g_lam, double* u, double* v, int mode,
+                           hipStream_t s);
+
+}  // namespace aeo

@@ -1,0 +1,315 @@
+// Point-set (geodesic) back end: dense FP64 basis built on the device by the
+// stable normalised recurrence, synthesis as a dense contraction with fused
+// pointwise products, analysis as a chunked reduction over points.
+//
+// Reference: PointSetSphericalHarmonics (num/fast_geodesic_sh.py:15-107):
+//   analysis   a_k = sum_i conj(Y_ik) (w_i f_i)
+//   synthesis  f_i = 2 Re(sum_k Y_ik a_k) - Re(sum_{m=0} Y_ik a_k)
+//                  = sum_k c_k (Re Y_ik Re a_k - Im Y_ik Im a_k),  c = 1 (m = 0), 2 (m > 0)
+#include "plan.hpp"
+#include "stage.hpp"
+
+namespace aeo {
+
+constexpr int kPtsThreads = 256;
+constexpr int kPtsChunk = 1024;  // points per analysis partial sum
+
+__device__ inline void k_to_lm(int k, int& l, int& m) {
+    int t = (int)((sqrt(8.0 * k + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= k) ++t;
+    while (t * (t + 1) / 2 > k) --t;
+    l = t;
+    m = k - t * (t + 1) / 2;
+}
+
+// One thread per point: x = cos(pi/2 - lat) (cu/sh_matrix.cu:25-27), stable
+// normalised recurrence, phase (cos m lon, sin m lon) (cu/sh_matrix.cu:62-63).
+__global__ void points_basis_kernel(int npts, int L, const double* __restrict__ lat,
+                                    const double* __restrict__ lon, double* __restrict__ Yk,
+                                    double2* __restrict__ Yp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const int B = (L + 1) * (L + 2) / 2;
+    const double x = cos(M_PI / 2.0 - lat[i]);
+    const double s = sqrt((1.0 - x) * (1.0 + x));
+    const double lo = lon[i];
+    double pmm = 0.28209479177387814347;
+    for (int m = 0; m <= L; ++m) {
+        if (m > 0) pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * s;
+        double sn, cs;
+        sincos(m * lo, &sn, &cs);
+        double p2 = 0.0, p1 = 0.0;
+        for (int l = m; l <= L; ++l) {
+            double p;
+            if (l == m) {
+                p = pmm;
+            } else if (l == m + 1) {
+                p = sqrt(2.0 * m + 3.0) * x * pmm;
+            } else {
+                const double ll = (double)l * l, mm = (double)m * m;
+                const double a = sqrt((4.0 * ll - 1.0) / (ll - mm));
+                const double b =
+                    sqrt(((l - 1.0) * (l - 1.0) - mm) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                p = a * (x * p1 - b * p2);
+            }
+            p2 = p1;
+            p1 = p;
+            const int k = l * (l + 1) / 2 + m;
+            const double yr = p * cs, yi = p * sn;
+            Yk[(size_t)k * npts + i] = yr;
+            Yk[(size_t)(B + k) * npts + i] = yi;
+            Yp[(size_t)i * B + k] = c2(yr, yi);
+        }
+    }
+}
+
+// Pointwise stages applied to the synthesised values at one point.
+struct PwStore {  // generic synthesis
+    double* out;
+    int npts;
+    int nf;
+    template <int NF>
+    __device__ void apply(int i, const double* v, double, double) const {
+        for (int f = 0; f < nf && f < NF; ++f) out[(size_t)f * npts + i] = v[f];
+    }
+};
+
+struct PwJac {  // v = (a_l, S a, b_l, S b) -> w J, with a_s = Syn(S a) / R
+    double* gw;
+    double R;
+    template <int NF>
+    __device__ void apply(int i, const double* v, double w, double c) const {
+        const double as = v[1] / R, bs = v[3] / R;
+        gw[i] = w * jac_point(v[0], as, v[2], bs, clamp_coslat(c));
+    }
+};
+
+struct PwSwe {  // 12 synthesised fields -> w * (zeta_dot, curl, kinetic, phi_dot)
+    double* gw;
+    int npts;
+    double R;
+    template <int NF>
+    __device__ void apply(int i, const double* v, double w, double c) const {
+        double g[12], o[4];
+#pragma unroll
+        for (int r = 0; r < 12; ++r) g[r] = v[r];
+        g[1] /= R;
+        g[3] /= R;
+        g[5] /= R;
+        g[8] /= R;
+        swe_point(g, clamp_coslat(c), o);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gw[(size_t)r * npts + i] = w * o[r];
+    }
+};
+
+template <class Loader, class Pw>
+__global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld, Pw pw) {
+    constexpr int NF = Loader::NF;
+    extern __shared__ double2 lds[];  // [NF][B] coefficient values times c_k
+    const int B = p.B;
+    for (int t = threadIdx.x; t < NF * B; t += kPtsThreads) {
+        const int f = t / B, k = t - f * B;
+        int l, m;
+        k_to_lm(k, l, m);
+        const double2 v = ld.value(f, l, m);
+        lds[t] = (m == 0) ? v : cscale(v, 2.0);
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kPtsThreads + threadIdx.x;
+    if (i >= p.npts) return;
+    double acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+    const double* yr = p.Yk + i;
+    const double* yi = p.Yk + (size_t)B * p.npts + i;
+    for (int k = 0; k < B; ++k) {
+        const double a = yr[(size_t)k * p.npts];
+        const double b = yi[(size_t)k * p.npts];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const double2 c = lds[f * B + k];
+            acc[f] = fma(a, c.x, acc[f]);
+            acc[f] = fma(-b, c.y, acc[f]);
+        }
+    }
+    pw.template apply<NF>(i, acc, p.w[i], p.coslat[i]);
+}
+
+// Partial analysis sums: part[chunk][f][k] = sum_{i in chunk} conj(Y_ik) g_i,
+// g = values * w (weights != nullptr) or values already weighted.
+// grid (ceil(B/64), nchunks, nf); block 256 = 64 k-lanes x 4 point groups.
+__global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
+    PointsDev p, const double* __restrict__ vals, const double* __restrict__ weights,
+    int nf, double2* __restrict__ part) {
+    __shared__ double2 red[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int chunk = blockIdx.y, f = blockIdx.z;
+    const int i0 = chunk * kPtsChunk;
+    const int i1 = (i0 + kPtsChunk < p.npts) ? i0 + kPtsChunk : p.npts;
+    const double* v = vals + (size_t)f * p.npts;
+    double2 acc = c2(0.0, 0.0);
+    if (k < p.B) {
+        for (int i = i0 + q; i < i1; i += 4) {
+            const double g = weights ? v[i] * weights[i] : v[i];
+            const double2 y = p.Yp[(size_t)i * p.B + k];
+            acc.x = fma(y.x, g, acc.x);
+            acc.y = fma(-y.y, g, acc.y);
+        }
+    }
+    red[q][lane] = acc;
+    __syncthreads();
+    if (q == 0 && k < p.B) {
+        double2 s = red[0][lane];
+        s = cadd(s, red[1][lane]);
+        s = cadd(s, red[2][lane]);
+        s = cadd(s, red[3][lane]);
+        part[((size_t)chunk * nf + f) * p.B + k] = s;
+    }
+}
+
+// Sum the partials (fixed chunk order) and run the epilogue for every (l, m).
+template <class Epi>
+__global__ void points_finish_kernel(int L, int B, int nchunks, int nf_part, int nf_valid,
+                                     const double2* __restrict__ part, int lmax_a, int mmax_a,
+                                     Epi ep) {
+    constexpr int NF = Epi::NF;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = L + 1;
+    if (t >= n * n) return;
+    const int l = t / n, m = t - l * n;
+    double2 acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = c2(0.0, 0.0);
+    if (m <= l && l <= lmax_a && m <= mmax_a) {
+        const int k = l * (l + 1) / 2 + m;
+        for (int c = 0; c < nchunks; ++c) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                if (f < nf_valid) acc[f] = cadd(acc[f], part[((size_t)c * nf_part + f) * B + k]);
+            }
+        }
+    }
+    ep.apply(l, m, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <class Loader, class Pw>
+static int launch_psyn(const aeo_plan_s* p, const Loader& ld, const Pw& pw, hipStream_t s) {
+    const size_t lds = sizeof(double2) * Loader::NF * p->B;
+    dim3 grid((p->npts + kPtsThreads - 1) / kPtsThreads);
+    if (lds > 65536) {
+        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)points_syn_kernel<Loader, Pw>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL((points_syn_kernel<Loader, Pw>), grid, dim3(kPtsThreads), lds, s,
+                       p->pdev(), ld, pw);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+static int launch_partial(const aeo_plan_s* p, const double* vals, const double* w, int nf,
+                          double2* part, hipStream_t s) {
+    dim3 grid((p->B + 63) / 64, p->nchunks, nf);
+    hipLaunchKernelGGL(points_partial_kernel, grid, dim3(kPtsThreads), 0, s, p->pdev(), vals, w,
+                       nf, part);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+template <class Epi>
+static int launch_finish(const aeo_plan_s* p, int nf_part, int nf_valid, const double2* part,
+                         int lmax_a, int mmax_a, const Epi& ep, hipStream_t s) {
+    const int n = (p->L + 1) * (p->L + 1);
+    hipLaunchKernelGGL(points_finish_kernel<Epi>, dim3((n + 255) / 256), dim3(256), 0, s, p->L,
+                       p->B, p->nchunks, nf_part, nf_valid, part, lmax_a, mmax_a, ep);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s) {
+    hipLaunchKernelGGL(points_basis_kernel, dim3((p->npts + 255) / 256), dim3(256), 0, s,
+                       p->npts, p->L, d_lat, d_lon, p->d_Yk, p->d_Yp);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+int points_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
+    const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
+    for (int g0 = 0; g0 < nf; g0 += RawLoader::NF) {
+        const int nb = (nf - g0 < RawLoader::NF) ? nf - g0 : RawLoader::NF;
+        RawLoader ld{a + g0 * fs, p->L, nb};
+        PwStore pw{f + (size_t)g0 * p->npts, p->npts, nb};
+        int rc = launch_psyn(p, ld, pw, s);
+        if (rc) return rc;
+    }
+    return AEO_OK;
+}
+
+int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s) {
+    const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
+    for (int f0 = 0; f0 < nf; f0 += p->max_fields) {
+        const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
+        int rc = launch_partial(p, f + (size_t)f0 * p->npts, p->d_w, nb, p->d_part, s);
+        if (rc) return rc;
+        for (int g0 = 0; g0 < nb; g0 += EpiStore::NF) {
+            const int ng = (nb - g0 < EpiStore::NF) ? nb - g0 : EpiStore::NF;
+            EpiStore ep{a + (size_t)(f0 + g0) * fs, p->L, ng};
+            // partials of this group start at field g0: view them as an nb-field array
+            // offset by g0 * B within each chunk.
+            rc = launch_finish(p, nb, ng, p->d_part + (size_t)g0 * p->B, p->L, p->L, ep, s);
+            if (rc) return rc;
+        }
+    }
+    return AEO_OK;
+}
+
+int points_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,
+                    int dealias, hipStream_t s) {
+    SpecK k{p->L, R, p->d_Cp, p->d_Cm};
+    JacLoader<true> ld{a, b, false, nullptr, 0.0, k};
+    PwJac pw{p->d_gw, R};
+    int rc = launch_psyn(p, ld, pw, s);
+    if (rc) return rc;
+    rc = launch_partial(p, p->d_gw, nullptr, 1, p->d_part, s);
+    if (rc) return rc;
+    const int lim = dealias ? (2 * p->L) / 3 : p->L;
+    EpiStore1 ep{out, p->L};
+    return launch_finish(p, 1, 1, p->d_part, lim, lim, ep, s);
+}
+
+int points_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
+                       const double2* forcing, double2* kout, int stage, double dt,
+                       double2* ynext, hipStream_t s) {
+    aeo_plan_s* p = md->plan;
+    const int L = p->L;
+    const int cut = (2 * L) / 3;
+    SpecK k{L, md->R, p->d_Cp, p->d_Cm};
+    const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
+    Rk4 rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
+    if (md->kind == kBVE) {
+        JacLoader<true> ld{ys, nullptr, true, md->d_lap, f10, k};
+        PwJac pw{md->d_gw, md->R};
+        int rc = launch_psyn(p, ld, pw, s);
+        if (rc) return rc;
+        rc = launch_partial(p, md->d_gw, nullptr, 1, md->d_part, s);
+        if (rc) return rc;
+        EpiBve ep{L, ys, forcing, md->d_nulap, rk};
+        return launch_finish(p, 1, 1, md->d_part, cut, cut, ep, s);
+    }
+    SweLoader<true> ld{ys, md->d_lap, f10, k};
+    PwSwe pw{md->d_gw, p->npts, md->R};
+    int rc = launch_psyn(p, ld, pw, s);
+    if (rc) return rc;
+    rc = launch_partial(p, md->d_gw, nullptr, 4, md->d_part, s);
+    if (rc) return rc;
+    EpiSwe ep{L, ys, md->d_lap, md->nu4 > 0.0 ? md->d_nu4lap2 : nullptr,
+              md->has_topo ? md->d_lap_phi_s : nullptr,
+              (md->has_topo && md->nu4 > 0.0) ? md->d_nu4_lap2_phi_s : nullptr, md->phi0, rk};
+    return launch_finish(p, 4, 4, md->d_part, cut, cut, ep, s);
+}
+
+}  // namespace aeo

@@ -1,0 +1,189 @@
+// Spectral algebra on the dense (L+1)^2 layout, spectral invariants with
+// compensated sums, and the velocity / gradient pointwise stage.
+#include "plan.hpp"
+#include "stage.hpp"
+
+namespace aeo {
+
+// C+ and C- of num/spectral_operators.py:135-176 (clamped numerators and
+// denominators, zero for m > l, C+[L,:] = 0, C-[0,:] = 0).
+__device__ inline double c_plus(int l, int m, int L) {
+    if (m > l || l == L) return 0.0;
+    const double Lb = l, Mb = m;
+    const double nu = fmax((Lb + 1.0) * (Lb + 1.0) - Mb * Mb, 0.0);
+    const double de = fmax((2.0 * Lb + 1.0) * (2.0 * Lb + 3.0), 1.0);
+    return Lb * sqrt(nu / de);
+}
+
+__device__ inline double c_minus(int l, int m) {
+    if (m > l || l == 0) return 0.0;
+    const double Lb = l, Mb = m;
+    const double nu = fmax(Lb * Lb - Mb * Mb, 0.0);
+    const double de = fmax((2.0 * Lb - 1.0) * (2.0 * Lb + 1.0), 1.0);
+    return -(Lb + 1.0) * sqrt(nu / de);
+}
+
+// One thread per (field, l, m).  Formulas: num/spectral_operators.py:213-334
+// and phy/barotropic.py:68-93.
+__global__ void spec_op_kernel(int op, int L, double R, const double2* __restrict__ in,
+                               double2* __restrict__ out, int nf) {
+    const int n = L + 1;
+    const size_t fs = (size_t)n * n;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fs * nf) return;
+    const int f = (int)(t / fs);
+    const int r = (int)(t - (size_t)f * fs);
+    const int l = r / n, m = r - l * n;
+    const double2* a = in + (size_t)f * fs;
+    const double lap = -(double)l * (l + 1.0) / (R * R);
+    double2 v;
+    switch (op) {
+        case AEO_OP_LAP: v = cscale(a[r], lap); break;
+        case AEO_OP_INVLAP: {
+            const double d = (l == 0) ? 1.0 / (R * R) : lap;
+            v = c2(a[r].x / d, a[r].y / d);
+            break;
+        }
+        case AEO_OP_INVLAP_ZERO: {
+            v = (l == 0) ? c2(0.0, 0.0) : c2(a[r].x / lap, a[r].y / lap);
+            break;
+        }
+        case AEO_OP_DLAMBDA: v = dlam(a[r], m, R); break;
+        case AEO_OP_SINTH_DTH: {
+            // g_l = C+_{l-1} a_{l-1} + C-_{l+1} a_{l+1}
+            v = c2(0.0, 0.0);
+            if (l >= 1) v = cadd(v, cscale(a[r - n], c_plus(l - 1, m, L)));
+            if (l <= L - 1) v = cadd(v, cscale(a[r + n], c_minus(l + 1, m)));
+            break;
+        }
+        case AEO_OP_SINTH_DTH_T: {
+            // (S^T b)_l = C+_l b_{l+1} + C-_l b_{l-1}
+            v = c2(0.0, 0.0);
+            if (l <= L - 1) v = cadd(v, cscale(a[r + n], c_plus(l, m, L)));
+            if (l >= 1) v = cadd(v, cscale(a[r - n], c_minus(l, m)));
+            break;
+        }
+        default: {  // AEO_OP_TRUNC23
+            const int cut = (2 * L) / 3;
+            v = (l > cut || m > cut) ? c2(0.0, 0.0) : a[r];
+            break;
+        }
+    }
+    out[t] = v;
+}
+
+int launch_spec_op(int op, int L, double R, const double2* in, double2* out, int nf,
+                   hipStream_t s) {
+    const size_t total = (size_t)(L + 1) * (L + 1) * nf;
+    if (total == 0) return AEO_OK;
+    hipLaunchKernelGGL(spec_op_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, op,
+                       L, R, in, out, nf);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Compensated (Neumaier / TwoSum) accumulation.
+// ---------------------------------------------------------------------------
+struct CSum {
+    double s, c;
+};
+
+__device__ inline CSum cs_add(CSum a, double x) {
+    const double t = a.s + x;
+    const double bp = t - a.s;
+    const double err = (a.s - (t - bp)) + (x - bp);
+    return {t, a.c + err};
+}
+
+__device__ inline CSum cs_merge(CSum a, CSum b) {
+    CSum r = cs_add(a, b.s);
+    r.c += b.c;
+    return r;
+}
+
+// One block per field: circulation, energy, relative and absolute enstrophy
+// (run/bve/diagnostics.py:52-97).
+__global__ void spectral_diag_kernel(int L, double R, double omega,
+                                     const double2* __restrict__ z, double* __restrict__ out) {
+    const int n = L + 1;
+    const double2* a = z + (size_t)blockIdx.x * n * n;
+    const double f10 = 2.0 * omega * sqrt(4.0 * M_PI / 3.0);
+    CSum e = {0, 0}, zr = {0, 0}, za = {0, 0};
+    for (int r = threadIdx.x; r < n * n; r += blockDim.x) {
+        const int l = r / n, m = r - l * n;
+        if (m > l) continue;
+        const double2 v = a[r];
+        const double p = (m == 0) ? v.x * v.x : 2.0 * (v.x * v.x + v.y * v.y);
+        zr = cs_add(zr, p);
+        if (l >= 1) e = cs_add(e, p / ((double)l * (l + 1.0)));
+        double2 q = v;
+        if (l == 1 && m == 0) q.x += f10;
+        const double pq = (m == 0) ? q.x * q.x : 2.0 * (q.x * q.x + q.y * q.y);
+        za = cs_add(za, pq);
+    }
+    __shared__ CSum se[256], sz[256], sa[256];
+    se[threadIdx.x] = e;
+    sz[threadIdx.x] = zr;
+    sa[threadIdx.x] = za;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            se[threadIdx.x] = cs_merge(se[threadIdx.x], se[threadIdx.x + w]);
+            sz[threadIdx.x] = cs_merge(sz[threadIdx.x], sz[threadIdx.x + w]);
+            sa[threadIdx.x] = cs_merge(sa[threadIdx.x], sa[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* o = out + 4 * blockIdx.x;
+        o[0] = R * R * sqrt(4.0 * M_PI) * a[0].x;
+        o[1] = 0.5 * R * R * R * R * (se[0].s + se[0].c);
+        o[2] = 0.5 * R * R * (sz[0].s + sz[0].c);
+        o[3] = 0.5 * R * R * (sa[0].s + sa[0].c);
+    }
+}
+
+int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf, double* out,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(spectral_diag_kernel, dim3(nf), dim3(256), 0, s, L, R, omega, z, out);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+// mode 0: velocity (num/spectral_operators.py:517-538):
+//   u = g_s / (R c), v = g_lam / c, with |c| < 1e-6 -> 0
+// mode 1: gradient (num/spectral_operators.py:542-558):
+//   dq/dx = g_lam / c, dq/dy = -g_s / (R c)
+// coslat holds one value per point (row_len = 0) or per row of row_len points.
+__global__ void velocity_kernel(int npts, int row_len, double R,
+                                const double* __restrict__ coslat,
+                                const double* __restrict__ g_s, const double* __restrict__ g_lam,
+                                double* __restrict__ u, double* __restrict__ v, int mode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double c = coslat[row_len > 0 ? i / row_len : i];
+    if (mode == 0) {
+        if (fabs(c) < 1e-6) {
+            u[i] = 0.0;
+            v[i] = 0.0;
+        } else {
+            u[i] = g_s[i] / (R * c);
+            v[i] = g_lam[i] / c;
+        }
+    } else {
+        u[i] = g_lam[i] / c;
+        v[i] = -g_s[i] / (R * c);
+    }
+}
+
+int launch_velocity_points(int n, int row_len, double R, const double* coslat,
+                           const double* g_s, const double* g_lam, double* u, double* v, int mode,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(velocity_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, row_len, R,
+                       coslat, g_s, g_lam, u, v, mode);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+}  // namespace aeo

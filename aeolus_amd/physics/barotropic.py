@@ -1,0 +1,102 @@
+"""Barotropic vorticity model on the HIP device (``phy/barotropic.py:8-167``).
+
+``dzeta/dt = -J(psi, zeta + f) + nu lap' zeta + forcing`` with ``psi = zeta/lap``
+(l >= 1), ``f`` exact in spectral space (only (1, 0) = 2 Omega sqrt(4 pi/3)),
+the Jacobian analysed on the backend's product space and truncated once,
+and the l = 0 row pinned to zero.  The whole tendency is one fused library
+call (Legendre synthesis, FFT + product + FFT, Legendre analysis with the
+tendency epilogue).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import torch
+
+from .. import _native as nat
+
+__all__ = ["BarotropicState", "BarotropicVorticity"]
+
+
+@dataclass
+class BarotropicState:
+    coeffs: torch.Tensor
+    tendency: torch.Tensor = None
+
+
+class BarotropicVorticity:
+    def __init__(self, planet, scenario: str = "two_vortices", viscosity: float = 0.0):
+        self.planet = planet
+        self.nu = float(viscosity)
+        self.sh = planet.sh
+        self.so = planet.so
+        self.grid = planet.grid
+        self.R = float(planet.params.radius)
+        self.Omega = float(planet.params.angular_velocity)
+        dev = self.sh.device
+        self.device = dev
+        lat = torch.as_tensor(self.grid.point_latitudes, device=dev)
+        f = 2 * self.Omega * torch.sin(lat)
+        shape = getattr(self.grid, "grid_shape", None)
+        if shape is not None and len(shape) == 2 and f.numel() == shape[0] * shape[1]:
+            f = f.reshape(shape)
+        self.f = f
+        L = self.sh.l_max
+        self.f_lm = torch.zeros((L + 1, L + 1), dtype=torch.complex128, device=dev)
+        self.f_lm[1, 0] = 2.0 * self.Omega * math.sqrt(4.0 * math.pi / 3.0)
+        l = torch.arange(L + 1, dtype=torch.float64, device=dev)
+        self.laplacian_eig = -l * (l + 1) / self.R ** 2
+        self.laplacian_eig[0] = 1.0 / self.R ** 2
+        self._ps = self.so.backend.product_space(self.so.product_quadrature)
+        handle = ctypes.c_void_p()
+        nat.check(nat.lib().aeo_bve_create(self._ps.sh._plan, self.R, self.Omega, self.nu,
+                                           ctypes.byref(handle)), "BarotropicVorticity")
+        self._model = handle
+
+    def __del__(self):
+        h = getattr(self, "_model", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().aeo_model_destroy(h)
+            except Exception:
+                pass
+            self._model = ctypes.c_void_p()
+
+    def _coeffs(self, c) -> torch.Tensor:
+        return torch.as_tensor(c, device=self.device).to(torch.complex128).contiguous()
+
+    def vorticity_to_streamfunction(self, vrt_state: BarotropicState) -> torch.Tensor:
+        assert isinstance(vrt_state, BarotropicState)
+        z = self._coeffs(vrt_state.coeffs)
+        out = torch.empty_like(z)
+        nat.check(nat.lib().aeo_spec_op(nat.OP_INVLAP_ZERO, self.sh.l_max, self.R, z.data_ptr(),
+                                        out.data_ptr(), 1, nat.stream()),
+                  "vorticity_to_streamfunction")
+        return out
+
+    def streamfunction_to_vorticity(self, psi_coeffs) -> BarotropicState:
+        z = torch.zeros_like(self._coeffs(psi_coeffs))
+        z[1:, :] = self._coeffs(psi_coeffs)[1:, :] * self.laplacian_eig[1:, None]
+        return BarotropicState(z)
+
+    def tendency(self, vrt_state: BarotropicState, forcing_coeffs=None) -> torch.Tensor:
+        z = self._coeffs(vrt_state.coeffs)
+        out = torch.empty_like(z)
+        frc = None if forcing_coeffs is None else self._coeffs(forcing_coeffs)
+        nat.check(nat.lib().aeo_model_tendency(self._model, z.data_ptr(),
+                                               None if frc is None else frc.data_ptr(),
+                                               out.data_ptr(), nat.stream()), "tendency")
+        return out
+
+    def rk4(self, coeffs, dt: float, nsteps: int = 1, forcing_coeffs=None,
+            out: torch.Tensor | None = None) -> torch.Tensor:
+        """``nsteps`` fused RK4 steps (one library call); returns the new coefficients."""
+        z = self._coeffs(coeffs)
+        y = torch.empty_like(z) if out is None else out
+        frc = None if forcing_coeffs is None else self._coeffs(forcing_coeffs)
+        nat.check(nat.lib().aeo_model_rk4(self._model, z.data_ptr(), float(dt), int(nsteps),
+                                          None if frc is None else frc.data_ptr(),
+                                          y.data_ptr(), nat.stream()), "rk4")
+        return y

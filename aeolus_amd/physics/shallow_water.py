@@ -1,0 +1,215 @@
+"""Rotating shallow-water model on the HIP device (``phy/shallow_water.py:95-539``).
+
+State: one ``(3, L+1, L+1)`` complex128 stack (zeta, delta, phi').  The
+tendency is one fused library call: 10 Legendre syntheses, 12 inverse FFTs,
+the flux products of ``phy/shallow_water.py:377-386`` on the product grid, 4
+forward FFTs, 4 truncated Legendre analyses and the assembly epilogue
+(``:389-418``) including topography and hyperdiffusion terms.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import torch
+
+from .. import _native as nat
+from .topography import Topography
+
+__all__ = ["ShallowWaterModel", "ShallowWaterState", "ShallowWaterStateError",
+           "WILLIAMSON_GRAVITY", "ZETA", "DELTA", "PHI", "PROGNOSTICS"]
+
+WILLIAMSON_GRAVITY = 9.80616
+ZETA, DELTA, PHI = 0, 1, 2
+PROGNOSTICS = ("zeta", "delta", "phi")
+
+
+class ShallowWaterStateError(ValueError):
+    """A shallow-water state violated a hard physical/numerical constraint."""
+
+
+@dataclass
+class ShallowWaterState:
+    coeffs: torch.Tensor
+
+    @classmethod
+    def from_fields(cls, zeta_lm, delta_lm, phi_lm) -> "ShallowWaterState":
+        return cls(torch.stack([torch.as_tensor(x).to(torch.complex128)
+                                for x in (zeta_lm, delta_lm, phi_lm)]))
+
+    @classmethod
+    def zeros(cls, l_max: int, device=None) -> "ShallowWaterState":
+        n = l_max + 1
+        return cls(torch.zeros((3, n, n), dtype=torch.complex128, device=device))
+
+    @property
+    def zeta(self):
+        return self.coeffs[ZETA]
+
+    @property
+    def delta(self):
+        return self.coeffs[DELTA]
+
+    @property
+    def phi(self):
+        return self.coeffs[PHI]
+
+
+class ShallowWaterModel:
+    _MONOPOLE_RTOL = 1e-10
+
+    def __init__(self, planet, *, gravity: float = WILLIAMSON_GRAVITY, mean_depth: float,
+                 hyperdiffusion_nu4: float = 0.0, topography: Topography | None = None):
+        if not (math.isfinite(gravity) and gravity > 0):
+            raise ValueError(f"gravity must be finite and > 0, got {gravity}")
+        if not (math.isfinite(mean_depth) and mean_depth > 0):
+            raise ValueError(f"mean_depth must be finite and > 0, got {mean_depth}")
+        if not (math.isfinite(hyperdiffusion_nu4) and hyperdiffusion_nu4 >= 0):
+            raise ValueError(f"hyperdiffusion_nu4 must be finite and >= 0, got {hyperdiffusion_nu4}")
+        self.planet = planet
+        self.sh = planet.sh
+        self.so = planet.so
+        self.grid = planet.grid
+        self.device = self.sh.device
+        self.R = float(planet.params.radius)
+        self.Omega = float(planet.params.angular_velocity)
+        self.gravity = float(gravity)
+        self.mean_depth = float(mean_depth)
+        self.phi0 = self.gravity * self.mean_depth
+        self.nu4 = float(hyperdiffusion_nu4)
+        self.l_max = self.sh.l_max
+        n = self.l_max + 1
+        self.f_lm = torch.zeros((n, n), dtype=torch.complex128, device=self.device)
+        self.f_lm[1, 0] = 2.0 * self.Omega * math.sqrt(4.0 * math.pi / 3.0)
+        l = torch.arange(n, dtype=torch.float64, device=self.device)
+        self.lap_eigs = -l * (l + 1.0) / self.R ** 2
+        self._ps = self.so.backend.product_space(self.so.product_quadrature)
+        coslat = getattr(self.grid, "coslat", None)
+        if coslat is None:
+            coslat = torch.cos(torch.as_tensor(self.grid.point_latitudes))
+        self._state_coslat = torch.clamp(torch.as_tensor(coslat, device=self.device), min=1e-8)
+        self._trunc_cut = (2 * self.l_max) // 3
+        if topography is None:
+            topography = Topography.flat(self.l_max, device=self.device)
+        if topography.l_max != self.l_max:
+            raise ValueError(f"topography truncation l_max={topography.l_max} does not match "
+                             f"the model truncation l_max={self.l_max}")
+        self.topography = topography
+        self.has_topography = not topography.is_flat
+        if self.has_topography:
+            self.phi_s_lm = topography.surface_geopotential_lm(self.gravity).to(self.device)
+            self.phi_s_anom_lm = self.phi_s_lm.clone()
+            self.phi_s_anom_lm[0, 0] = 0.0
+            samplings = [self.sh] + ([self._ps.sh] if self._ps.sh is not self.sh else [])
+            lo, hi = math.inf, -math.inf
+            for sh in samplings:
+                e = topography.elevation_on(sh)
+                lo, hi = min(lo, float(e.min())), max(hi, float(e.max()))
+            self.surface_elevation_extrema = (lo, hi)
+            self._phi_s_state = self.sh.inv_transform(self.phi_s_lm)
+            phi_s_ptr = self.phi_s_lm.contiguous()
+        else:
+            self.phi_s_lm = torch.zeros((n, n), dtype=torch.complex128, device=self.device)
+            self.phi_s_anom_lm = self.phi_s_lm.clone()
+            self.surface_elevation_extrema = (0.0, 0.0)
+            self._phi_s_state = None
+            phi_s_ptr = None
+        handle = ctypes.c_void_p()
+        nat.check(nat.lib().aeo_swe_create(self._ps.sh._plan, self.R, self.Omega, self.gravity,
+                                           self.mean_depth, self.nu4,
+                                           None if phi_s_ptr is None else phi_s_ptr.data_ptr(),
+                                           ctypes.byref(handle)), "ShallowWaterModel")
+        self._model = handle
+
+    def __del__(self):
+        h = getattr(self, "_model", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().aeo_model_destroy(h)
+            except Exception:
+                pass
+            self._model = ctypes.c_void_p()
+
+    def _stack(self, coeffs) -> torch.Tensor:
+        y = torch.as_tensor(coeffs, device=self.device).to(torch.complex128).contiguous()
+        n = self.l_max + 1
+        if tuple(y.shape) != (3, n, n):
+            raise ValueError(f"shallow-water state must have shape (3, {n}, {n})")
+        return y
+
+    # -- Helmholtz / wind (phy/shallow_water.py:286-330) -------------------
+    def _inv_laplacian(self, coeffs):
+        c = torch.as_tensor(coeffs, device=self.device).to(torch.complex128).contiguous()
+        out = torch.empty_like(c)
+        nat.check(nat.lib().aeo_spec_op(nat.OP_INVLAP_ZERO, self.l_max, self.R, c.data_ptr(),
+                                        out.data_ptr(), c.numel() // (self.l_max + 1) ** 2,
+                                        nat.stream()), "inv_laplacian")
+        return out
+
+    def helmholtz(self, state: ShallowWaterState):
+        return self._inv_laplacian(state.coeffs[ZETA]), self._inv_laplacian(state.coeffs[DELTA])
+
+    def _deriv_fields(self, sh, coeffs):
+        lam = sh.inv_transform(self.so.d_lambda_coeffs(coeffs))
+        snt = sh.inv_transform(self.so.sin_theta_d_theta_coeffs(coeffs)) / self.R
+        return lam, snt
+
+    def _wind_from(self, sh, coslat, psi_lm, chi_lm):
+        psi_lam, psi_snt = self._deriv_fields(sh, psi_lm)
+        chi_lam, chi_snt = self._deriv_fields(sh, chi_lm)
+        return (psi_snt + chi_lam) / coslat, (psi_lam - chi_snt) / coslat
+
+    def wind_on_state_grid(self, state: ShallowWaterState):
+        psi, chi = self.helmholtz(state)
+        return self._wind_from(self.sh, self._state_coslat, psi, chi)
+
+    def surface_geopotential_on_state_grid(self):
+        return self._phi_s_state
+
+    # -- tendencies --------------------------------------------------------
+    def tendency(self, coeffs) -> torch.Tensor:
+        y = self._stack(coeffs)
+        out = torch.empty_like(y)
+        nat.check(nat.lib().aeo_model_tendency(self._model, y.data_ptr(), None, out.data_ptr(),
+                                               nat.stream()), "tendency")
+        return out
+
+    def tendency_state(self, state: ShallowWaterState) -> ShallowWaterState:
+        return ShallowWaterState(self.tendency(state.coeffs))
+
+    def rk4(self, coeffs, dt: float, nsteps: int = 1, out: torch.Tensor | None = None):
+        """``nsteps`` fused RK4 steps without stage validation (one library call)."""
+        y = self._stack(coeffs)
+        res = torch.empty_like(y) if out is None else out
+        nat.check(nat.lib().aeo_model_rk4(self._model, y.data_ptr(), float(dt), int(nsteps), None,
+                                          res.data_ptr(), nat.stream()), "rk4")
+        return res
+
+    # -- validation (phy/shallow_water.py:428-539) -------------------------
+    def total_geopotential_extrema(self, state: ShallowWaterState) -> tuple[float, float]:
+        phi_c = state.coeffs[PHI]
+        samplings = [self.sh] + ([self._ps.sh] if self._ps.sh is not self.sh else [])
+        lo, hi = math.inf, -math.inf
+        for sh in samplings:
+            g = sh.inv_transform(phi_c)
+            lo, hi = min(lo, float(g.min())), max(hi, float(g.max()))
+        return self.phi0 + lo, self.phi0 + hi
+
+    def validate_state(self, state: ShallowWaterState, *, context: str = "") -> None:
+        where = f" {context}" if context else ""
+        coeffs = state.coeffs
+        if not bool(torch.isfinite(coeffs).all()):
+            raise ShallowWaterStateError(f"shallow-water state contains NaN/Inf coefficients{where}")
+        for idx, name in enumerate(PROGNOSTICS):
+            field = coeffs[idx]
+            mono = float(torch.abs(field[0, 0]))
+            scale = float(torch.linalg.norm(field))
+            if mono > self._MONOPOLE_RTOL * max(scale, 1e-30):
+                raise ShallowWaterStateError(
+                    f"{name} monopole is nonzero{where}: |a00| = {mono:g} (field norm {scale:g})")
+        lo, _ = self.total_geopotential_extrema(state)
+        if not lo > 0.0:
+            raise ShallowWaterStateError(
+                f"fluid thickness is not strictly positive{where}: min(Phi0 + phi) = {lo:g} "
+                f"m^2/s^2 over the state/product samplings (Phi0 = {self.phi0:g})")

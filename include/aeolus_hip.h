@@ -1,0 +1,171 @@
+/*
+ * aeolus_hip: MI355X (gfx950) spectral time step behind the spherical-backend
+ * interface of Aeolus.
+ *
+ * C ABI.  Every function returns an int status (AEO_OK = 0, negative on
+ * failure) and leaves a thread-local message for aeo_last_error().  All
+ * array arguments of execute calls are DEVICE pointers (for example
+ * torch.Tensor.data_ptr() of a CUDA/HIP tensor); arguments documented as
+ * "host" are host pointers read during the call.  Complex arrays are
+ * interleaved FP64 pairs (the complex128 layout).  `stream` is a hipStream_t
+ * (NULL = the default stream).  Plans and models allocate their device
+ * memory when created; execute calls allocate nothing and only enqueue work
+ * on `stream`.  A plan or model is not safe for concurrent use from several
+ * host threads or streams; distinct objects are.
+ *
+ * Spectral layout (the reference's shared coefficient layout): one field is
+ * a dense (L+1) x (L+1) complex128 array a[l][m], C order, m >= 0 only;
+ * entries with m > l are padding (synthesis ignores them, analysis writes
+ * 0).  A real field is implied by a_{l,-m} = (-1)^m conj(a_lm).  Several
+ * fields are stacked contiguously (field stride (L+1)^2).
+ *
+ * Grid layout: Gauss grids are (nlat, nlon) FP64, C order, latitudes north
+ * to south, longitudes 2 pi k / nlon.  Point sets are (npoints,) FP64.
+ *
+ * Reference paths cited below are relative to the Aeolus repository root;
+ * num/ = src/planetary_sandbox/numerics/, phy/ = src/planetary_sandbox/physics/,
+ * run/ = src/planetary_sandbox/run/.
+ */
+#ifndef AEOLUS_HIP_H
+#define AEOLUS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AEO_ABI_VERSION 1
+
+enum aeo_status {
+    AEO_OK = 0,
+    AEO_ERR_INVALID = -1,      /* bad argument (size, null pointer, layout) */
+    AEO_ERR_UNSUPPORTED = -2,  /* valid request this build does not support */
+    AEO_ERR_HIP = -3,          /* HIP runtime or launch failure */
+    AEO_ERR_STATE = -4         /* object used in a state that forbids the call */
+};
+
+/* Spectral operator codes for aeo_spec_op. */
+enum aeo_spec_op_code {
+    AEO_OP_LAP = 0,         /* -l(l+1)/R^2 a                      num/spectral_operators.py:213-218 */
+    AEO_OP_INVLAP = 1,      /* a / lap, lap_0 = 1/R^2            num/spectral_operators.py:275-292 */
+    AEO_OP_DLAMBDA = 2,     /* (i m / R) a, m = 0 column zero     num/spectral_operators.py:220-226 */
+    AEO_OP_SINTH_DTH = 3,   /* S a (sin(theta) d/dtheta, l+-1)    num/spectral_operators.py:228-241 */
+    AEO_OP_SINTH_DTH_T = 4, /* S^T a                              num/spectral_operators.py:243-270 */
+    AEO_OP_TRUNC23 = 5,     /* zero rows and columns > 2L/3       num/spectral_operators.py:329-334 */
+    AEO_OP_INVLAP_ZERO = 6  /* a / lap with the l = 0 row zeroed  phy/barotropic.py:68-93 */
+};
+
+typedef struct aeo_plan_s* aeo_plan_t;
+typedef struct aeo_model_s* aeo_model_t;
+
+/* ABI version (AEO_ABI_VERSION) of the loaded library. */
+int aeo_version(void);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char* aeo_last_error(void);
+
+/* Gauss-Legendre x uniform-longitude transform plan.
+ * Replaces GaussLatLonSphericalHarmonics.__init__ + PointSetSphericalHarmonics
+ * basis construction (num/latlon_grid.py:160-184, num/fast_geodesic_sh.py:15-66,
+ * cu/sh_matrix.cu:7-76).  The FP64 normalised associated-Legendre table is
+ * built on the device by a stable recurrence.
+ *   x_nodes   host, nlat: cos(colatitude) of each row, north to south, symmetric
+ *             (x[j] = -x[nlat-1-j], as numpy.polynomial.legendre.leggauss gives)
+ *   w_rows    host, nlat: solid-angle weight of every point of row j
+ *             (Gauss weight * 2 pi / nlon), symmetric
+ *   coslat_rows host, nlat: max(cos(latitude), 1e-8) of each row (product metric)
+ *   max_fields: largest number of fields one execute call may pass (>= 1). */
+int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
+                          const double* w_rows, const double* coslat_rows,
+                          int max_fields, aeo_plan_t* out);
+
+/* Arbitrary point-set transform plan (dense basis, FP64).
+ * Replaces PointSetSphericalHarmonics.__init__ (num/fast_geodesic_sh.py:15-66)
+ * and, with Voronoi weights, GeodesicSphericalHarmonics (num/optimized_geodesic_sh.py:42-133).
+ *   lat, lon, weights, coslat: host, npoints each (weights NULL = uniform 4 pi / N,
+ *   num/fast_geodesic_sh.py:35-37; coslat = max(cos(lat), 1e-8)). */
+int aeo_plan_create_points(int npoints, int l_max, const double* lat, const double* lon,
+                           const double* weights, const double* coslat, int max_fields,
+                           aeo_plan_t* out);
+
+int aeo_plan_destroy(aeo_plan_t plan);
+
+/* Query: l_max, number of grid points, number of basis functions. */
+int aeo_plan_info(aeo_plan_t plan, int* l_max, int* npoints, int* nbasis);
+
+/* Copy the plan's dense basis row-major (npoints, B) complex128 into `Y_out`
+ * (device).  Y[i, l(l+1)/2 + m] = Y_lm(point i).  Replaces the Y_matrix
+ * attribute (num/fast_geodesic_sh.py:48-56). */
+int aeo_plan_basis(aeo_plan_t plan, double* Y_out, void* stream);
+
+/* Analysis of `nfields` grid fields -> dense coefficients.
+ * Replaces PointSetSphericalHarmonics.transform (num/fast_geodesic_sh.py:68-87). */
+int aeo_analysis(aeo_plan_t plan, const double* f, int nfields, double* a_out, void* stream);
+
+/* Synthesis of `nfields` coefficient fields -> grid fields (uses Re(a_l0) only).
+ * Replaces PointSetSphericalHarmonics.inv_transform (num/fast_geodesic_sh.py:89-107). */
+int aeo_synthesis(aeo_plan_t plan, const double* a, int nfields, double* f_out, void* stream);
+
+/* Spectral algebra on `nfields` dense coefficient fields (in may equal out
+ * except for AEO_OP_SINTH_DTH and AEO_OP_SINTH_DTH_T).
+ * Replaces the SpectralOperators methods named at each op code. */
+int aeo_spec_op(int op, int l_max, double radius, const double* in, double* out,
+                int nfields, void* stream);
+
+/* Pseudospectral Jacobian J(a, b) on the product plan, analysed back to
+ * coefficients and (dealias != 0) 2/3-truncated once.
+ * Replaces SpectralOperators.jacobian_pseudospectral(a, b, dealias,
+ * return_spectral=True) (num/spectral_operators.py:590-658). */
+int aeo_jacobian(aeo_plan_t prod, double radius, const double* a, const double* b,
+                 double* out, int dealias, void* stream);
+
+/* Velocity (u east, v north) on the plan's points from streamfunction
+ * coefficients, with the |cos(lat)| < 1e-6 -> 0 mask.
+ * Replaces SpectralOperators.velocity_from_streamfunction (num/spectral_operators.py:517-538). */
+int aeo_velocity(aeo_plan_t plan, double radius, const double* psi, double* u, double* v,
+                 void* stream);
+
+/* Gradient (d/dx, d/dy) of a scalar on the plan's points (unclamped cos(lat)).
+ * Replaces SpectralOperators.grad_from_scalar (num/spectral_operators.py:542-558). */
+int aeo_gradient(aeo_plan_t plan, double radius, const double* q, double* dqdx, double* dqdy,
+                 void* stream);
+
+/* Barotropic vorticity model on a product plan.
+ * Replaces BarotropicVorticity.__init__ (phy/barotropic.py:21-66). */
+int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosity,
+                   aeo_model_t* out);
+
+/* Shallow-water model on a product plan.  phi_s_lm: device (L+1)^2 complex
+ * surface geopotential g*h_s, or NULL for a flat bottom.
+ * Replaces ShallowWaterModel.__init__ (phy/shallow_water.py:172-283). */
+int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
+                   double mean_depth, double hyperdiffusion_nu4, const double* phi_s_lm,
+                   aeo_model_t* out);
+
+int aeo_model_destroy(aeo_model_t model);
+
+/* Tendency of one state (BVE: one field; SWE: the (3, L+1, L+1) stack zeta,
+ * delta, phi).  forcing: device, same shape, or NULL (BVE only).
+ * Replaces BarotropicVorticity.tendency (phy/barotropic.py:120-167) and
+ * ShallowWaterModel.tendency (phy/shallow_water.py:347-418). */
+int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing, double* out,
+                       void* stream);
+
+/* `nsteps` classical RK4 steps of size dt: y_out = RK4^nsteps(y).  y_out may
+ * equal y.  forcing as for aeo_model_tendency.
+ * Replaces rk4_step (run/bve/runner.py:211-216) and rk4_step_array without a
+ * stage validator (run/engine.py:406-431), called nsteps times. */
+int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
+                  const double* forcing, double* y_out, void* stream);
+
+/* Spectral invariants of `nfields` vorticity fields with compensated sums:
+ * out[4 f + 0..3] = circulation, energy, relative enstrophy, absolute
+ * enstrophy (device FP64).
+ * Replaces spectral_diagnostics (run/bve/diagnostics.py:52-97). */
+int aeo_spectral_diag(int l_max, double radius, double omega, const double* zeta, int nfields,
+                      double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEOLUS_HIP_H */
