@@ -47,6 +47,8 @@ _SIGNATURES = {
     "aeo_model_tendency": [_vp, _vp, _vp, _vp, _vp],
     "aeo_model_rk4": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp],
     "aeo_spectral_diag": [_c_int, _c_double, _c_double, _vp, _c_int, _vp, _vp],
+    "aeo_model_profile": [_vp, _c_int, _c_int],
+    "aeo_model_profile_read": [_vp, _dp, ctypes.POINTER(_c_int)],
 }
 
 _lib = None

@@ -63,8 +63,17 @@ static void free_plan(aeo_plan_s* p) {
     delete p;
 }
 
+static void free_prof(ProfRec& pr) {
+    if (pr.ev) {
+        for (int i = 0; i < 2 * pr.cap; ++i) (void)hipEventDestroy(pr.ev[i]);
+        delete[] pr.ev;
+    }
+    pr = ProfRec();
+}
+
 static void free_model(aeo_model_s* m) {
     if (!m) return;
+    free_prof(m->prof);
     for (void* q : {(void*)m->d_nulap, (void*)m->d_lap, (void*)m->d_nu4lap2,
                     (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
                     (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
@@ -517,6 +526,48 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
         }
         cur = Yout;
     }
+    return AEO_OK;
+}
+
+int aeo_model_profile(aeo_model_t model, int kernel_class, int max_launches) {
+    AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_profile: model is NULL");
+    AEO_REQUIRE(kernel_class >= -1 && kernel_class <= kClsAnalysis, AEO_ERR_INVALID,
+                "aeo_model_profile: kernel_class must be -1 (off), 0, 1 or 2");
+    free_prof(model->prof);
+    if (kernel_class < 0 || max_launches <= 0) return AEO_OK;
+    ProfRec& pr = model->prof;
+    pr.ev = new (std::nothrow) hipEvent_t[2 * (size_t)max_launches];
+    AEO_REQUIRE(pr.ev != nullptr, AEO_ERR_HIP, "aeo_model_profile: out of host memory");
+    for (int i = 0; i < 2 * max_launches; ++i) {
+        hipError_t e = hipEventCreate(&pr.ev[i]);
+        if (e != hipSuccess) {
+            pr.cap = i / 2;
+            for (int j = 2 * pr.cap; j < i; ++j) (void)hipEventDestroy(pr.ev[j]);
+            free_prof(pr);
+            set_error(std::string("aeo_model_profile: hipEventCreate: ") + hipGetErrorString(e));
+            return AEO_ERR_HIP;
+        }
+    }
+    pr.cap = max_launches;
+    pr.cls = kernel_class;
+    pr.n = 0;
+    return AEO_OK;
+}
+
+int aeo_model_profile_read(aeo_model_t model, double* total_ms, int* launches) {
+    AEO_REQUIRE(model && total_ms && launches, AEO_ERR_INVALID,
+                "aeo_model_profile_read: NULL argument");
+    ProfRec& pr = model->prof;
+    double tot = 0.0;
+    for (int i = 0; i < pr.n; ++i) {
+        AEO_CHECK_HIP(hipEventSynchronize(pr.ev[2 * i + 1]));
+        float ms = 0.0f;
+        AEO_CHECK_HIP(hipEventElapsedTime(&ms, pr.ev[2 * i], pr.ev[2 * i + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = pr.n;
+    pr.n = 0;
     return AEO_OK;
 }
 

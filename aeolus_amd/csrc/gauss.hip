@@ -470,19 +470,29 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     SpecK k{L, md->R, p->d_Cp, p->d_Cm};
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     Rk4 rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
+    ProfRec& pr = md->prof;
     if (md->kind == kBVE) {
         JacLoader<false> ld{ys, nullptr, true, md->d_lap, f10, k};
+        pr.begin(kClsSynthesis, s);
         int rc = launch_syn(p, ld, md->d_E, md->d_O, s);
+        pr.end(kClsSynthesis, s);
         if (rc) return rc;
+        pr.begin(kClsGrid, s);
         hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh), dim3(kGridThreads),
                            sizeof(double2) * 4 * p->nlon, s, p->gdev(), md->d_E, md->d_O,
                            md->R, cut, md->d_S, md->d_D);
+        pr.end(kClsGrid, s);
         AEO_LAUNCH_CHECK();
         EpiBve ep{L, ys, forcing, md->d_nulap, rk};
-        return launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
+        pr.begin(kClsAnalysis, s);
+        rc = launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
+        pr.end(kClsAnalysis, s);
+        return rc;
     }
     SweLoader<false> ld{ys, md->d_lap, f10, k};
+    pr.begin(kClsSynthesis, s);
     int rc = launch_syn(p, ld, md->d_E, md->d_O, s);
+    pr.end(kClsSynthesis, s);
     if (rc) return rc;
     const size_t swe_lds = sizeof(double2) * 12 * p->nlon;
     if (swe_lds > 65536) {
@@ -490,13 +500,18 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)swe_lds));
     }
+    pr.begin(kClsGrid, s);
     hipLaunchKernelGGL(grid_swe_kernel, dim3(p->nh), dim3(kGridThreads), swe_lds, s, p->gdev(), md->d_E, md->d_O,
                        md->R, cut, md->d_S, md->d_D);
+    pr.end(kClsGrid, s);
     AEO_LAUNCH_CHECK();
     EpiSwe ep{L, ys, md->d_lap, md->nu4 > 0.0 ? md->d_nu4lap2 : nullptr,
               md->has_topo ? md->d_lap_phi_s : nullptr,
               (md->has_topo && md->nu4 > 0.0) ? md->d_nu4_lap2_phi_s : nullptr, md->phi0, rk};
-    return launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
+    pr.begin(kClsAnalysis, s);
+    rc = launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
+    pr.end(kClsAnalysis, s);
+    return rc;
 }
 
 }  // namespace aeo

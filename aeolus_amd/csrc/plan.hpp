@@ -79,6 +79,26 @@ struct aeo_plan_s {
 
 enum ModelKind { kBVE = 0, kSWE = 1 };
 
+// Kernel classes of one model stage (for aeo_model_profile).
+enum KernelClass { kClsSynthesis = 0, kClsGrid = 1, kClsAnalysis = 2 };
+
+// Optional HIP-event bracketing of every launch of one kernel class.
+struct ProfRec {
+    int cls = -1;
+    int cap = 0;
+    int n = 0;
+    hipEvent_t* ev = nullptr;  // 2 * cap events
+    void begin(int c, hipStream_t s) {
+        if (c == cls && n < cap) (void)hipEventRecord(ev[2 * n], s);
+    }
+    void end(int c, hipStream_t s) {
+        if (c == cls && n < cap) {
+            (void)hipEventRecord(ev[2 * n + 1], s);
+            ++n;
+        }
+    }
+};
+
 struct aeo_model_s {
     int kind = kBVE;
     aeo_plan_s* plan = nullptr;
@@ -103,6 +123,7 @@ struct aeo_model_s {
     double2* d_part = nullptr;
     double2* d_ystage = nullptr;
     double2* d_ksum = nullptr;
+    ProfRec prof;
 };
 
 namespace aeo {

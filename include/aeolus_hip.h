@@ -157,6 +157,17 @@ int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing
 int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
                   const double* forcing, double* y_out, void* stream);
 
+/* Bracket every launch of one kernel class of the model's stages with a
+ * pair of HIP events on the launch stream (0 = Legendre synthesis, 1 = FFT
+ * grid stage with the pointwise products, 2 = Legendre analysis with the
+ * tendency/RK4 epilogue; -1 disables).  Events are created here, not in the
+ * execute calls.  No counterpart in the reference (measurement support). */
+int aeo_model_profile(aeo_model_t model, int kernel_class, int max_launches);
+
+/* Sum of the bracketed launch durations (ms) and their count since the last
+ * read; resets the count. */
+int aeo_model_profile_read(aeo_model_t model, double* total_ms, int* launches);
+
 /* Spectral invariants of `nfields` vorticity fields with compensated sums:
  * out[4 f + 0..3] = circulation, energy, relative enstrophy, absolute
  * enstrophy (device FP64).
