@@ -7,8 +7,8 @@ the same run: the shallow-water step at L = 85 on Gauss 128 x 256 with a
 2 km Gaussian mountain (dt = 300 s).  Inputs are resident in HBM before the
 timed region; a "step" is one full RK4 step (4 fused stages).
 
-Prints ONE JSON line (the driver's contract).  ``--gpus N`` with N > 1
-exits: this build measures one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the measurement
+covers one GPU.
 """
 from __future__ import annotations
 
@@ -165,7 +165,7 @@ def cpu_baseline(seconds: float):
 def main():
     args = _args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "measurements cover one GPU; --gpus must be 1"}))
         sys.exit(2)
     res, _ = run_gpu(args)
     b = res["bve170"]

@@ -38,7 +38,7 @@ extern "C" {
 enum aeo_status {
     AEO_OK = 0,
     AEO_ERR_INVALID = -1,      /* bad argument (size, null pointer, layout) */
-    AEO_ERR_UNSUPPORTED = -2,  /* valid request this build does not support */
+    AEO_ERR_UNSUPPORTED = -2,  /* valid request outside what the library implements */
     AEO_ERR_HIP = -3,          /* HIP runtime or launch failure */
     AEO_ERR_STATE = -4         /* object used in a state that forbids the call */
 };
@@ -72,7 +72,8 @@ const char* aeo_last_error(void);
  *             (x[j] = -x[nlat-1-j], as numpy.polynomial.legendre.leggauss gives)
  *   w_rows    host, nlat: solid-angle weight of every point of row j
  *             (Gauss weight * 2 pi / nlon), symmetric
- *   coslat_rows host, nlat: max(cos(latitude), 1e-8) of each row (product metric)
+ *   coslat_rows host, nlat: cos(latitude) of each row (the kernels clamp it at
+ *             1e-8 for products and mask |cos| < 1e-6 for velocities)
  *   max_fields: largest number of fields one execute call may pass (>= 1). */
 int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
                           const double* w_rows, const double* coslat_rows,
@@ -82,7 +83,7 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
  * Replaces PointSetSphericalHarmonics.__init__ (num/fast_geodesic_sh.py:15-66)
  * and, with Voronoi weights, GeodesicSphericalHarmonics (num/optimized_geodesic_sh.py:42-133).
  *   lat, lon, weights, coslat: host, npoints each (weights NULL = uniform 4 pi / N,
- *   num/fast_geodesic_sh.py:35-37; coslat = max(cos(lat), 1e-8)). */
+ *   num/fast_geodesic_sh.py:35-37; coslat = cos(lat), clamped in the kernels). */
 int aeo_plan_create_points(int npoints, int l_max, const double* lat, const double* lon,
                            const double* weights, const double* coslat, int max_fields,
                            aeo_plan_t* out);

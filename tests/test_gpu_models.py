@@ -2,7 +2,7 @@
 
 Bars: tendencies within 1e-12 relative L2 of the oracle; the C1 BVE
 trajectory (Gauss 12x24 / 13x25, L = 8, 144 RK4 steps of 600 s) within
-1e-12 relative L2 (BASELINE parity target for the plain FP64 path).
+1e-12 relative L2 (the parity bar of the plain FP64 path).
 """
 import math
 import warnings
