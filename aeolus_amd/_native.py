@@ -32,6 +32,9 @@ _SIGNATURES = {
     "aeo_plan_create_gauss": [_c_int, _c_int, _c_int, _dp, _dp, _dp, _c_int, ctypes.POINTER(_vp)],
     "aeo_plan_create_points": [_c_int, _c_int, _dp, _dp, _dp, _dp, _c_int, ctypes.POINTER(_vp)],
     "aeo_plan_destroy": [_vp],
+    "aeo_plan_enable_split": [_vp, _c_int],
+    "aeo_model_set_precision": [_vp, _c_int],
+    "aeo_model_set_tiles": [_vp, _c_int],
     "aeo_plan_info": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
     "aeo_plan_basis": [_vp, _vp, _vp],
     "aeo_analysis": [_vp, _vp, _c_int, _vp, _vp],
@@ -41,6 +44,7 @@ _SIGNATURES = {
     "aeo_velocity": [_vp, _c_double, _vp, _vp, _vp, _vp],
     "aeo_gradient": [_vp, _c_double, _vp, _vp, _vp, _vp],
     "aeo_bve_create": [_vp, _c_double, _c_double, _c_double, ctypes.POINTER(_vp)],
+    "aeo_bve_create_batch": [_vp, _c_double, _c_double, _c_double, _c_int, ctypes.POINTER(_vp)],
     "aeo_swe_create": [_vp, _c_double, _c_double, _c_double, _c_double, _c_double, _vp,
                        ctypes.POINTER(_vp)],
     "aeo_model_destroy": [_vp],

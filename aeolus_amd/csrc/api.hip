@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "plan.hpp"
+#include "split.hpp"
 
 namespace aeo {
 
@@ -58,7 +59,9 @@ static void free_plan(aeo_plan_s* p) {
                     (void*)p->d_coslat, (void*)p->d_tw, (void*)p->d_E, (void*)p->d_O,
                     (void*)p->d_S, (void*)p->d_D, (void*)p->d_Yk, (void*)p->d_Yp, (void*)p->d_w,
                     (void*)p->d_pcoslat, (void*)p->d_gw, (void*)p->d_part, (void*)p->d_Cp,
-                    (void*)p->d_Cm, (void*)p->d_spec})
+                    (void*)p->d_Cm, (void*)p->d_spec, (void*)p->d_Psl, (void*)p->d_offs_ss,
+                    (void*)p->d_esyn, (void*)p->d_Pasl, (void*)p->d_offs_sa, (void*)p->d_eana,
+                    (void*)p->d_offs_ea})
         dev_free(q);
     delete p;
 }
@@ -77,7 +80,7 @@ static void free_model(aeo_model_s* m) {
     for (void* q : {(void*)m->d_nulap, (void*)m->d_lap, (void*)m->d_nu4lap2,
                     (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
                     (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
-                    (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ksum})
+                    (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum})
         dev_free(q);
     delete m;
 }
@@ -254,6 +257,66 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
     return AEO_OK;
 }
 
+int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
+    AEO_REQUIRE(plan != nullptr, AEO_ERR_INVALID, "aeo_plan_enable_split: plan is NULL");
+    AEO_REQUIRE(plan->kind == kGauss, AEO_ERR_UNSUPPORTED,
+                "aeo_plan_enable_split: split-precision contractions need a Gauss plan");
+    AEO_REQUIRE(nslices >= 2 && nslices <= 8, AEO_ERR_INVALID,
+                "aeo_plan_enable_split: nslices must be in [2, 8]");
+    const int L = plan->L, nh = plan->nh;
+    AEO_REQUIRE(split_exact(nslices, split_kp(L, 0)) && split_exact(nslices, nh),
+                AEO_ERR_UNSUPPORTED,
+                "aeo_plan_enable_split: FP32 accumulation would not be exact at this size");
+    if (plan->nslices == nslices) return AEO_OK;
+    for (void* q : {(void*)plan->d_Psl, (void*)plan->d_offs_ss, (void*)plan->d_esyn,
+                    (void*)plan->d_Pasl, (void*)plan->d_offs_sa, (void*)plan->d_eana,
+                    (void*)plan->d_offs_ea})
+        dev_free(q);
+    plan->d_Psl = nullptr; plan->d_offs_ss = nullptr; plan->d_esyn = nullptr;
+    plan->d_Pasl = nullptr; plan->d_offs_sa = nullptr; plan->d_eana = nullptr;
+    plan->d_offs_ea = nullptr;
+    plan->nslices = 0;
+    const int nhp = rup(nh, 16), nhk = rup(nh, 32);
+    std::vector<long long> oss(L + 2, 0), osa(L + 2, 0), oea(L + 2, 0);
+    for (int m = 0; m <= L; ++m) {
+        const int kp = split_kp(L, m);
+        oss[m + 1] = oss[m] + 2LL * nslices * nhp * rup(kp, 32);
+        osa[m + 1] = osa[m] + 2LL * nslices * rup(kp, 16) * nhk;
+        oea[m + 1] = oea[m] + 2LL * rup(kp, 16);
+    }
+    int rc;
+    if ((rc = dev_alloc(&plan->d_Psl, (size_t)oss[L + 1]))) return rc;
+    if ((rc = dev_alloc(&plan->d_Pasl, (size_t)osa[L + 1]))) return rc;
+    if ((rc = dev_alloc(&plan->d_esyn, (size_t)(L + 1) * 2 * nhp))) return rc;
+    if ((rc = dev_alloc(&plan->d_eana, (size_t)oea[L + 1]))) return rc;
+    if ((rc = upload(&plan->d_offs_ss, oss.data(), oss.size()))) return rc;
+    if ((rc = upload(&plan->d_offs_sa, osa.data(), osa.size()))) return rc;
+    if ((rc = upload(&plan->d_offs_ea, oea.data(), oea.size()))) return rc;
+    if ((rc = gauss_build_split_tables(plan, nslices, nullptr))) return rc;
+    AEO_CHECK_HIP(hipDeviceSynchronize());
+    plan->nslices = nslices;
+    return AEO_OK;
+}
+
+int aeo_model_set_precision(aeo_model_t model, int nslices) {
+    AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_set_precision: model is NULL");
+    AEO_REQUIRE(nslices == 0 || (model->plan->kind == kGauss && model->plan->nslices == nslices),
+                AEO_ERR_STATE,
+                "aeo_model_set_precision: enable the split tables on the plan first "
+                "(aeo_plan_enable_split with the same nslices)");
+    model->nslices = nslices;
+    return AEO_OK;
+}
+
+int aeo_model_set_tiles(aeo_model_t model, int members_per_block) {
+    AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_set_tiles: model is NULL");
+    AEO_REQUIRE(members_per_block == 1 || members_per_block == 2 || members_per_block == 4 ||
+                    members_per_block == 8,
+                AEO_ERR_INVALID, "aeo_model_set_tiles: members_per_block must be 1, 2, 4 or 8");
+    model->mb = members_per_block;
+    return AEO_OK;
+}
+
 int aeo_plan_destroy(aeo_plan_t plan) {
     free_plan(plan);
     return AEO_OK;
@@ -377,23 +440,32 @@ static int model_common(aeo_plan_t prod, double radius, aeo_model_s* m) {
     const int nsyn = (m->kind == kBVE) ? 4 : 12;
     const int nana = (m->kind == kBVE) ? 1 : 4;
     const size_t fs = (size_t)(L + 1) * (L + 1);
+    const size_t nm = (size_t)m->nmem;
     if (prod->kind == kGauss) {
         const size_t plane = (size_t)prod->nh * (L + 1);
-        if ((rc = dev_alloc(&m->d_E, plane * nsyn))) return rc;
-        if ((rc = dev_alloc(&m->d_O, plane * nsyn))) return rc;
-        if ((rc = dev_alloc(&m->d_S, plane * nana))) return rc;
-        if ((rc = dev_alloc(&m->d_D, plane * nana))) return rc;
+        if ((rc = dev_alloc(&m->d_E, plane * nsyn * nm))) return rc;
+        if ((rc = dev_alloc(&m->d_O, plane * nsyn * nm))) return rc;
+        if ((rc = dev_alloc(&m->d_S, plane * nana * nm))) return rc;
+        if ((rc = dev_alloc(&m->d_D, plane * nana * nm))) return rc;
     } else {
         if ((rc = dev_alloc(&m->d_gw, (size_t)prod->npts * nana))) return rc;
         if ((rc = dev_alloc(&m->d_part, (size_t)prod->nchunks * nana * prod->B))) return rc;
     }
-    if ((rc = dev_alloc(&m->d_ystage, fs * m->nstate))) return rc;
-    if ((rc = dev_alloc(&m->d_ksum, fs * m->nstate))) return rc;
+    const size_t st = fs * m->nstate * nm;
+    if ((rc = dev_alloc(&m->d_ystage, 2 * st))) return rc;
+    if ((rc = dev_alloc(&m->d_ybuf, 2 * st))) return rc;
+    if ((rc = dev_alloc(&m->d_ksum, st))) return rc;
     return AEO_OK;
 }
 
-int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosity,
-                   aeo_model_t* out) {
+static int default_mb(int nmem) {
+    int mb = 1;
+    while (mb * 2 <= nmem && mb < 4) mb *= 2;
+    return mb;
+}
+
+int aeo_bve_create_batch(aeo_plan_t prod, double radius, double omega, double viscosity,
+                         int nmembers, aeo_model_t* out) {
     AEO_REQUIRE(out != nullptr, AEO_ERR_INVALID, "aeo_bve_create: out is NULL");
     *out = nullptr;
     AEO_REQUIRE(prod != nullptr, AEO_ERR_INVALID, "aeo_bve_create: plan is NULL");
@@ -401,12 +473,17 @@ int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosit
                 "aeo_bve_create: radius must be finite and > 0");
     AEO_REQUIRE(std::isfinite(omega) && std::isfinite(viscosity), AEO_ERR_INVALID,
                 "aeo_bve_create: omega and viscosity must be finite");
+    AEO_REQUIRE(nmembers >= 1, AEO_ERR_INVALID, "aeo_bve_create: nmembers must be >= 1");
+    AEO_REQUIRE(nmembers == 1 || prod->kind == kGauss, AEO_ERR_UNSUPPORTED,
+                "aeo_bve_create: member batches need a Gauss plan");
     aeo_model_s* m = new (std::nothrow) aeo_model_s();
     AEO_REQUIRE(m != nullptr, AEO_ERR_HIP, "aeo_bve_create: out of host memory");
     m->kind = kBVE;
     m->nstate = 1;
     m->omega = omega;
     m->nu = viscosity;
+    m->nmem = nmembers;
+    m->mb = default_mb(nmembers);
     int rc = model_common(prod, radius, m);
     if (rc) {
         free_model(m);
@@ -414,6 +491,11 @@ int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosit
     }
     *out = m;
     return AEO_OK;
+}
+
+int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosity,
+                   aeo_model_t* out) {
+    return aeo_bve_create_batch(prod, radius, omega, viscosity, 1, out);
 }
 
 int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
@@ -434,6 +516,8 @@ int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
     AEO_REQUIRE(m != nullptr, AEO_ERR_HIP, "aeo_swe_create: out of host memory");
     m->kind = kSWE;
     m->nstate = 3;
+    m->nmem = 1;
+    m->mb = 1;
     m->omega = omega;
     m->gravity = gravity;
     m->mean_depth = mean_depth;
@@ -483,21 +567,21 @@ int aeo_model_destroy(aeo_model_t model) {
     return AEO_OK;
 }
 
-static int run_stage(aeo_model_s* m, const double2* ys, const double2* y0, const double2* forcing,
-                     double2* kout, int stage, double dt, double2* ynext, hipStream_t s) {
-    if (m->plan->kind == kGauss)
-        return gauss_model_stage(m, ys, y0, forcing, kout, stage, dt, ynext, s);
-    return points_model_stage(m, ys, y0, forcing, kout, stage, dt, ynext, s);
-}
-
 int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing, double* out,
                        void* stream) {
     AEO_REQUIRE(model && y && out, AEO_ERR_INVALID, "aeo_model_tendency: NULL argument");
     AEO_REQUIRE(!(forcing && model->kind == kSWE), AEO_ERR_UNSUPPORTED,
                 "aeo_model_tendency: forcing is only defined for the BVE model");
+    hipStream_t s = as_stream(stream);
     auto Y = reinterpret_cast<const double2*>(y);
-    return run_stage(model, Y, Y, reinterpret_cast<const double2*>(forcing),
-                     reinterpret_cast<double2*>(out), -1, 0.0, nullptr, as_stream(stream));
+    auto F = reinterpret_cast<const double2*>(forcing);
+    auto K = reinterpret_cast<double2*>(out);
+    if (model->plan->kind == kGauss) {
+        int rc = gauss_model_prime(model, Y, s);
+        if (rc) return rc;
+        return gauss_model_stage(model, Y, Y, F, K, -1, 0.0, nullptr, false, s);
+    }
+    return points_model_stage(model, Y, Y, F, K, -1, 0.0, nullptr, s);
 }
 
 int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
@@ -508,24 +592,37 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
     AEO_REQUIRE(!(forcing && model->kind == kSWE), AEO_ERR_UNSUPPORTED,
                 "aeo_model_rk4: forcing is only defined for the BVE model");
     hipStream_t s = as_stream(stream);
-    const size_t fs = (size_t)(model->L + 1) * (model->L + 1) * model->nstate;
+    const size_t st = (size_t)(model->L + 1) * (model->L + 1) * model->nstate * model->nmem;
     auto F = reinterpret_cast<const double2*>(forcing);
-    auto Yout = reinterpret_cast<double2*>(y_out);
     const double2* cur = reinterpret_cast<const double2*>(y);
     if (nsteps == 0) {
         if (y != y_out)
-            AEO_CHECK_HIP(hipMemcpyAsync(y_out, y, fs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+            AEO_CHECK_HIP(hipMemcpyAsync(y_out, y, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
         return AEO_OK;
     }
+    const bool gauss = model->plan->kind == kGauss;
+    double2* ystage[2] = {model->d_ystage, model->d_ystage + st};
+    double2* ybuf[2] = {model->d_ybuf, model->d_ybuf + st};
+    if (gauss) {
+        int rc = gauss_model_prime(model, cur, s);
+        if (rc) return rc;
+    }
     for (int step = 0; step < nsteps; ++step) {
-        for (int st = 0; st < 4; ++st) {
-            const double2* ys = (st == 0) ? cur : model->d_ystage;
-            double2* ynext = (st == 3) ? Yout : model->d_ystage;
-            int rc = run_stage(model, ys, cur, F, nullptr, st, dt, ynext, s);
+        for (int k = 0; k < 4; ++k) {
+            const double2* ys = (k == 0) ? cur : ystage[(k - 1) % 2];
+            double2* ynext = (k == 3) ? ybuf[step % 2] : ystage[k % 2];
+            int rc;
+            if (gauss) {
+                const bool syn_next = !(step == nsteps - 1 && k == 3);
+                rc = gauss_model_stage(model, ys, cur, F, nullptr, k, dt, ynext, syn_next, s);
+            } else {
+                rc = points_model_stage(model, ys, cur, F, nullptr, k, dt, ynext, s);
+            }
             if (rc) return rc;
         }
-        cur = Yout;
+        cur = ybuf[step % 2];
     }
+    AEO_CHECK_HIP(hipMemcpyAsync(y_out, cur, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
     return AEO_OK;
 }
 

@@ -15,12 +15,11 @@
 // S = w (F_north + F_south) and D = w (F_north - F_south).  One complex FFT
 // of length nlon transforms a north/south row pair at once
 // (z = f_north + i f_south).
-#include "plan.hpp"
-#include "stage.hpp"
+#include "legendre.hpp"
+#include "split.hpp"
 
 namespace aeo {
 
-constexpr int kLegThreads = 256;  // 4 waves; wave w takes l - m = w (mod 4)
 constexpr int kRowElems = 4096;   // complex values per LDS pass of the generic grid kernels
 
 // ---------------------------------------------------------------------------
@@ -66,106 +65,53 @@ __global__ void legendre_table_kernel(int L, int nh, const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Legendre synthesis: E/O[f][jp][m] = sum over even/odd (l-m) of Pbar * value.
-// grid (L+1, ceil(nh/64), field groups); block 256 = 4 waves x 64 latitudes.
+// Split-precision digit tables (see split.hpp).  grid (L+1, 2 parities,
+// row tiles of 64); one thread per table row.
 // ---------------------------------------------------------------------------
-template <class Loader>
-__global__ __launch_bounds__(kLegThreads) void legendre_syn_kernel(GaussDev g, Loader ld,
-                                                                  double2* __restrict__ E,
-                                                                  double2* __restrict__ O) {
-    constexpr int NF = Loader::NF;
-    extern __shared__ double2 lds[];
-    const int m = blockIdx.x;
+__global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
+                                   int* __restrict__ esyn, __bf16* __restrict__ Pasl,
+                                   int* __restrict__ eana, int nhp, int nhk) {
+    const int m = blockIdx.x, p = blockIdx.y;
+    const int r = blockIdx.z * blockDim.x + threadIdx.x;
     const int L = g.L, nh = g.nh;
     const int K = L - m + 1;
-    double2* col = lds;                 // [NF][K]; reused as [2][NF][64] after the sum
-    for (int t = threadIdx.x; t < NF * K; t += kLegThreads) {
-        const int f = t / K, i = t - f * K;
-        col[t] = ld.value(f, m + i, m);
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int jp = blockIdx.y * 64 + lane;
-    double2 acc[NF];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) acc[f] = c2(0.0, 0.0);
-    if (jp < nh) {
-        const double* P = g.Ps + g.offs[m] + jp;
-        for (int i = wave; i < K; i += 4) {
-            const double p = P[(size_t)i * nh];
-#pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                const double2 c = col[f * K + i];
-                acc[f].x = fma(p, c.x, acc[f].x);
-                acc[f].y = fma(p, c.y, acc[f].y);
+    const int kp = split_kp(L, m), kpad = rup(kp, 32), mpad = rup(kp, 16);
+    float d[8];
+    if (r < nhp) {
+        const double* P = g.Ps + g.offs[m] + r;
+        double mx = 0.0;
+        if (r < nh)
+            for (int ip = 0; ip < kp; ++ip) {
+                const int i = 2 * ip + p;
+                if (i < K) mx = fmax(mx, fabs(P[(size_t)i * nh]));
             }
+        const int e = exp_above(mx);
+        esyn[((size_t)m * 2 + p) * nhp + r] = e;
+        __bf16* dst = Psl + g.offs_ss[m];
+        for (int ip = 0; ip < kpad; ++ip) {
+            const int i = 2 * ip + p;
+            const double v = (r < nh && ip < kp && i < K) ? P[(size_t)i * nh] : 0.0;
+            digits_rt(v, e, ns, d);
+            for (int s = 0; s < ns; ++s)
+                dst[((size_t)(p * ns + s) * nhp + r) * kpad + ip] = (__bf16)d[s];
         }
     }
-    __syncthreads();
-    double2* red = lds;
-    if (wave >= 2) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) red[((wave - 2) * NF + f) * 64 + lane] = acc[f];
-    }
-    __syncthreads();
-    // wave 0 -> E = (l-m even) sums, wave 1 -> O = (l-m odd) sums
-    if (wave < 2 && jp < nh) {
-        double2* dst = (wave == 0) ? E : O;
-        const size_t plane = (size_t)nh * (L + 1);
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const double2 b = red[(wave * NF + f) * 64 + lane];
-            dst[f * plane + (size_t)jp * (L + 1) + m] = cadd(acc[f], b);
+    if (r < mpad) {
+        const int l = m + 2 * r + p;
+        const bool valid = (r < kp) && (l <= L);
+        const double* P = g.Pa + g.offs[m] + (l - m);
+        double mx = 0.0;
+        if (valid)
+            for (int j = 0; j < nh; ++j) mx = fmax(mx, fabs(P[(size_t)j * K]));
+        const int e = exp_above(mx);
+        eana[g.offs_ea[m] + (size_t)p * mpad + r] = e;
+        __bf16* dst = Pasl + g.offs_sa[m];
+        for (int j = 0; j < nhk; ++j) {
+            const double v = (valid && j < nh) ? P[(size_t)j * K] : 0.0;
+            digits_rt(v, e, ns, d);
+            for (int s = 0; s < ns; ++s)
+                dst[((size_t)(p * ns + s) * mpad + r) * nhk + j] = (__bf16)d[s];
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Legendre analysis: acc_f(l, m) = sum_jp Pa * (S or D)[f][jp][m], then the
-// epilogue for every (l, m) of column m.  grid (L+1); block 256.
-// ---------------------------------------------------------------------------
-template <class Epi>
-__global__ __launch_bounds__(kLegThreads) void legendre_ana_kernel(GaussDev g,
-                                                                  const double2* __restrict__ S,
-                                                                  const double2* __restrict__ D,
-                                                                  int lmax_a, int mmax_a,
-                                                                  Epi ep) {
-    constexpr int NF = Epi::NF;
-    extern __shared__ double2 lds[];
-    const int m = blockIdx.x;
-    const int L = g.L, nh = g.nh;
-    const int K = L - m + 1;
-    double2* sS = lds;              // [NF][nh]
-    double2* sD = lds + NF * nh;    // [NF][nh]
-    const bool active = (m <= mmax_a);
-    if (active) {
-        const size_t plane = (size_t)nh * (L + 1);
-        for (int t = threadIdx.x; t < NF * nh; t += kLegThreads) {
-            const int f = t / nh, jp = t - f * nh;
-            const size_t src = f * plane + (size_t)jp * (L + 1) + m;
-            sS[t] = S[src];
-            sD[t] = D[src];
-        }
-    }
-    __syncthreads();
-    for (int l = threadIdx.x; l <= L; l += kLegThreads) {
-        double2 acc[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) acc[f] = c2(0.0, 0.0);
-        if (active && l >= m && l <= lmax_a) {
-            const double* P = g.Pa + g.offs[m] + (l - m);
-            const double2* X = ((l - m) & 1) ? sD : sS;
-            for (int jp = 0; jp < nh; ++jp) {
-                const double p = P[(size_t)jp * K];
-#pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    const double2 x = X[f * nh + jp];
-                    acc[f].x = fma(p, x.x, acc[f].x);
-                    acc[f].y = fma(p, x.y, acc[f].y);
-                }
-            }
-        }
-        ep.apply(l, m, acc);
     }
 }
 
@@ -285,8 +231,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
 
 // Jacobian: fields (a, S a, b, S b) -> J = (a_s b_l - a_l b_s)/c^2 -> S/D.
 __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g,
-                                                               const double2* __restrict__ E,
-                                                               const double2* __restrict__ O,
+                                                               const double2* E,
+                                                               const double2* O,
                                                                double R, int mmax,
                                                                double2* __restrict__ S,
                                                                double2* __restrict__ D) {
@@ -296,6 +242,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g,
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t plane = (size_t)nh * (L + 1);
     const size_t roff = (size_t)jp * (L + 1);
+    E += (size_t)blockIdx.y * 4 * plane;       // member blockIdx.y
+    O += (size_t)blockIdx.y * 4 * plane;
+    S += (size_t)blockIdx.y * plane;
+    D += (size_t)blockIdx.y * plane;
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
         const size_t off = (size_t)r * plane + roff;
@@ -323,14 +273,18 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g,
                                                                const double2* __restrict__ E,
                                                                const double2* __restrict__ O,
                                                                double R, int mmax,
-                                                               double2* __restrict__ S,
-                                                               double2* __restrict__ D) {
+                                                               double2* S,
+                                                               double2* D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
     const int n = g.nlon, L = g.L, nh = g.nh;
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t plane = (size_t)nh * (L + 1);
     const size_t roff = (size_t)jp * (L + 1);
+    E += (size_t)blockIdx.y * 10 * plane;      // member blockIdx.y
+    O += (size_t)blockIdx.y * 10 * plane;
+    S += (size_t)blockIdx.y * 4 * plane;
+    D += (size_t)blockIdx.y * 4 * plane;
     // grid row r <- (Legendre field, spectral factor)
     constexpr int src[12] = {0, 1, 2, 3, 4, 5, 4, 6, 7, 6, 8, 9};
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
@@ -370,29 +324,56 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g,
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static size_t syn_lds(int L, int NF) {
-    const int a = NF * (L + 1), b = 2 * NF * 64;
-    return sizeof(double2) * (a > b ? a : b);
+static ColK col_consts(const aeo_plan_s* p, double R, const double* lap, double f10) {
+    return ColK{p->L, R, p->d_Cp, p->d_Cm, lap, f10};
 }
-static size_t ana_lds(int nh, int NF) { return sizeof(double2) * 2 * NF * nh; }
 
-template <class Loader>
-static int launch_syn(const aeo_plan_s* p, const Loader& ld, double2* E, double2* O,
-                      hipStream_t s) {
-    dim3 grid(p->L + 1, (p->nh + 63) / 64);
-    hipLaunchKernelGGL(legendre_syn_kernel<Loader>, grid, dim3(kLegThreads),
-                       syn_lds(p->L, Loader::NF), s, p->gdev(), ld, E, O);
+template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
+static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& ep,
+                           hipStream_t s) {
+    const size_t lds = StageLds<KS, Epi, MB, NS>::bytes(p->L, p->nh, ANA, SYN);
+    AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
+                "Legendre stage kernel: LDS demand exceeds 160 KiB (reduce members per block)");
+    auto kern = leg_stage_kernel<KS, Epi, MB, ANA, SYN, NS>;
+    if (lds > 65536) {
+        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    dim3 grid(p->L + 1, SYN ? (p->nh + 63) / 64 : 1, (a.nmem + MB - 1) / MB);
+    hipLaunchKernelGGL(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
 
-template <class Epi>
-static int launch_ana(const aeo_plan_s* p, const double2* S, const double2* D, int lmax_a,
-                      int mmax_a, const Epi& ep, hipStream_t s) {
-    hipLaunchKernelGGL(legendre_ana_kernel<Epi>, dim3(p->L + 1), dim3(kLegThreads),
-                       ana_lds(p->nh, Epi::NF), s, p->gdev(), S, D, lmax_a, mmax_a, ep);
-    AEO_LAUNCH_CHECK();
-    return AEO_OK;
+template <class KS, class Epi, bool ANA, bool SYN, int NS>
+static int launch_stage_ns(const aeo_plan_s* p, int mb, const StageArgs& a, const Epi& ep,
+                           hipStream_t s) {
+    switch (mb) {
+        case 1: return launch_stage_mb<KS, Epi, 1, ANA, SYN, NS>(p, a, ep, s);
+        case 2: return launch_stage_mb<KS, Epi, 2, ANA, SYN, NS>(p, a, ep, s);
+        case 4: return launch_stage_mb<KS, Epi, 4, ANA, SYN, NS>(p, a, ep, s);
+        default: return launch_stage_mb<KS, Epi, 8, ANA, SYN, NS>(p, a, ep, s);
+    }
+}
+
+// ns = 0: FP64 contractions; ns in {4, 5, 6, 7}: split digits (plan tables must match).
+template <class KS, class Epi, bool ANA, bool SYN>
+static int launch_stage(const aeo_plan_s* p, int mb, const StageArgs& a, const Epi& ep,
+                        hipStream_t s, int ns = 0) {
+    if (ns != 0) {
+        AEO_REQUIRE(p->nslices == ns, AEO_ERR_STATE,
+                    "split-precision stage: the plan's digit tables have a different slice count");
+    }
+    switch (ns) {
+        case 0: return launch_stage_ns<KS, Epi, ANA, SYN, 0>(p, mb, a, ep, s);
+        case 4: return launch_stage_ns<KS, Epi, ANA, SYN, 4>(p, mb, a, ep, s);
+        case 5: return launch_stage_ns<KS, Epi, ANA, SYN, 5>(p, mb, a, ep, s);
+        case 6: return launch_stage_ns<KS, Epi, ANA, SYN, 6>(p, mb, a, ep, s);
+        case 7: return launch_stage_ns<KS, Epi, ANA, SYN, 7>(p, mb, a, ep, s);
+        default:
+            set_error("split-precision stage: nslices must be 4, 5, 6 or 7");
+            return AEO_ERR_UNSUPPORTED;
+    }
 }
 
 int gauss_build_tables(aeo_plan_s* p, const double* d_x, hipStream_t s) {
@@ -403,22 +384,38 @@ int gauss_build_tables(aeo_plan_s* p, const double* d_x, hipStream_t s) {
     return AEO_OK;
 }
 
+int gauss_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s) {
+    const int nhp = rup(p->nh, 16), nhk = rup(p->nh, 32);
+    const int rows = (nhp > rup(split_kp(p->L, 0), 16)) ? nhp : rup(split_kp(p->L, 0), 16);
+    aeo::GaussDev g = p->gdev();
+    dim3 grid(p->L + 1, 2, (rows + 63) / 64);
+    hipLaunchKernelGGL(split_table_kernel, grid, dim3(64), 0, s, g, ns, p->d_Psl, p->d_esyn,
+                       p->d_Pasl, p->d_eana, nhp, nhk);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+static size_t row_lds(const aeo_plan_s* p) {
+    return sizeof(double2) * (size_t)p->nlon * ((kRowElems / p->nlon > 0) ? kRowElems / p->nlon : 1);
+}
+
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
     const int L = p->L;
     const size_t fs = (size_t)(L + 1) * (L + 1);
     for (int f0 = 0; f0 < nf; f0 += p->max_fields) {
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
-        for (int g0 = 0; g0 < nb; g0 += RawLoader::NF) {
-            RawLoader ld{a + (size_t)(f0 + g0) * fs, L,
-                         (nb - g0 < RawLoader::NF) ? nb - g0 : RawLoader::NF};
-            const size_t plane = (size_t)p->nh * (L + 1);
-            int rc = launch_syn(p, ld, p->d_E + g0 * plane, p->d_O + g0 * plane, s);
-            if (rc) return rc;
-        }
-        const size_t lds = sizeof(double2) * (size_t)p->nlon *
-                           ((kRowElems / p->nlon > 0) ? kRowElems / p->nlon : 1);
-        hipLaunchKernelGGL(grid_syn_kernel, dim3(p->nh), dim3(kGridThreads), lds, s, p->gdev(),
-                           p->d_E, p->d_O, nb, f + (size_t)f0 * p->nlat * p->nlon);
+        StageArgs sa{};
+        sa.ys = a + (size_t)f0 * fs;
+        sa.ys_stride = fs;
+        sa.nmem = nb;
+        sa.nstate_valid = 1;
+        sa.E = p->d_E;
+        sa.O = p->d_O;
+        sa.k = col_consts(p, 1.0, nullptr, 0.0);
+        int rc = launch_stage<KindRaw, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(grid_syn_kernel, dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+                           p->gdev(), p->d_E, p->d_O, nb, f + (size_t)f0 * p->nlat * p->nlon);
         AEO_LAUNCH_CHECK();
     }
     return AEO_OK;
@@ -427,20 +424,22 @@ int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStrea
 int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s) {
     const int L = p->L;
     const size_t fs = (size_t)(L + 1) * (L + 1);
-    const size_t plane = (size_t)p->nh * (L + 1);
     for (int f0 = 0; f0 < nf; f0 += p->max_fields) {
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
-        const size_t lds = sizeof(double2) * (size_t)p->nlon *
-                           ((kRowElems / p->nlon > 0) ? kRowElems / p->nlon : 1);
-        hipLaunchKernelGGL(grid_ana_kernel, dim3(p->nh), dim3(kGridThreads), lds, s, p->gdev(),
-                           f + (size_t)f0 * p->nlat * p->nlon, nb, L, p->d_S, p->d_D);
+        hipLaunchKernelGGL(grid_ana_kernel, dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+                           p->gdev(), f + (size_t)f0 * p->nlat * p->nlon, nb, L, p->d_S, p->d_D);
         AEO_LAUNCH_CHECK();
-        for (int g0 = 0; g0 < nb; g0 += EpiStore::NF) {
-            EpiStore ep{a + (size_t)(f0 + g0) * fs, L,
-                        (nb - g0 < EpiStore::NF) ? nb - g0 : EpiStore::NF};
-            int rc = launch_ana(p, p->d_S + g0 * plane, p->d_D + g0 * plane, L, L, ep, s);
-            if (rc) return rc;
-        }
+        StageArgs sa{};
+        sa.nmem = nb;
+        sa.nstate_valid = 0;
+        sa.S = p->d_S;
+        sa.D = p->d_D;
+        sa.lmax_a = L;
+        sa.mmax_a = L;
+        sa.k = col_consts(p, 1.0, nullptr, 0.0);
+        EpiStore1S ep{a + (size_t)f0 * fs, L, fs};
+        int rc = launch_stage<KindNone, EpiStore1S, true, false>(p, 1, sa, ep, s);
+        if (rc) return rc;
     }
     return AEO_OK;
 }
@@ -448,68 +447,118 @@ int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream
 int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,
                    int dealias, hipStream_t s) {
     const int L = p->L;
-    SpecK k{L, R, p->d_Cp, p->d_Cm};
-    JacLoader<false> ld{a, b, false, nullptr, 0.0, k};
-    int rc = launch_syn(p, ld, p->d_E, p->d_O, s);
+    const size_t fs = (size_t)(L + 1) * (L + 1);
+    AEO_CHECK_HIP(hipMemcpyAsync(p->d_spec, a, fs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    AEO_CHECK_HIP(hipMemcpyAsync(p->d_spec + fs, b, fs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    StageArgs sa{};
+    sa.ys = p->d_spec;
+    sa.ys_stride = 2 * fs;
+    sa.nmem = 1;
+    sa.nstate_valid = 2;
+    sa.E = p->d_E;
+    sa.O = p->d_O;
+    sa.k = col_consts(p, R, nullptr, 0.0);
+    int rc = launch_stage<KindJac, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
-    hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh), dim3(kGridThreads),
-                       sizeof(double2) * 4 * p->nlon, s, p->gdev(), p->d_E, p->d_O, R,
-                       lim, p->d_S, p->d_D);
+    hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, 1), dim3(kGridThreads),
+                       sizeof(double2) * 4 * p->nlon, s, p->gdev(), p->d_E, p->d_O, R, lim,
+                       p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
-    EpiStore1 ep{out, L};
-    return launch_ana(p, p->d_S, p->d_D, lim, lim, ep, s);
+    StageArgs sb{};
+    sb.nmem = 1;
+    sb.nstate_valid = 0;
+    sb.S = p->d_S;
+    sb.D = p->d_D;
+    sb.lmax_a = lim;
+    sb.mmax_a = lim;
+    sb.k = sa.k;
+    EpiStore1S ep{out, L, fs};
+    return launch_stage<KindNone, EpiStore1S, true, false>(p, 1, sb, ep, s);
+}
+
+int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
+    aeo_plan_s* p = md->plan;
+    const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
+    const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
+    StageArgs sa{};
+    sa.ys = y;
+    sa.ys_stride = fs * md->nstate;
+    sa.nmem = md->nmem;
+    sa.nstate_valid = md->nstate;
+    sa.E = md->d_E;
+    sa.O = md->d_O;
+    sa.k = col_consts(p, md->R, md->d_lap, f10);
+    ProfRec& pr = md->prof;
+    pr.begin(kClsSynthesis, s);
+    int rc = (md->kind == kBVE)
+                 ? launch_stage<KindBve, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
+                                                                 md->nslices)
+                 : launch_stage<KindSwe, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
+                                                                 md->nslices);
+    pr.end(kClsSynthesis, s);
+    return rc;
+}
+
+template <class KS, class Epi>
+static int stage_ana(aeo_model_s* md, const StageArgs& sa, const Epi& ep, bool syn_next,
+                     hipStream_t s) {
+    return syn_next
+               ? launch_stage<KS, Epi, true, true>(md->plan, md->mb, sa, ep, s, md->nslices)
+               : launch_stage<KS, Epi, true, false>(md->plan, md->mb, sa, ep, s, md->nslices);
 }
 
 int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
-                      double2* ynext, hipStream_t s) {
+                      double2* ynext, bool syn_next, hipStream_t s) {
     aeo_plan_s* p = md->plan;
     const int L = p->L;
     const int cut = (2 * L) / 3;
-    SpecK k{L, md->R, p->d_Cp, p->d_Cm};
+    const size_t fs = (size_t)(L + 1) * (L + 1);
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
-    Rk4 rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
     ProfRec& pr = md->prof;
+    pr.begin(kClsGrid, s);
     if (md->kind == kBVE) {
-        JacLoader<false> ld{ys, nullptr, true, md->d_lap, f10, k};
-        pr.begin(kClsSynthesis, s);
-        int rc = launch_syn(p, ld, md->d_E, md->d_O, s);
-        pr.end(kClsSynthesis, s);
-        if (rc) return rc;
-        pr.begin(kClsGrid, s);
-        hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh), dim3(kGridThreads),
+        hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads),
                            sizeof(double2) * 4 * p->nlon, s, p->gdev(), md->d_E, md->d_O,
                            md->R, cut, md->d_S, md->d_D);
-        pr.end(kClsGrid, s);
-        AEO_LAUNCH_CHECK();
-        EpiBve ep{L, ys, forcing, md->d_nulap, rk};
-        pr.begin(kClsAnalysis, s);
-        rc = launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
-        pr.end(kClsAnalysis, s);
-        return rc;
+    } else {
+        const size_t swe_lds = sizeof(double2) * 12 * p->nlon;
+        if (swe_lds > 65536) {
+            AEO_CHECK_HIP(hipFuncSetAttribute((const void*)grid_swe_kernel,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)swe_lds));
+        }
+        hipLaunchKernelGGL(grid_swe_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds,
+                           s, p->gdev(), md->d_E, md->d_O, md->R, cut, md->d_S, md->d_D);
     }
-    SweLoader<false> ld{ys, md->d_lap, f10, k};
-    pr.begin(kClsSynthesis, s);
-    int rc = launch_syn(p, ld, md->d_E, md->d_O, s);
-    pr.end(kClsSynthesis, s);
-    if (rc) return rc;
-    const size_t swe_lds = sizeof(double2) * 12 * p->nlon;
-    if (swe_lds > 65536) {
-        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)grid_swe_kernel,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)swe_lds));
-    }
-    pr.begin(kClsGrid, s);
-    hipLaunchKernelGGL(grid_swe_kernel, dim3(p->nh), dim3(kGridThreads), swe_lds, s, p->gdev(), md->d_E, md->d_O,
-                       md->R, cut, md->d_S, md->d_D);
     pr.end(kClsGrid, s);
     AEO_LAUNCH_CHECK();
-    EpiSwe ep{L, ys, md->d_lap, md->nu4 > 0.0 ? md->d_nu4lap2 : nullptr,
-              md->has_topo ? md->d_lap_phi_s : nullptr,
-              (md->has_topo && md->nu4 > 0.0) ? md->d_nu4_lap2_phi_s : nullptr, md->phi0, rk};
+    StageArgs sa{};
+    sa.ys = ys;
+    sa.ys_stride = fs * md->nstate;
+    sa.nmem = md->nmem;
+    sa.nstate_valid = md->nstate;
+    sa.S = md->d_S;
+    sa.D = md->d_D;
+    sa.lmax_a = cut;
+    sa.mmax_a = cut;
+    sa.E = md->d_E;
+    sa.O = md->d_O;
+    sa.k = col_consts(p, md->R, md->d_lap, f10);
+    Rk4B rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
+    int rc;
     pr.begin(kClsAnalysis, s);
-    rc = launch_ana(p, md->d_S, md->d_D, cut, cut, ep, s);
+    if (md->kind == kBVE) {
+        EpiBveS ep{L, fs, forcing, md->d_nulap, rk};
+        rc = stage_ana<KindBve>(md, sa, ep, syn_next, s);
+    } else {
+        EpiSweS ep{L, 3 * fs, md->d_lap, md->nu4 > 0.0 ? md->d_nu4lap2 : nullptr,
+                   md->has_topo ? md->d_lap_phi_s : nullptr,
+                   (md->has_topo && md->nu4 > 0.0) ? md->d_nu4_lap2_phi_s : nullptr, md->phi0,
+                   rk};
+        rc = stage_ana<KindSwe>(md, sa, ep, syn_next, s);
+    }
     pr.end(kClsAnalysis, s);
     return rc;
 }

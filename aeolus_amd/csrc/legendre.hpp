@@ -1,0 +1,636 @@
+// Per-m Legendre stage kernel for Gauss grids (FP64), batched over members.
+//
+// One workgroup owns one zonal wavenumber m (grid.x), one 64-latitude tile
+// (grid.y) and a group of MB members (grid.z).  Depending on ANA / SYN it
+//   ANA: contracts the analysis table block Pa[m] with the weighted Fourier
+//        coefficients S/D of column m (lanes over degree l, the four waves
+//        over latitude quarters, reduced in LDS), then runs the epilogue for
+//        every (l, m) of the column: store, tendency, RK4 stage update.  The
+//        epilogue leaves the new stage state of column m in LDS.
+//   SYN: derives the synthesis fields of column m from the state column in
+//        LDS (psi, eta, the S recurrence, ...), and contracts them with the
+//        synthesis table block Ps[m] (lanes over latitude, the four waves over
+//        l - m mod 4, so waves 0/2 hold the even and 1/3 the odd parity), giving
+//        E/O = even/odd partial sums per latitude.
+// With both set, the analysis of RK4 stage s and the synthesis of stage s+1
+// run in one launch: column m of the new stage state never leaves the CU.
+// The analysis part is recomputed by every latitude tile of a column (only
+// tile 0 writes the epilogue's global outputs).
+#pragma once
+
+#include "plan.hpp"
+#include "split.hpp"
+#include "stage.hpp"
+
+namespace aeo {
+
+constexpr int kStageThreads = 256;
+
+struct ColK {
+    int L;
+    double R;
+    const double* Cp;   // (L+1)^2
+    const double* Cm;   // (L+1)^2
+    const double* lap;  // -l(l+1)/R^2 (l = 0 entry 0)
+    double f10;         // f at (1, 0)
+};
+
+// ---------------------------------------------------------------------------
+// Synthesis kinds: which fields a column contributes.  SF[f] >= 0 marks field
+// f as S applied to field SF[f]; other fields come from base().
+// st points at the member's state columns, NSTATE x (L+1) complex.
+// ---------------------------------------------------------------------------
+struct KindNone {
+    static constexpr int NSTATE = 1, NF = 1;
+    static constexpr int SF[1] = {-1};
+    __device__ static double2 base(int, const double2*, int, int, const ColK&) {
+        return c2(0.0, 0.0);
+    }
+};
+
+// Raw field (generic synthesis: one member per field).
+struct KindRaw {
+    static constexpr int NSTATE = 1, NF = 1;
+    static constexpr int SF[1] = {-1};
+    __device__ static double2 base(int, const double2* st, int l, int, const ColK&) {
+        return st[l];
+    }
+};
+
+// Jacobian of two given fields: a, S a, b, S b.
+struct KindJac {
+    static constexpr int NSTATE = 2, NF = 4;
+    static constexpr int SF[4] = {-1, 0, -1, 2};
+    __device__ static double2 base(int f, const double2* st, int l, int, const ColK& k) {
+        return st[(f >> 1) * (k.L + 1) + l];
+    }
+};
+
+// BVE: psi = zeta / lap (l >= 1), S psi, eta = zeta + f, S eta.
+struct KindBve {
+    static constexpr int NSTATE = 1, NF = 4;
+    static constexpr int SF[4] = {-1, 0, -1, 2};
+    __device__ static double2 base(int f, const double2* st, int l, int m, const ColK& k) {
+        double2 z = st[l];
+        if (f == 0) {
+            if (l == 0) return c2(0.0, 0.0);
+            const double d = k.lap[l];
+            return c2(z.x / d, z.y / d);
+        }
+        if (l == 1 && m == 0) z.x += k.f10;
+        return z;
+    }
+};
+
+// SWE: psi, S psi, chi, S chi, eta, S eta, phi, S phi, zeta, delta.
+struct KindSwe {
+    static constexpr int NSTATE = 3, NF = 10;
+    static constexpr int SF[10] = {-1, 0, -1, 2, -1, 4, -1, 6, -1, -1};
+    __device__ static double2 base(int f, const double2* st, int l, int m, const ColK& k) {
+        const int n = k.L + 1;
+        switch (f) {
+            case 0:
+            case 2: {
+                if (l == 0) return c2(0.0, 0.0);
+                const double2 z = st[(f == 0 ? 0 : n) + l];
+                const double d = k.lap[l];
+                return c2(z.x / d, z.y / d);
+            }
+            case 4: {
+                double2 z = st[l];
+                if (l == 1 && m == 0) z.x += k.f10;
+                return z;
+            }
+            case 6: return st[2 * n + l];
+            case 8: return st[l];
+            default: return st[n + l];
+        }
+    }
+};
+
+// Fill cols[f][i] (i = l - m) for the member's state column st.
+template <class KS>
+__device__ inline void derive_columns(const double2* st, int m, const ColK& k, double2* cols) {
+    const int K = k.L - m + 1;
+    const int n = k.L + 1;
+    for (int t = threadIdx.x; t < KS::NF * K; t += kStageThreads) {
+        const int f = t / K, i = t - f * K;
+        if (KS::SF[f] < 0) cols[t] = KS::base(f, st, m + i, m, k);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < KS::NF * K; t += kStageThreads) {
+        const int f = t / K, i = t - f * K;
+        const int src = KS::SF[f];
+        if (src >= 0) {
+            const int l = m + i;
+            const double2* s = cols + src * K;
+            double2 v = c2(0.0, 0.0);
+            if (i >= 1) v = cadd(v, cscale(s[i - 1], k.Cp[(l - 1) * n + m]));
+            if (l <= k.L - 1) v = cadd(v, cscale(s[i + 1], k.Cm[(l + 1) * n + m]));
+            cols[t] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Stage epilogues.  apply(b, l, m, acc, st, writer): b = member, acc = analysed
+// values (zero outside the band), st = the member's state columns in LDS
+// (read the stage input, overwrite with the next stage state), writer =
+// whether this workgroup writes the global outputs.
+// ---------------------------------------------------------------------------
+struct EpiNoneS {
+    static constexpr int NA = 1, NSTATE = 1;
+    __device__ void apply(int, int, int, const double2*, double2*, bool) const {}
+};
+
+// Store the analysed field (generic analysis: one member per field; Jacobian).
+struct EpiStore1S {
+    static constexpr int NA = 1, NSTATE = 1;
+    double2* out;
+    int L;
+    size_t member_stride;
+    __device__ void apply(int b, int l, int m, const double2* acc, double2*, bool writer) const {
+        if (writer) out[b * member_stride + l * (L + 1) + m] = acc[0];
+    }
+};
+
+// RK4 stage bookkeeping on one entry (see Rk4 in stage.hpp); returns the
+// next stage state (or k for the tendency-only mode).
+struct Rk4B {
+    int stage;      // -1: tendency only
+    double cst;
+    double dt6;
+    const double2* y0;
+    double2* ksum;
+    double2* ynext;
+    double2* kout;
+    __device__ double2 apply(size_t idx, double2 k, bool writer) const {
+        if (stage < 0) {
+            if (writer) kout[idx] = k;
+            return k;
+        }
+        const double2 y = y0[idx];
+        double2 nx;
+        if (stage == 0) {
+            if (writer) ksum[idx] = k;
+            nx = cadd(y, cscale(k, cst));
+        } else if (stage < 3) {
+            // only the writer tile touches ksum here: the other latitude tiles
+            // of the column must not read an entry the writer is updating
+            if (writer) ksum[idx] = cadd(ksum[idx], cscale(k, 2.0));
+            nx = cadd(y, cscale(k, cst));
+        } else {
+            const double2 s = cadd(ksum[idx], k);
+            nx = cadd(y, cscale(s, dt6));
+        }
+        if (writer) ynext[idx] = nx;
+        return nx;
+    }
+};
+
+// BVE: k = -J + nu lap' zeta + forcing, row 0 pinned.
+struct EpiBveS {
+    static constexpr int NA = 1, NSTATE = 1;
+    int L;
+    size_t member_stride;
+    const double2* forcing;  // per member, or nullptr
+    const double* nulap;
+    Rk4B rk;
+    __device__ void apply(int b, int l, int m, const double2* acc, double2* st,
+                          bool writer) const {
+        const size_t idx = b * member_stride + (size_t)l * (L + 1) + m;
+        double2 k = c2(0.0, 0.0);
+        if (l > 0) {
+            k = cadd(c2(-acc[0].x, -acc[0].y), cscale(st[l], nulap[l]));
+            if (forcing) k = cadd(k, forcing[idx]);
+        }
+        st[l] = rk.apply(idx, k, writer);
+    }
+};
+
+// SWE assembly (phy/shallow_water.py:389-418).
+struct EpiSweS {
+    static constexpr int NA = 4, NSTATE = 3;
+    int L;
+    size_t member_stride;
+    const double* lap;
+    const double* nu4lap2;
+    const double2* lap_phi_s;
+    const double2* nu4_lap2_phi_s;
+    double phi0;
+    Rk4B rk;
+    __device__ void apply(int b, int l, int m, const double2* acc, double2* st,
+                          bool writer) const {
+        const int n = L + 1;
+        const size_t fs = (size_t)n * n;
+        const size_t e = (size_t)l * n + m;
+        const size_t base = b * member_stride;
+        double2 kz = c2(0.0, 0.0), kd = c2(0.0, 0.0), kp = c2(0.0, 0.0);
+        if (l > 0) {
+            const double2 zeta = st[l], delta = st[n + l], phi = st[2 * n + l];
+            kz = acc[0];
+            kp = csub(acc[3], cscale(delta, phi0));
+            kd = csub(acc[1], cscale(cadd(acc[2], phi), lap[l]));
+            if (lap_phi_s) kd = csub(kd, lap_phi_s[e]);
+            if (nu4lap2) {
+                const double q = nu4lap2[l];
+                kz = csub(kz, cscale(zeta, q));
+                kd = csub(kd, cscale(delta, q));
+                kp = csub(kp, cscale(phi, q));
+                if (nu4_lap2_phi_s) kp = csub(kp, nu4_lap2_phi_s[e]);
+            }
+        }
+        st[l] = rk.apply(base + e, kz, writer);
+        st[n + l] = rk.apply(base + fs + e, kd, writer);
+        st[2 * n + l] = rk.apply(base + 2 * fs + e, kp, writer);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// The stage kernel.
+// ---------------------------------------------------------------------------
+struct StageArgs {
+    const double2* ys;      // stage input states [member][NSTATE][n][n]
+    size_t ys_stride;       // complex entries per member
+    int nmem;
+    int nstate_valid;       // state fields present in ys (<= NSTATE)
+    const double2* S;       // [member][NA][nh][n]
+    const double2* D;
+    int lmax_a, mmax_a;
+    double2* E;             // [member][NF][nh][n]
+    double2* O;
+    ColK k;
+};
+
+template <class KS, class Epi, int MB, int NS>
+struct StageLds {
+    static constexpr int NST = (KS::NSTATE > Epi::NSTATE) ? KS::NSTATE : Epi::NSTATE;
+    static constexpr int NCA = MB * Epi::NA * 2, NTA = (NCA + 15) / 16;
+    static constexpr int NCS = MB * KS::NF * 2, NTS = (NCS + 15) / 16;
+    static size_t bytes(int L, int nh, bool ana, bool syn) {
+        const int n = L + 1;
+        size_t s = sizeof(double2) * (size_t)MB * NST * n;     // state columns
+        size_t a = 0, c = 0;
+        if (ana) {
+            a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) + 4 * MB * Epi::NA * 64);
+            if (NS > 0) a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n)) +
+                            sizeof(int) * 2 * NTA * 16 + 2 * sizeof(__bf16) * NS * NTA * 16 * rup(nh, 32) + 64;
+        }
+        if (syn) {
+            c = sizeof(double2) * ((size_t)MB * KS::NF * n + 2 * (size_t)MB * KS::NF * 64);
+            if (NS > 0) c = sizeof(double2) * ((size_t)MB * KS::NF * n) + sizeof(int) * NTS * 16 +
+                            2 * sizeof(__bf16) * NS * NTS * 16 * rup(split_kp(L, 0), 32) + 64;
+        }
+        return s + (a > c ? a : c);
+    }
+};
+
+__device__ inline double cpart(double2 v, int ri) { return ri ? v.y : v.x; }
+
+// 16-byte aligned pointer at or after p
+template <class T>
+__device__ inline T* align16(void* p) {
+    return reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+}
+
+template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
+__global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, StageArgs a,
+                                                                  Epi ep) {
+    constexpr int NST = StageLds<KS, Epi, MB, NS>::NST;
+    constexpr int NA = Epi::NA;
+    constexpr int NF = KS::NF;
+    extern __shared__ double2 lds[];
+    const int m = blockIdx.x;
+    const int jt = blockIdx.y;
+    const int b0 = blockIdx.z * MB;
+    const int L = g.L, nh = g.nh, n = L + 1;
+    const int K = L - m + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double2* st = lds;                                  // [MB][NST][n]
+    double2* work = lds + (size_t)MB * NST * n;
+
+    // -- state columns of this m --------------------------------------------
+    for (int t = threadIdx.x; t < MB * NST * n; t += kStageThreads) {
+        const int bb = t / (NST * n);
+        const int r = t - bb * NST * n;
+        const int s = r / n, l = r - s * n;
+        const int b = b0 + bb;
+        double2 v = c2(0.0, 0.0);
+        if (b < a.nmem && s < a.nstate_valid) v = a.ys[b * a.ys_stride + (size_t)s * n * n + l * n + m];
+        st[t] = v;
+    }
+    __syncthreads();
+
+    if constexpr (ANA) {
+        double2* X = work;                                  // [MB][NA][2][nh]  (S then D)
+        double2* accl = X + (size_t)MB * NA * 2 * nh;       // [MB][NA][n]
+        double2* red = accl + (size_t)MB * NA * n;          // [4][MB][NA][64]
+        const bool active = (m <= a.mmax_a);
+        if (active) {
+            const size_t plane = (size_t)nh * n;
+            for (int t = threadIdx.x; t < MB * NA * nh; t += kStageThreads) {
+                const int bb = t / (NA * nh);
+                const int r = t - bb * NA * nh;
+                const int f = r / nh, jp = r - f * nh;
+                const int b = b0 + bb;
+                double2 sv = c2(0.0, 0.0), dv = c2(0.0, 0.0);
+                if (b < a.nmem) {
+                    const size_t src = ((size_t)b * NA + f) * plane + (size_t)jp * n + m;
+                    sv = a.S[src];
+                    dv = a.D[src];
+                }
+                X[((bb * NA + f) * 2 + 0) * nh + jp] = sv;
+                X[((bb * NA + f) * 2 + 1) * nh + jp] = dv;
+            }
+        }
+        for (int t = threadIdx.x; t < MB * NA * n; t += kStageThreads) accl[t] = c2(0.0, 0.0);
+        __syncthreads();
+        if constexpr (NS > 0) if (active) {
+            // ---- split-precision analysis: BF16 digit MFMA, exact FP32, FP64 recombination
+            constexpr int NCA = StageLds<KS, Epi, MB, NS>::NCA;
+            constexpr int NTA = StageLds<KS, Epi, MB, NS>::NTA;
+            constexpr int NCP = NTA * 16;
+            const int top = (a.lmax_a < L) ? a.lmax_a : L;
+            const int nhk = rup(nh, 32);
+            const int kp = split_kp(L, m), mpad = rup(kp, 16);
+            int* gexp = reinterpret_cast<int*>(accl + (size_t)MB * NA * n);   // [2][NCP]
+            __bf16* xs = align16<__bf16>(gexp + 2 * NCP);                      // [2][NS][NCP][nhk]
+            for (int t = threadIdx.x; t < 2 * NCP; t += kStageThreads) {
+                const int p = t / NCP, c = t - p * NCP;
+                double mx = 0.0;
+                if (c < NCA) {
+                    const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                    const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
+                    for (int j = 0; j < nh; ++j) mx = fmax(mx, fabs(cpart(xr[j], ri)));
+                }
+                gexp[t] = exp_above(mx);
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < 2 * NCP * nhk; t += kStageThreads) {
+                const int p = t / (NCP * nhk);
+                const int r2 = t - p * NCP * nhk;
+                const int c = r2 / nhk, j = r2 - c * nhk;
+                double v = 0.0;
+                if (c < NCA && j < nh) {
+                    const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                    v = cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
+                }
+                float d[NS];
+                digits<NS>(v, gexp[p * NCP + c], d);
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    xs[((size_t)(p * NS + q) * NCP + c) * nhk + j] = (__bf16)d[q];
+            }
+            __syncthreads();
+            const __bf16* Pt = g.Pasl + g.offs_sa[m];
+            const int* er = g.eana + g.offs_ea[m];
+            double* accd = reinterpret_cast<double*>(accl);
+            for (int p = 0; p < 2; ++p) {
+                const int Mv = (top >= m + p) ? (top - m - p) / 2 + 1 : 0;
+                const int MT = (Mv + 15) / 16;
+                for (int mt = wave; mt < MT; mt += 4) {
+                    floatx4 acc[NS][NTA];
+#pragma unroll
+                    for (int q = 0; q < NS; ++q)
+#pragma unroll
+                        for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    for (int k0 = 0; k0 < nhk; k0 += 32) {
+                        bf16x8 av[NS];
+#pragma unroll
+                        for (int q = 0; q < NS; ++q)
+                            av[q] = *reinterpret_cast<const bf16x8*>(
+                                Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) * nhk + k0 +
+                                (lane >> 4) * 8);
+#pragma unroll
+                        for (int nt = 0; nt < NTA; ++nt)
+#pragma unroll
+                            for (int t2 = 0; t2 < NS; ++t2) {
+                                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                    xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * nhk +
+                                    k0 + (lane >> 4) * 8);
+#pragma unroll
+                                for (int q = 0; q + t2 < NS; ++q)
+                                    acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        av[q], bv, acc[q + t2][nt], 0, 0, 0);
+                            }
+                    }
+#pragma unroll
+                    for (int nt = 0; nt < NTA; ++nt) {
+                        const int c = nt * 16 + (lane & 15);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int ir = mt * 16 + (lane >> 4) * 4 + r;
+                            double v = 0.0;
+#pragma unroll
+                            for (int q = NS - 1; q >= 0; --q) v = v * (1.0 / 256.0) + (double)acc[q][nt][r];
+                            if (c < NCA && ir < Mv) {
+                                const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                                const int l = m + 2 * ir + p;
+                                accd[2 * ((size_t)(bb * NA + f) * n + l) + ri] =
+                                    ldexp(v, er[p * mpad + ir] + gexp[p * NCP + c] - 14);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (NS == 0) if (active) {
+            const int top = (a.lmax_a < L) ? a.lmax_a : L;
+            const double* Pa = g.Pa + g.offs[m];
+            for (int l0 = m; l0 <= top; l0 += 64) {
+                const int l = l0 + lane;
+                double2 acc[MB][NA];
+#pragma unroll
+                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                    for (int f = 0; f < NA; ++f) acc[bb][f] = c2(0.0, 0.0);
+                if (l <= top) {
+                    const int par = (l - m) & 1;
+                    const double* P = Pa + (l - m);
+#pragma unroll 4
+                    for (int jp = wave; jp < nh; jp += 4) {
+                        const double p = P[(size_t)jp * K];
+#pragma unroll
+                        for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                            for (int f = 0; f < NA; ++f) {
+                                const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
+                                acc[bb][f].x = fma(p, x.x, acc[bb][f].x);
+                                acc[bb][f].y = fma(p, x.y, acc[bb][f].y);
+                            }
+                    }
+                }
+#pragma unroll
+                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                    for (int f = 0; f < NA; ++f)
+                        red[((wave * MB + bb) * NA + f) * 64 + lane] = acc[bb][f];
+                __syncthreads();
+                for (int t = threadIdx.x; t < MB * NA * 64; t += kStageThreads) {
+                    const int ll = l0 + (t & 63);
+                    if (ll <= top) {
+                        const int q = t >> 6;  // (bb * NA + f)
+                        double2 s = red[(0 * MB * NA + q) * 64 + (t & 63)];
+                        s = cadd(s, red[(1 * MB * NA + q) * 64 + (t & 63)]);
+                        s = cadd(s, red[(2 * MB * NA + q) * 64 + (t & 63)]);
+                        s = cadd(s, red[(3 * MB * NA + q) * 64 + (t & 63)]);
+                        accl[q * n + ll] = s;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // epilogue over every (l, m) of the column and every member
+        const bool writer = (jt == 0);
+        for (int t = threadIdx.x; t < MB * n; t += kStageThreads) {
+            const int bb = t / n, l = t - bb * n;
+            const int b = b0 + bb;
+            if (b >= a.nmem) continue;
+            double2 acc[NA];
+#pragma unroll
+            for (int f = 0; f < NA; ++f) acc[f] = accl[(bb * NA + f) * n + l];
+            ep.apply(b, l, m, acc, st + (size_t)bb * NST * n, writer);
+        }
+        __syncthreads();
+    }
+
+    if constexpr (SYN && NS > 0) {
+        // ---- split-precision synthesis: BF16 digit MFMA, exact FP32, FP64 recombination
+        constexpr int NCS = StageLds<KS, Epi, MB, NS>::NCS;
+        constexpr int NTS = StageLds<KS, Epi, MB, NS>::NTS;
+        constexpr int NCP = NTS * 16;
+        double2* cols = work;                               // [MB][NF][K]
+        for (int bb = 0; bb < MB; ++bb)
+            derive_columns<KS>(st + (size_t)bb * NST * n, m, a.k, cols + (size_t)bb * NF * K);
+        const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
+        int* fexp = reinterpret_cast<int*>(cols + (size_t)MB * NF * n);       // [NCP]
+        __bf16* qs = align16<__bf16>(fexp + NCP);                             // [2][NS][NCP][kpad]
+        for (int c = threadIdx.x; c < NCP; c += kStageThreads) {
+            double mx = 0.0;
+            if (c < NCS) {
+                const double2* cr = cols + (size_t)(c >> 1) * K;
+                for (int i = 0; i < K; ++i) mx = fmax(mx, fabs(cpart(cr[i], c & 1)));
+            }
+            fexp[c] = exp_above(mx);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < 2 * NCP * kpad; t += kStageThreads) {
+            const int p = t / (NCP * kpad);
+            const int r2 = t - p * NCP * kpad;
+            const int c = r2 / kpad, ip = r2 - c * kpad;
+            const int i = 2 * ip + p;
+            const double v = (c < NCS && i < K) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
+            float d[NS];
+            digits<NS>(v, fexp[c], d);
+#pragma unroll
+            for (int q = 0; q < NS; ++q) qs[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = (__bf16)d[q];
+        }
+        __syncthreads();
+        const int j0 = jt * 64 + wave * 16;
+        if (j0 < nh) {
+            const __bf16* Pt = g.Psl + g.offs_ss[m];
+            const int* er = g.esyn + (size_t)m * 2 * nhp;
+            const size_t plane = (size_t)nh * n;
+            for (int p = 0; p < 2; ++p) {
+                floatx4 acc[NS][NTS];
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+#pragma unroll
+                    for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                for (int k0 = 0; k0 < kpad; k0 += 32) {
+                    bf16x8 av[NS];
+#pragma unroll
+                    for (int q = 0; q < NS; ++q)
+                        av[q] = *reinterpret_cast<const bf16x8*>(
+                            Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
+                            (lane >> 4) * 8);
+#pragma unroll
+                    for (int nt = 0; nt < NTS; ++nt)
+#pragma unroll
+                        for (int t2 = 0; t2 < NS; ++t2) {
+                            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * kpad + k0 +
+                                (lane >> 4) * 8);
+#pragma unroll
+                            for (int q = 0; q + t2 < NS; ++q)
+                                acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    av[q], bv, acc[q + t2][nt], 0, 0, 0);
+                        }
+                }
+                double2* dst = (p == 0) ? a.E : a.O;
+#pragma unroll
+                for (int nt = 0; nt < NTS; ++nt) {
+                    const int c = nt * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = j0 + (lane >> 4) * 4 + r;
+                        double v = 0.0;
+#pragma unroll
+                        for (int q = NS - 1; q >= 0; --q) v = v * (1.0 / 256.0) + (double)acc[q][nt][r];
+                        v = ldexp(v, er[p * nhp + j] + fexp[c] - 14);
+                        const double w = __shfl_xor(v, 1);
+                        if (!(c & 1) && c < NCS && j < nh) {
+                            const int bb = c / (NF * 2), f = (c >> 1) % NF;
+                            const int b = b0 + bb;
+                            if (b < a.nmem)
+                                dst[((size_t)b * NF + f) * plane + (size_t)j * n + m] = c2(v, w);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (SYN && NS == 0) {
+        double2* cols = work;                               // [MB][NF][K]
+        double2* red = cols + (size_t)MB * NF * n;          // [2][MB][NF][64]
+        for (int bb = 0; bb < MB; ++bb)
+            derive_columns<KS>(st + (size_t)bb * NST * n, m, a.k, cols + (size_t)bb * NF * K);
+        const int jp = jt * 64 + lane;
+        double2 acc[MB][NF];
+#pragma unroll
+        for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+            for (int f = 0; f < NF; ++f) acc[bb][f] = c2(0.0, 0.0);
+        if (jp < nh) {
+            const double* P = g.Ps + g.offs[m] + jp;
+#pragma unroll 4
+            for (int i = wave; i < K; i += 4) {
+                const double p = P[(size_t)i * nh];
+#pragma unroll
+                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) {
+                        const double2 c = cols[(bb * NF + f) * K + i];
+                        acc[bb][f].x = fma(p, c.x, acc[bb][f].x);
+                        acc[bb][f].y = fma(p, c.y, acc[bb][f].y);
+                    }
+            }
+        }
+        if (wave >= 2) {
+#pragma unroll
+            for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    red[(((wave - 2) * MB + bb) * NF + f) * 64 + lane] = acc[bb][f];
+        }
+        __syncthreads();
+        if (wave < 2 && jp < nh) {
+            double2* dst = (wave == 0) ? a.E : a.O;
+            const size_t plane = (size_t)nh * n;
+#pragma unroll
+            for (int bb = 0; bb < MB; ++bb) {
+                const int b = b0 + bb;
+                if (b >= a.nmem) continue;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    const double2 o = red[((wave * MB + bb) * NF + f) * 64 + lane];
+                    dst[((size_t)b * NF + f) * plane + (size_t)jp * n + m] = cadd(acc[bb][f], o);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace aeo

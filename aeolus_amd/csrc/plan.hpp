@@ -17,6 +17,16 @@ struct GaussDev {
     const double* w_rows;     // nlat solid-angle weights per point of a row
     const double* coslat;     // nlat cos(latitude), unclamped
     FftDev fft;
+    // Split-precision tables (nslices > 0): BF16 integer digits of Pbar with
+    // one power-of-two exponent per table row (see split.hpp).
+    int nslices;
+    const __bf16* Psl;        // synthesis digits, block m: [parity][slice][nhp][kpad_m]
+    const long long* offs_ss; // element offset of block m
+    const int* esyn;          // row exponents, block m: [parity][nhp] at m * 2 * nhp
+    const __bf16* Pasl;       // analysis digits, block m: [parity][slice][mpad_m][nhk]
+    const long long* offs_sa;
+    const int* eana;          // row exponents, block m: [parity][mpad_m] at offs_ea[m]
+    const long long* offs_ea;
 };
 
 // Device view of a point-set plan.
@@ -42,6 +52,14 @@ struct aeo_plan_s {
     double* d_coslat = nullptr;
     double2* d_tw = nullptr;
     aeo::FftDev fft{};
+    int nslices = 0;
+    __bf16* d_Psl = nullptr;
+    long long* d_offs_ss = nullptr;
+    int* d_esyn = nullptr;
+    __bf16* d_Pasl = nullptr;
+    long long* d_offs_sa = nullptr;
+    int* d_eana = nullptr;
+    long long* d_offs_ea = nullptr;
     // Gauss workspaces: [fields][nh][L+1] complex
     double2* d_E = nullptr;
     double2* d_O = nullptr;
@@ -67,6 +85,9 @@ struct aeo_plan_s {
         g.L = L; g.nlat = nlat; g.nlon = nlon; g.nh = nh;
         g.Ps = d_Ps; g.Pa = d_Pa; g.offs = d_offs;
         g.w_rows = d_w_rows; g.coslat = d_coslat; g.fft = fft;
+        g.nslices = nslices;
+        g.Psl = d_Psl; g.offs_ss = d_offs_ss; g.esyn = d_esyn;
+        g.Pasl = d_Pasl; g.offs_sa = d_offs_sa; g.eana = d_eana; g.offs_ea = d_offs_ea;
         return g;
     }
     aeo::PointsDev pdev() const {
@@ -121,8 +142,12 @@ struct aeo_model_s {
     double2* d_D = nullptr;
     double* d_gw = nullptr;
     double2* d_part = nullptr;
-    double2* d_ystage = nullptr;
+    double2* d_ystage = nullptr;   // 2 slots (ping-pong between stages)
+    double2* d_ybuf = nullptr;     // 2 slots (ping-pong between steps)
     double2* d_ksum = nullptr;
+    int nmem = 1;                  // members (independent states) in the group
+    int mb = 1;                    // members per workgroup in the Legendre stage kernel
+    int nslices = 0;               // 0: FP64 contractions; > 0: split BF16 MFMA digits
     ProfRec prof;
 };
 
@@ -130,13 +155,19 @@ namespace aeo {
 
 // Launchers implemented in gauss.hip / points.hip / spectral.hip.
 int gauss_build_tables(aeo_plan_s* p, const double* h_x, hipStream_t s);
+int gauss_build_split_tables(aeo_plan_s* p, int nslices, hipStream_t s);
 int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
 int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,
                    int dealias, hipStream_t s);
+// Synthesis of the model's Legendre fields from states y (all members) into
+// the model's E/O workspace (first stage of a run, or a tendency call).
+int gauss_model_prime(aeo_model_s* m, const double2* y, hipStream_t s);
+// Fourier stage + analysis/epilogue of one RK4 stage (stage -1: tendency into
+// kout); with syn_next the same launch also synthesises the next stage.
 int gauss_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
-                      double2* ynext, hipStream_t s);
+                      double2* ynext, bool syn_next, hipStream_t s);
 
 int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s);
 int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);

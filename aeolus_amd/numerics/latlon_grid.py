@@ -123,3 +123,8 @@ class GaussLatLonSphericalHarmonics(_TransformBase):
         self.longitudes = grid.point_longitudes
         self.grid_shape = (grid.n_points,)
         self.coslat_rows = torch.as_tensor(c, device=grid.device)
+
+    def enable_split(self, nslices: int) -> None:
+        """Build the BF16 digit tables for split-precision contractions."""
+        nat.check(nat.lib().aeo_plan_enable_split(self._plan, int(nslices)), "enable_split")
+        self.nslices = int(nslices)

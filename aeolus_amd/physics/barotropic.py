@@ -17,7 +17,7 @@ import torch
 
 from .. import _native as nat
 
-__all__ = ["BarotropicState", "BarotropicVorticity"]
+__all__ = ["BarotropicState", "BarotropicVorticity", "BarotropicBatch"]
 
 
 @dataclass
@@ -50,10 +50,13 @@ class BarotropicVorticity:
         self.laplacian_eig = -l * (l + 1) / self.R ** 2
         self.laplacian_eig[0] = 1.0 / self.R ** 2
         self._ps = self.so.backend.product_space(self.so.product_quadrature)
+        self._model = self._create()
+
+    def _create(self):
         handle = ctypes.c_void_p()
         nat.check(nat.lib().aeo_bve_create(self._ps.sh._plan, self.R, self.Omega, self.nu,
                                            ctypes.byref(handle)), "BarotropicVorticity")
-        self._model = handle
+        return handle
 
     def __del__(self):
         h = getattr(self, "_model", None)
@@ -66,6 +69,22 @@ class BarotropicVorticity:
 
     def _coeffs(self, c) -> torch.Tensor:
         return torch.as_tensor(c, device=self.device).to(torch.complex128).contiguous()
+
+    def set_precision(self, nslices: int = 0) -> None:
+        """0: FP64 Legendre contractions; k in {4..7}: split BF16 MFMA digits.
+
+        Builds the product plan's digit tables on first use.  Everything
+        else of the step stays FP64."""
+        k = int(nslices)
+        if k:
+            sh = self._ps.sh
+            if getattr(sh, "nslices", 0) != k:
+                sh.enable_split(k)
+        nat.check(nat.lib().aeo_model_set_precision(self._model, k), "set_precision")
+        self.nslices = k
+
+    def set_tiles(self, members_per_block: int) -> None:
+        nat.check(nat.lib().aeo_model_set_tiles(self._model, int(members_per_block)), "set_tiles")
 
     def vorticity_to_streamfunction(self, vrt_state: BarotropicState) -> torch.Tensor:
         assert isinstance(vrt_state, BarotropicState)
@@ -100,3 +119,21 @@ class BarotropicVorticity:
                                           None if frc is None else frc.data_ptr(),
                                           y.data_ptr(), nat.stream()), "rk4")
         return y
+
+
+class BarotropicBatch(BarotropicVorticity):
+    """``nmembers`` independent BVE states on one Gauss product space, stepped
+    together: states are ``(nmembers, L+1, L+1)`` stacks and every kernel of
+    the step covers all members (no reference counterpart beyond running
+    ``nmembers`` ``BarotropicVorticity`` models in lockstep)."""
+
+    def __init__(self, planet, nmembers: int, viscosity: float = 0.0):
+        self.nmembers = int(nmembers)
+        super().__init__(planet, viscosity=viscosity)
+
+    def _create(self):
+        handle = ctypes.c_void_p()
+        nat.check(nat.lib().aeo_bve_create_batch(self._ps.sh._plan, self.R, self.Omega, self.nu,
+                                                 self.nmembers, ctypes.byref(handle)),
+                  "BarotropicBatch")
+        return handle

@@ -131,6 +131,22 @@ class ShallowWaterModel:
                 pass
             self._model = ctypes.c_void_p()
 
+    def set_precision(self, nslices: int = 0) -> None:
+        """0: FP64 Legendre contractions; k in {4..7}: split BF16 MFMA digits.
+
+        Builds the product plan's digit tables on first use.  Everything
+        else of the step stays FP64."""
+        k = int(nslices)
+        if k:
+            sh = self._ps.sh
+            if getattr(sh, "nslices", 0) != k:
+                sh.enable_split(k)
+        nat.check(nat.lib().aeo_model_set_precision(self._model, k), "set_precision")
+        self.nslices = k
+
+    def set_tiles(self, members_per_block: int) -> None:
+        nat.check(nat.lib().aeo_model_set_tiles(self._model, int(members_per_block)), "set_tiles")
+
     def _stack(self, coeffs) -> torch.Tensor:
         y = torch.as_tensor(coeffs, device=self.device).to(torch.complex128).contiguous()
         n = self.l_max + 1

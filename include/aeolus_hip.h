@@ -90,6 +90,14 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
 
 int aeo_plan_destroy(aeo_plan_t plan);
 
+/* Build the split-precision tables of a Gauss plan: the FP64 Legendre tables
+ * cut into `nslices` BF16 integer digits per entry with one power-of-two
+ * exponent per table row (2 <= nslices <= 8).  Models on this plan can then
+ * run their Legendre contractions as BF16 MFMA products with exact FP32
+ * accumulation and FP64 recombination (aeo_model_set_precision).  No
+ * counterpart in the reference (its transforms are FP64 only). */
+int aeo_plan_enable_split(aeo_plan_t plan, int nslices);
+
 /* Query: l_max, number of grid points, number of basis functions. */
 int aeo_plan_info(aeo_plan_t plan, int* l_max, int* npoints, int* nbasis);
 
@@ -135,6 +143,13 @@ int aeo_gradient(aeo_plan_t plan, double radius, const double* q, double* dqdx, 
 int aeo_bve_create(aeo_plan_t prod, double radius, double omega, double viscosity,
                    aeo_model_t* out);
 
+/* A batch of `nmembers` independent BVE states on one Gauss product plan
+ * (same L, grid, radius, rotation, viscosity).  States are stacked
+ * (nmembers, L+1, L+1) complex128.  Replaces nmembers BarotropicVorticity
+ * instances stepped in lockstep (phy/barotropic.py:21-167). */
+int aeo_bve_create_batch(aeo_plan_t prod, double radius, double omega, double viscosity,
+                         int nmembers, aeo_model_t* out);
+
 /* Shallow-water model on a product plan.  phi_s_lm: device (L+1)^2 complex
  * surface geopotential g*h_s, or NULL for a flat bottom.
  * Replaces ShallowWaterModel.__init__ (phy/shallow_water.py:172-283). */
@@ -143,6 +158,16 @@ int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
                    aeo_model_t* out);
 
 int aeo_model_destroy(aeo_model_t model);
+
+/* 0 = FP64 Legendre contractions; k > 0 = split BF16 digits with k slices
+ * (the plan must have aeo_plan_enable_split(plan, k)).  Everything else of the
+ * step (recurrence tables, FFTs, products, spectral algebra, RK4 update)
+ * stays FP64.  No counterpart in the reference. */
+int aeo_model_set_precision(aeo_model_t model, int nslices);
+
+/* Members per workgroup of the Legendre stage kernels (1, 2, 4 or 8): the
+ * tile shape the autotuner chooses.  No counterpart in the reference. */
+int aeo_model_set_tiles(aeo_model_t model, int members_per_block);
 
 /* Tendency of one state (BVE: one field; SWE: the (3, L+1, L+1) stack zeta,
  * delta, phi).  forcing: device, same shape, or NULL (BVE only).

@@ -127,3 +127,30 @@ def test_swe_rk4_matches_oracle_short():
         y2 = rk4_step_array(model.tendency, y2, 0.0, 300.0,
                             stage_validator=lambda s: None)
     assert rel_l2(_np(y2), _np(y)) < 1e-13
+
+
+@pytest.mark.parametrize("nmem", [1, 3, 4, 9])
+def test_bve_batch_members_match_single_runs(nmem):
+    """Every member of a batched step equals the single-member step bitwise."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.physics.barotropic import BarotropicBatch
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print(f"case bve batch nmem={nmem}", flush=True)
+    pl, _ = _planets("latlon", 42, 64, 128)
+    single = BarotropicVorticity(pl)
+    batch = BarotropicBatch(pl, nmem)
+    zs = np.stack([spectrum_l15(42, np.random.default_rng(100 + k), rms=5e-5)
+                   for k in range(nmem)])
+    out = batch.rk4(torch.as_tensor(zs, device="cuda"), 600.0, 5)
+    tend = batch.tendency(BarotropicState_(torch.as_tensor(zs, device="cuda")))
+    for k in range(nmem):
+        ref = single.rk4(torch.as_tensor(zs[k], device="cuda"), 600.0, 5)
+        assert torch.equal(out[k], ref), k
+        tk = single.tendency(BarotropicState_(torch.as_tensor(zs[k], device="cuda")))
+        assert torch.equal(tend[k], tk), k
+
+
+def BarotropicState_(c):
+    from aeolus_amd.physics import BarotropicState
+    return BarotropicState(c)
