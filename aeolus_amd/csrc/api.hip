@@ -80,7 +80,8 @@ static void free_model(aeo_model_s* m) {
     for (void* q : {(void*)m->d_nulap, (void*)m->d_lap, (void*)m->d_nu4lap2,
                     (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
                     (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
-                    (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum})
+                    (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum,
+                    (void*)m->d_fint})
         dev_free(q);
     delete m;
 }
@@ -455,6 +456,9 @@ static int model_common(aeo_plan_t prod, double radius, aeo_model_s* m) {
     if ((rc = dev_alloc(&m->d_ystage, 2 * st))) return rc;
     if ((rc = dev_alloc(&m->d_ybuf, 2 * st))) return rc;
     if ((rc = dev_alloc(&m->d_ksum, st))) return rc;
+    if (prod->kind == kGauss && m->kind == kBVE) {
+        if ((rc = dev_alloc(&m->d_fint, st))) return rc;
+    }
     return AEO_OK;
 }
 
@@ -567,6 +571,12 @@ int aeo_model_destroy(aeo_model_t model) {
     return AEO_OK;
 }
 
+// The Gauss model kernels keep states in the internal [member][field][m][l]
+// layout (columns of one m contiguous); the boundary layout is [..][l][m].
+static int to_internal(aeo_model_s* m, const double2* user, double2* internal, hipStream_t s) {
+    return launch_transpose(user, internal, m->L + 1, m->nstate * m->nmem, s);
+}
+
 int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing, double* out,
                        void* stream) {
     AEO_REQUIRE(model && y && out, AEO_ERR_INVALID, "aeo_model_tendency: NULL argument");
@@ -577,9 +587,16 @@ int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing
     auto F = reinterpret_cast<const double2*>(forcing);
     auto K = reinterpret_cast<double2*>(out);
     if (model->plan->kind == kGauss) {
-        int rc = gauss_model_prime(model, Y, s);
-        if (rc) return rc;
-        return gauss_model_stage(model, Y, Y, F, K, -1, 0.0, nullptr, false, s);
+        const size_t st = (size_t)(model->L + 1) * (model->L + 1) * model->nstate * model->nmem;
+        double2* yi = model->d_ystage;
+        double2* ki = model->d_ystage + st;
+        int rc = to_internal(model, Y, yi, s);
+        if (!rc && F) rc = to_internal(model, F, model->d_fint, s);
+        if (!rc) rc = gauss_model_prime(model, yi, s);
+        if (!rc) rc = gauss_model_stage(model, yi, yi, F ? model->d_fint : nullptr, ki, -1, 0.0,
+                                        nullptr, false, s);
+        if (!rc) rc = launch_transpose(ki, K, model->L + 1, model->nstate * model->nmem, s);
+        return rc;
     }
     return points_model_stage(model, Y, Y, F, K, -1, 0.0, nullptr, s);
 }
@@ -594,7 +611,6 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
     hipStream_t s = as_stream(stream);
     const size_t st = (size_t)(model->L + 1) * (model->L + 1) * model->nstate * model->nmem;
     auto F = reinterpret_cast<const double2*>(forcing);
-    const double2* cur = reinterpret_cast<const double2*>(y);
     if (nsteps == 0) {
         if (y != y_out)
             AEO_CHECK_HIP(hipMemcpyAsync(y_out, y, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
@@ -603,14 +619,22 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
     const bool gauss = model->plan->kind == kGauss;
     double2* ystage[2] = {model->d_ystage, model->d_ystage + st};
     double2* ybuf[2] = {model->d_ybuf, model->d_ybuf + st};
+    const double2* cur = reinterpret_cast<const double2*>(y);
     if (gauss) {
-        int rc = gauss_model_prime(model, cur, s);
+        // internal copy of the input in ybuf[1]: step 0 reads it and writes ybuf[0]
+        int rc = to_internal(model, cur, ybuf[1], s);
+        if (!rc && F) rc = to_internal(model, F, model->d_fint, s);
+        if (rc) return rc;
+        cur = ybuf[1];
+        F = F ? model->d_fint : nullptr;
+        rc = gauss_model_prime(model, cur, s);
         if (rc) return rc;
     }
     for (int step = 0; step < nsteps; ++step) {
+        double2* out = ybuf[step % 2];
         for (int k = 0; k < 4; ++k) {
             const double2* ys = (k == 0) ? cur : ystage[(k - 1) % 2];
-            double2* ynext = (k == 3) ? ybuf[step % 2] : ystage[k % 2];
+            double2* ynext = (k == 3) ? out : ystage[k % 2];
             int rc;
             if (gauss) {
                 const bool syn_next = !(step == nsteps - 1 && k == 3);
@@ -620,8 +644,10 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
             }
             if (rc) return rc;
         }
-        cur = ybuf[step % 2];
+        cur = out;
     }
+    if (gauss) return launch_transpose(cur, reinterpret_cast<double2*>(y_out), model->L + 1,
+                                       model->nstate * model->nmem, s);
     AEO_CHECK_HIP(hipMemcpyAsync(y_out, cur, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
     return AEO_OK;
 }

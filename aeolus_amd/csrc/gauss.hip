@@ -127,13 +127,14 @@ enum SpecMode { kSpecPlain = 0, kSpecDlam = 1, kSpecScale = 2 };
 // optional (i m / R) or 1/R factor.  Its unnormalised inverse DFT is
 // f_N + i f_S.
 __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er,
-                                     const double2* Or, int mode, double R, int t0,
+                                     const double2* Or, size_t ms, int mode, double R, int t0,
                                      int tstride) {
+    // Er[m * ms], Or[m * ms]: even/odd partial sums of one field at wavenumber m
     const double invR = 1.0 / R;
     for (int b = t0; b < n; b += tstride) {
         double2 z = c2(0.0, 0.0);
         for (int m = b; m <= L; m += n) {
-            double2 e = Er[m], o = Or[m];
+            double2 e = Er[m * ms], o = Or[m * ms];
             double2 gn = cadd(e, o), gs = csub(e, o);
             if (mode == kSpecDlam) { gn = dlam(gn, m, R); gs = dlam(gs, m, R); }
             else if (mode == kSpecScale) { gn = cscale(gn, invR); gs = cscale(gs, invR); }
@@ -141,7 +142,7 @@ __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er
             z = cadd(z, c2(gn.x - gs.y, gn.y + gs.x));
         }
         for (int m = (b == 0 ? n : n - b); m <= L; m += n) {
-            double2 e = Er[m], o = Or[m];
+            double2 e = Er[m * ms], o = Or[m * ms];
             double2 gn = cadd(e, o), gs = csub(e, o);
             if (mode == kSpecDlam) { gn = dlam(gn, m, R); gs = dlam(gs, m, R); }
             else if (mode == kSpecScale) { gn = cscale(gn, invR); gs = cscale(gs, invR); }
@@ -152,9 +153,9 @@ __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er
 }
 
 // Split the forward DFT of z = g_N + i g_S into S = w (F_N + F_S) and
-// D = w (F_N - F_S) for m = 0..mmax.
+// D = w (F_N - F_S) for m = 0..mmax, stored at Sr[m * ms], Dr[m * ms].
 __device__ inline void unpack_sd(const double2* Z, int n, int mmax, double w, double2* Sr,
-                                 double2* Dr, int t0, int tstride) {
+                                 double2* Dr, size_t ms, int t0, int tstride) {
     for (int m = t0; m <= mmax; m += tstride) {
         const int b = m % n;
         const int bn = (n - b) % n;
@@ -163,28 +164,30 @@ __device__ inline void unpack_sd(const double2* Z, int n, int mmax, double w, do
         const double2 fn = cscale(cadd(zb, zn), 0.5);
         const double2 dd = csub(zb, zn);
         const double2 fs = c2(0.5 * dd.y, -0.5 * dd.x);  // (zb - zn) / (2i)
-        Sr[m] = cscale(cadd(fn, fs), w);
-        Dr[m] = cscale(csub(fn, fs), w);
+        Sr[m * ms] = cscale(cadd(fn, fs), w);
+        Dr[m * ms] = cscale(csub(fn, fs), w);
     }
 }
 
-// Synthesis of nf fields: E/O -> grid rows.  grid (nh); block 256.
+// Intermediate layout of every grid kernel: E/O/S/D[jp][m][column], one
+// column per (member, field) pair, `ncol` columns in all.
+
+// Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh); block 256.
 __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
                                                                const double2* __restrict__ E,
                                                                const double2* __restrict__ O,
                                                                int nf, double* __restrict__ f_out) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L, nh = g.nh;
+    const int n = g.nlon, L = g.L;
     const int jN = jp, jS = g.nlat - 1 - jp;
-    const size_t plane = (size_t)nh * (L + 1);
+    const size_t row = (size_t)jp * (L + 1) * nf;
     const int cap = kRowElems / n > 0 ? kRowElems / n : 1;  // rows per LDS pass
     for (int f0 = 0; f0 < nf; f0 += cap) {
         const int nr = (nf - f0 < cap) ? nf - f0 : cap;
         for (int r = 0; r < nr; ++r) {
-            const size_t off = (size_t)(f0 + r) * plane + (size_t)jp * (L + 1);
-            pack_spectrum(lds + r * n, n, L, E + off, O + off, kSpecPlain, 0.0, threadIdx.x,
-                          kGridThreads);
+            pack_spectrum(lds + r * n, n, L, E + row + f0 + r, O + row + f0 + r, nf, kSpecPlain,
+                          1.0, threadIdx.x, kGridThreads);
         }
         __syncthreads();
         fft_rows<+1>(lds, nr, g.fft);
@@ -207,9 +210,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
                                                                double2* __restrict__ D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L, nh = g.nh;
+    const int n = g.nlon, L = g.L;
     const int jN = jp, jS = g.nlat - 1 - jp;
-    const size_t plane = (size_t)nh * (L + 1);
+    const size_t row = (size_t)jp * (L + 1) * nf;
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const int cap = kRowElems / n > 0 ? kRowElems / n : 1;
     for (int f0 = 0; f0 < nf; f0 += cap) {
@@ -221,36 +224,29 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
         }
         __syncthreads();
         fft_rows<-1>(lds, nr, g.fft);
-        for (int r = 0; r < nr; ++r) {
-            const size_t off = (size_t)(f0 + r) * plane + (size_t)jp * (L + 1);
-            unpack_sd(lds + r * n, n, mmax, w, S + off, D + off, threadIdx.x, kGridThreads);
-        }
+        for (int r = 0; r < nr; ++r)
+            unpack_sd(lds + r * n, n, mmax, w, S + row + f0 + r, D + row + f0 + r, nf,
+                      threadIdx.x, kGridThreads);
         __syncthreads();
     }
 }
 
 // Jacobian: fields (a, S a, b, S b) -> J = (a_s b_l - a_l b_s)/c^2 -> S/D.
-__global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g,
-                                                               const double2* E,
-                                                               const double2* O,
-                                                               double R, int mmax,
-                                                               double2* __restrict__ S,
-                                                               double2* __restrict__ D) {
+// grid (nh, members); E/O columns (member, 4 fields), S/D columns (member).
+__global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, const double2* E,
+                                                               const double2* O, double R,
+                                                               int mmax, int nmem, double2* S,
+                                                               double2* D) {
     extern __shared__ double2 lds[];
-    const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L, nh = g.nh;
+    const int jp = blockIdx.x, b = blockIdx.y;
+    const int n = g.nlon, L = g.L;
     const int jN = jp, jS = g.nlat - 1 - jp;
-    const size_t plane = (size_t)nh * (L + 1);
-    const size_t roff = (size_t)jp * (L + 1);
-    E += (size_t)blockIdx.y * 4 * plane;       // member blockIdx.y
-    O += (size_t)blockIdx.y * 4 * plane;
-    S += (size_t)blockIdx.y * plane;
-    D += (size_t)blockIdx.y * plane;
+    const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
+    const size_t ms = (size_t)nmem * 4;
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
-        const size_t off = (size_t)r * plane + roff;
-        pack_spectrum(lds + r * n, n, L, E + off, O + off, (r & 1) ? kSpecScale : kSpecDlam,
-                      R, threadIdx.x, kGridThreads);
+        pack_spectrum(lds + r * n, n, L, E + eo + r, O + eo + r, ms,
+                      (r & 1) ? kSpecScale : kSpecDlam, R, threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1>(lds, 4, g.fft);
@@ -265,35 +261,30 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g,
     __syncthreads();
     fft_rows<-1>(lds, 1, g.fft);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
-    unpack_sd(lds, n, mmax, w, S + roff, D + roff, threadIdx.x, kGridThreads);
+    const size_t so = (size_t)jp * (L + 1) * nmem + b;
+    unpack_sd(lds, n, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
 }
 
 // Shallow water: 10 Legendre fields -> 12 grid rows -> 4 products -> S/D.
-__global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g,
-                                                               const double2* __restrict__ E,
-                                                               const double2* __restrict__ O,
-                                                               double R, int mmax,
-                                                               double2* S,
+// grid (nh, members); E/O columns (member, 10 fields), S/D columns (member, 4).
+__global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, const double2* E,
+                                                               const double2* O, double R,
+                                                               int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
-    const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L, nh = g.nh;
+    const int jp = blockIdx.x, b = blockIdx.y;
+    const int n = g.nlon, L = g.L;
     const int jN = jp, jS = g.nlat - 1 - jp;
-    const size_t plane = (size_t)nh * (L + 1);
-    const size_t roff = (size_t)jp * (L + 1);
-    E += (size_t)blockIdx.y * 10 * plane;      // member blockIdx.y
-    O += (size_t)blockIdx.y * 10 * plane;
-    S += (size_t)blockIdx.y * 4 * plane;
-    D += (size_t)blockIdx.y * 4 * plane;
+    const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 10;
+    const size_t ms = (size_t)nmem * 10;
     // grid row r <- (Legendre field, spectral factor)
     constexpr int src[12] = {0, 1, 2, 3, 4, 5, 4, 6, 7, 6, 8, 9};
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
                               kSpecScale, kSpecPlain, kSpecDlam, kSpecScale, kSpecPlain,
                               kSpecPlain, kSpecPlain};
     for (int r = 0; r < 12; ++r) {
-        const size_t off = (size_t)src[r] * plane + roff;
-        pack_spectrum(lds + r * n, n, L, E + off, O + off, mode[r], R, threadIdx.x,
-                      kGridThreads);
+        pack_spectrum(lds + r * n, n, L, E + eo + src[r], O + eo + src[r], ms, mode[r], R,
+                      threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1>(lds, 12, g.fft);
@@ -315,10 +306,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g,
     __syncthreads();
     fft_rows<-1>(lds, 4, g.fft);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
-    for (int r = 0; r < 4; ++r) {
-        const size_t off = (size_t)r * plane + roff;
-        unpack_sd(lds + r * n, n, mmax, w, S + off, D + off, threadIdx.x, kGridThreads);
-    }
+    const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
+    for (int r = 0; r < 4; ++r)
+        unpack_sd(lds + r * n, n, mmax, w, S + so + r, D + so + r, (size_t)nmem * 4,
+                  threadIdx.x, kGridThreads);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,7 +453,7 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
     hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, 1), dim3(kGridThreads),
-                       sizeof(double2) * 4 * p->nlon, s, p->gdev(), p->d_E, p->d_O, R, lim,
+                       sizeof(double2) * 4 * p->nlon, s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
                        p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
@@ -483,6 +474,7 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     StageArgs sa{};
     sa.ys = y;
+    sa.ys_mmajor = 1;
     sa.ys_stride = fs * md->nstate;
     sa.nmem = md->nmem;
     sa.nstate_valid = md->nstate;
@@ -521,7 +513,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     if (md->kind == kBVE) {
         hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads),
                            sizeof(double2) * 4 * p->nlon, s, p->gdev(), md->d_E, md->d_O,
-                           md->R, cut, md->d_S, md->d_D);
+                           md->R, cut, md->nmem, md->d_S, md->d_D);
     } else {
         const size_t swe_lds = sizeof(double2) * 12 * p->nlon;
         if (swe_lds > 65536) {
@@ -530,12 +522,14 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                                               (int)swe_lds));
         }
         hipLaunchKernelGGL(grid_swe_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds,
-                           s, p->gdev(), md->d_E, md->d_O, md->R, cut, md->d_S, md->d_D);
+                           s, p->gdev(), md->d_E, md->d_O, md->R, cut, md->nmem, md->d_S,
+                           md->d_D);
     }
     pr.end(kClsGrid, s);
     AEO_LAUNCH_CHECK();
     StageArgs sa{};
     sa.ys = ys;
+    sa.ys_mmajor = 1;
     sa.ys_stride = fs * md->nstate;
     sa.nmem = md->nmem;
     sa.nstate_valid = md->nstate;

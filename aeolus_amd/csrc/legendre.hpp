@@ -199,7 +199,7 @@ struct EpiBveS {
     Rk4B rk;
     __device__ void apply(int b, int l, int m, const double2* acc, double2* st,
                           bool writer) const {
-        const size_t idx = b * member_stride + (size_t)l * (L + 1) + m;
+        const size_t idx = b * member_stride + (size_t)m * (L + 1) + l;   // internal [m][l]
         double2 k = c2(0.0, 0.0);
         if (l > 0) {
             k = cadd(c2(-acc[0].x, -acc[0].y), cscale(st[l], nulap[l]));
@@ -224,7 +224,7 @@ struct EpiSweS {
                           bool writer) const {
         const int n = L + 1;
         const size_t fs = (size_t)n * n;
-        const size_t e = (size_t)l * n + m;
+        const size_t e = (size_t)m * n + l;   // internal [m][l]
         const size_t base = b * member_stride;
         double2 kz = c2(0.0, 0.0), kd = c2(0.0, 0.0), kp = c2(0.0, 0.0);
         if (l > 0) {
@@ -232,13 +232,14 @@ struct EpiSweS {
             kz = acc[0];
             kp = csub(acc[3], cscale(delta, phi0));
             kd = csub(acc[1], cscale(cadd(acc[2], phi), lap[l]));
-            if (lap_phi_s) kd = csub(kd, lap_phi_s[e]);
+            const size_t eu = (size_t)l * n + m;  // topography terms keep the boundary layout
+            if (lap_phi_s) kd = csub(kd, lap_phi_s[eu]);
             if (nu4lap2) {
                 const double q = nu4lap2[l];
                 kz = csub(kz, cscale(zeta, q));
                 kd = csub(kd, cscale(delta, q));
                 kp = csub(kp, cscale(phi, q));
-                if (nu4_lap2_phi_s) kp = csub(kp, nu4_lap2_phi_s[e]);
+                if (nu4_lap2_phi_s) kp = csub(kp, nu4_lap2_phi_s[eu]);
             }
         }
         st[l] = rk.apply(base + e, kz, writer);
@@ -253,12 +254,13 @@ struct EpiSweS {
 struct StageArgs {
     const double2* ys;      // stage input states [member][NSTATE][n][n]
     size_t ys_stride;       // complex entries per member
+    int ys_mmajor;          // 1: ys is [..][m][l] (internal layout); 0: [..][l][m]
     int nmem;
     int nstate_valid;       // state fields present in ys (<= NSTATE)
-    const double2* S;       // [member][NA][nh][n]
+    const double2* S;       // [jp][m][member][NA]
     const double2* D;
     int lmax_a, mmax_a;
-    double2* E;             // [member][NF][nh][n]
+    double2* E;             // [jp][m][member][NF]
     double2* O;
     ColK k;
 };
@@ -317,7 +319,9 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         const int s = r / n, l = r - s * n;
         const int b = b0 + bb;
         double2 v = c2(0.0, 0.0);
-        if (b < a.nmem && s < a.nstate_valid) v = a.ys[b * a.ys_stride + (size_t)s * n * n + l * n + m];
+        if (b < a.nmem && s < a.nstate_valid)
+            v = a.ys[b * a.ys_stride + (size_t)s * n * n +
+                     (a.ys_mmajor ? (size_t)m * n + l : (size_t)l * n + m)];
         st[t] = v;
     }
     __syncthreads();
@@ -328,15 +332,14 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         double2* red = accl + (size_t)MB * NA * n;          // [4][MB][NA][64]
         const bool active = (m <= a.mmax_a);
         if (active) {
-            const size_t plane = (size_t)nh * n;
             for (int t = threadIdx.x; t < MB * NA * nh; t += kStageThreads) {
-                const int bb = t / (NA * nh);
-                const int r = t - bb * NA * nh;
-                const int f = r / nh, jp = r - f * nh;
+                const int f = t % NA;
+                const int bb = (t / NA) % MB;
+                const int jp = t / (NA * MB);
                 const int b = b0 + bb;
                 double2 sv = c2(0.0, 0.0), dv = c2(0.0, 0.0);
                 if (b < a.nmem) {
-                    const size_t src = ((size_t)b * NA + f) * plane + (size_t)jp * n + m;
+                    const size_t src = (((size_t)jp * n + m) * a.nmem + b) * NA + f;
                     sv = a.S[src];
                     dv = a.D[src];
                 }
@@ -532,7 +535,6 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         if (j0 < nh) {
             const __bf16* Pt = g.Psl + g.offs_ss[m];
             const int* er = g.esyn + (size_t)m * 2 * nhp;
-            const size_t plane = (size_t)nh * n;
             for (int p = 0; p < 2; ++p) {
                 floatx4 acc[NS][NTS];
 #pragma unroll
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                             const int bb = c / (NF * 2), f = (c >> 1) % NF;
                             const int b = b0 + bb;
                             if (b < a.nmem)
-                                dst[((size_t)b * NF + f) * plane + (size_t)j * n + m] = c2(v, w);
+                                dst[(((size_t)j * n + m) * a.nmem + b) * NF + f] = c2(v, w);
                         }
                     }
                 }
@@ -618,7 +620,6 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         __syncthreads();
         if (wave < 2 && jp < nh) {
             double2* dst = (wave == 0) ? a.E : a.O;
-            const size_t plane = (size_t)nh * n;
 #pragma unroll
             for (int bb = 0; bb < MB; ++bb) {
                 const int b = b0 + bb;
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
                     const double2 o = red[((wave * MB + bb) * NF + f) * 64 + lane];
-                    dst[((size_t)b * NF + f) * plane + (size_t)jp * n + m] = cadd(acc[bb][f], o);
+                    dst[(((size_t)jp * n + m) * a.nmem + b) * NF + f] = cadd(acc[bb][f], o);
                 }
             }
         }

@@ -145,6 +145,7 @@ struct aeo_model_s {
     double2* d_ystage = nullptr;   // 2 slots (ping-pong between stages)
     double2* d_ybuf = nullptr;     // 2 slots (ping-pong between steps)
     double2* d_ksum = nullptr;
+    double2* d_fint = nullptr;     // forcing in the internal [m][l] layout
     int nmem = 1;                  // members (independent states) in the group
     int mb = 1;                    // members per workgroup in the Legendre stage kernel
     int nslices = 0;               // 0: FP64 contractions; > 0: split BF16 MFMA digits
@@ -180,6 +181,8 @@ int points_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
 
 int launch_spec_op(int op, int L, double R, const double2* in, double2* out, int nf,
                    hipStream_t s);
+// out[k] = transpose(in[k]) for nmat (n x n) complex matrices (in != out).
+int launch_transpose(const double2* in, double2* out, int n, int nmat, hipStream_t s);
 int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf, double* out,
                          hipStream_t s);
 int launch_velocity_points(int n, int row_len, double R, const double* coslat,

@@ -186,4 +186,28 @@ int launch_velocity_points(int n, int row_len, double R, const double* coslat,
     return AEO_OK;
 }
 
+// out[k][j][i] = in[k][i][j] for nmat square (n x n) complex matrices.
+__global__ void transpose_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                 int n) {
+    __shared__ double2 tile[32][33];
+    const size_t base = (size_t)blockIdx.z * n * n;
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int i = i0 + r, j = j0 + threadIdx.x;
+        if (i < n && j < n) tile[r][threadIdx.x] = in[base + (size_t)i * n + j];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int j = j0 + r, i = i0 + threadIdx.x;
+        if (i < n && j < n) out[base + (size_t)j * n + i] = tile[threadIdx.x][r];
+    }
+}
+
+int launch_transpose(const double2* in, double2* out, int n, int nmat, hipStream_t s) {
+    dim3 grid((n + 31) / 32, (n + 31) / 32, nmat);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, in, out, n);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
 }  // namespace aeo
