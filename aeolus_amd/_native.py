@@ -52,6 +52,10 @@ _SIGNATURES = {
     "aeo_model_rk4": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp],
     "aeo_spectral_diag": [_c_int, _c_double, _c_double, _vp, _c_int, _vp, _vp],
     "aeo_model_profile": [_vp, _c_int, _c_int],
+    "aeo_ensemble_create": [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+    "aeo_ensemble_destroy": [_vp],
+    "aeo_ensemble_rk4": [_vp, ctypes.POINTER(_vp), _c_double, _c_int, ctypes.POINTER(_vp), _c_int,
+                         _vp],
     "aeo_model_profile_read": [_vp, _dp, ctypes.POINTER(_c_int)],
 }
 

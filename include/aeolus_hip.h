@@ -56,6 +56,7 @@ enum aeo_spec_op_code {
 
 typedef struct aeo_plan_s* aeo_plan_t;
 typedef struct aeo_model_s* aeo_model_t;
+typedef struct aeo_ensemble_s* aeo_ensemble_t;
 
 /* ABI version (AEO_ABI_VERSION) of the loaded library. */
 int aeo_version(void);
@@ -193,6 +194,21 @@ int aeo_model_profile(aeo_model_t model, int kernel_class, int max_launches);
 /* Sum of the bracketed launch durations (ms) and their count since the last
  * read; resets the count. */
 int aeo_model_profile_read(aeo_model_t model, double* total_ms, int* launches);
+
+/* An ensemble of member batches with different L (one Gauss model per L
+ * class, e.g. from aeo_bve_create_batch); the ensemble does not own them.
+ * No counterpart in the reference (it runs one state at a time). */
+int aeo_ensemble_create(int ngroups, const aeo_model_t* models, aeo_ensemble_t* out);
+int aeo_ensemble_destroy(aeo_ensemble_t ens);
+
+/* `nsteps` RK4 steps of every group: y[g] -> y_out[g] (device, boundary
+ * layout; y_out[g] may equal y[g]).  With use_graph != 0 two steps of all
+ * groups are captured once (per dt) into one hipGraph with one concurrent
+ * branch per group, and replayed nsteps/2 times on `stream`; an odd last
+ * step runs eagerly.  Each group computes exactly what aeo_model_rk4 does.
+ * Replaces running rk4_step (run/bve/runner.py:211-216) per member. */
+int aeo_ensemble_rk4(aeo_ensemble_t ens, const double* const* y, double dt, int nsteps,
+                     double* const* y_out, int use_graph, void* stream);
 
 /* Spectral invariants of `nfields` vorticity fields with compensated sums:
  * out[4 f + 0..3] = circulation, energy, relative enstrophy, absolute

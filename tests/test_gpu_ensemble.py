@@ -1,0 +1,51 @@
+"""Uneven-L ensemble: graph-captured step == eager step == per-member runs
+(bitwise), the autotuner picks a tile that runs, odd and even step counts."""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+R = 6.371e6
+OM = 2.0 * math.pi / 86400.0
+CLASSES = [(21, 32, 64, 5), (42, 64, 128, 3), (85, 128, 256, 2)]
+
+
+def _planet(L, nlat, nlon):
+    from aeolus_amd.planet import Planet
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return Planet.with_radius(R, OM, grid_type="latlon", nlat=nlat, nlon=nlon, l_max=L)
+
+
+@pytest.mark.parametrize("precision", [0, 6])
+def test_ensemble_graph_matches_eager_and_single(precision, tmp_path, monkeypatch):
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    monkeypatch.setenv("AEOLUS_AMD_CACHE", str(tmp_path))
+    print(f"case ensemble precision={precision}", flush=True)
+    planets = [_planet(L, a, b) for L, a, b, _ in CLASSES]
+    ens = Ensemble([(pl, n) for pl, (_, _, _, n) in zip(planets, CLASSES)], precision=precision)
+    states = [torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(10 * L + k),
+                                                     rms=5e-5) for k in range(n)]), device="cuda")
+              for L, _, _, n in CLASSES]
+    tuning = ens.autotune(states, 300.0, steps=2)
+    assert all(t["mb"] in (1, 2, 4, 8) for t in tuning)
+    for nsteps in (4, 5):
+        g = ens.rk4(states, 300.0, nsteps, use_graph=True)
+        e = ens.rk4(states, 300.0, nsteps, use_graph=False)
+        for a, b in zip(g, e):
+            assert torch.equal(a, b)
+    for (L, _, _, n), pl, y0, y in zip(CLASSES, planets, states, g):
+        single = BarotropicVorticity(pl)
+        single.set_precision(precision)
+        for k in range(n):
+            ref = single.rk4(y0[k], 300.0, 5)
+            assert torch.equal(y[k], ref), (L, k)
+    # the cached choice is reused
+    again = ens.autotune(states, 300.0, steps=2)
+    assert all(t.get("cached") for t in again)
