@@ -81,7 +81,8 @@ static void free_model(aeo_model_s* m) {
                     (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
                     (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
                     (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum,
-                    (void*)m->d_fint})
+                    (void*)m->d_fint, (void*)m->d_aE, (void*)m->d_aO, (void*)m->d_aS,
+                    (void*)m->d_aD, (void*)m->d_asum, (void*)m->d_abuf, (void*)m->d_kbuf})
         dev_free(q);
     delete m;
 }
@@ -650,6 +651,147 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
                                        model->nstate * model->nmem, s);
     AEO_CHECK_HIP(hipMemcpyAsync(y_out, cur, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
     return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpointed forward run and the reverse (adjoint) sweep of the BVE step.
+// ---------------------------------------------------------------------------
+static size_t state_elems(const aeo_model_s* m) {
+    return (size_t)(m->L + 1) * (m->L + 1) * m->nstate * m->nmem;
+}
+
+static int adjoint_supported(const aeo_model_s* m, const char* who) {
+    if (m->kind != kBVE || m->plan->kind != kGauss) {
+        set_error(std::string(who) + ": the adjoint is implemented for BVE models on Gauss plans");
+        return AEO_ERR_UNSUPPORTED;
+    }
+    return AEO_OK;
+}
+
+static int ensure_adjoint(aeo_model_s* m) {
+    if (m->d_aE) return AEO_OK;
+    const size_t plane = (size_t)m->plan->nh * (m->L + 1) * m->nmem;
+    const size_t st = state_elems(m);
+    int rc;
+    if ((rc = dev_alloc(&m->d_aE, plane * 5))) return rc;
+    if ((rc = dev_alloc(&m->d_aO, plane * 5))) return rc;
+    if ((rc = dev_alloc(&m->d_aS, plane * 4))) return rc;
+    if ((rc = dev_alloc(&m->d_aD, plane * 4))) return rc;
+    if ((rc = dev_alloc(&m->d_asum, st))) return rc;
+    if ((rc = dev_alloc(&m->d_abuf, 2 * st))) return rc;
+    if ((rc = dev_alloc(&m->d_kbuf, 2 * st))) return rc;
+    return AEO_OK;
+}
+
+int aeo_model_checkpoint_elems(aeo_model_t model, int nsteps, long long* out) {
+    AEO_REQUIRE(model && out, AEO_ERR_INVALID, "aeo_model_checkpoint_elems: NULL argument");
+    AEO_REQUIRE(nsteps >= 0, AEO_ERR_INVALID, "aeo_model_checkpoint_elems: nsteps must be >= 0");
+    *out = (long long)nsteps * 4 * (long long)state_elems(model);
+    return AEO_OK;
+}
+
+int aeo_model_rk4_record(aeo_model_t model, const double* y, double dt, int nsteps,
+                         const double* forcing, double* y_out, double* checkpoints,
+                         void* stream) {
+    AEO_REQUIRE(model && y && y_out, AEO_ERR_INVALID, "aeo_model_rk4_record: NULL argument");
+    AEO_REQUIRE(nsteps >= 0 && std::isfinite(dt), AEO_ERR_INVALID,
+                "aeo_model_rk4_record: need nsteps >= 0 and a finite dt");
+    AEO_REQUIRE(nsteps == 0 || checkpoints, AEO_ERR_INVALID,
+                "aeo_model_rk4_record: checkpoints is NULL");
+    int rc = adjoint_supported(model, "aeo_model_rk4_record");
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const size_t st = state_elems(model);
+    if (nsteps == 0) {
+        if (y != y_out)
+            AEO_CHECK_HIP(hipMemcpyAsync(y_out, y, st * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        return AEO_OK;
+    }
+    // the stage inputs are written straight into their checkpoint slots:
+    // slot (step, k) = input state of RK4 stage k of that step
+    double2* ck = reinterpret_cast<double2*>(checkpoints);
+    auto slot = [&](int step, int k) { return ck + ((size_t)step * 4 + k) * st; };
+    const double2* F = reinterpret_cast<const double2*>(forcing);
+    rc = to_internal(model, reinterpret_cast<const double2*>(y), slot(0, 0), s);
+    if (!rc && F) rc = to_internal(model, F, model->d_fint, s);
+    if (!rc) rc = gauss_model_prime(model, slot(0, 0), s);
+    if (rc) return rc;
+    F = F ? model->d_fint : nullptr;
+    double2* last = model->d_ybuf;
+    for (int step = 0; step < nsteps; ++step) {
+        for (int k = 0; k < 4; ++k) {
+            double2* ynext = (k < 3) ? slot(step, k + 1) : (step + 1 < nsteps ? slot(step + 1, 0) : last);
+            const bool syn_next = !(step == nsteps - 1 && k == 3);
+            rc = gauss_model_stage(model, slot(step, k), slot(step, 0), F, nullptr, k, dt, ynext,
+                                   syn_next, s);
+            if (rc) return rc;
+        }
+    }
+    return launch_transpose(last, reinterpret_cast<double2*>(y_out), model->L + 1,
+                            model->nstate * model->nmem, s);
+}
+
+int aeo_model_rk4_adjoint(aeo_model_t model, const double* checkpoints, double dt, int nsteps,
+                          const double* ybar_out, double* ybar, void* stream) {
+    AEO_REQUIRE(model && ybar_out && ybar, AEO_ERR_INVALID, "aeo_model_rk4_adjoint: NULL argument");
+    AEO_REQUIRE(nsteps >= 0 && std::isfinite(dt), AEO_ERR_INVALID,
+                "aeo_model_rk4_adjoint: need nsteps >= 0 and a finite dt");
+    AEO_REQUIRE(nsteps == 0 || checkpoints, AEO_ERR_INVALID,
+                "aeo_model_rk4_adjoint: checkpoints is NULL");
+    int rc = adjoint_supported(model, "aeo_model_rk4_adjoint");
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const size_t st = state_elems(model);
+    if (nsteps == 0) {
+        if (ybar_out != ybar)
+            AEO_CHECK_HIP(hipMemcpyAsync(ybar, ybar_out, st * sizeof(double2),
+                                         hipMemcpyDeviceToDevice, s));
+        return AEO_OK;
+    }
+    if ((rc = ensure_adjoint(model))) return rc;
+    const double2* ck = reinterpret_cast<const double2*>(checkpoints);
+    auto slot = [&](int step, int k) { return ck + ((size_t)step * 4 + k) * st; };
+    double2* abuf[2] = {model->d_abuf, model->d_abuf + st};
+    double2* kbuf[2] = {model->d_kbuf, model->d_kbuf + st};
+    rc = to_internal(model, reinterpret_cast<const double2*>(ybar_out), abuf[0], s);
+    if (!rc) rc = launch_scale(abuf[0], kbuf[0], dt / 6.0, st, s);
+    if (!rc) rc = gauss_adjoint_prime(model, slot(nsteps - 1, 3), kbuf[0], s);
+    if (rc) return rc;
+    int ai = 0, ki = 0;
+    for (int step = nsteps - 1; step >= 0; --step) {
+        for (int r = 0; r < 4; ++r) {
+            const double2* fwd_next =
+                (r < 3) ? slot(step, 2 - r) : (step > 0 ? slot(step - 1, 3) : nullptr);
+            rc = gauss_adjoint_stage(model, kbuf[ki], r, dt, abuf[ai], abuf[ai ^ 1], nullptr,
+                                     kbuf[ki ^ 1], fwd_next, s);
+            if (rc) return rc;
+            ki ^= 1;
+        }
+        ai ^= 1;
+    }
+    return launch_transpose(abuf[ai], reinterpret_cast<double2*>(ybar), model->L + 1,
+                            model->nstate * model->nmem, s);
+}
+
+int aeo_model_tendency_adjoint(aeo_model_t model, const double* y, const double* kbar,
+                               double* ybar, void* stream) {
+    AEO_REQUIRE(model && y && kbar && ybar, AEO_ERR_INVALID,
+                "aeo_model_tendency_adjoint: NULL argument");
+    int rc = adjoint_supported(model, "aeo_model_tendency_adjoint");
+    if (rc) return rc;
+    if ((rc = ensure_adjoint(model))) return rc;
+    hipStream_t s = as_stream(stream);
+    const size_t st = state_elems(model);
+    double2* yi = model->d_abuf;
+    double2* out = model->d_abuf + st;
+    rc = to_internal(model, reinterpret_cast<const double2*>(y), yi, s);
+    if (!rc) rc = to_internal(model, reinterpret_cast<const double2*>(kbar), model->d_kbuf, s);
+    if (!rc) rc = gauss_adjoint_prime(model, yi, model->d_kbuf, s);
+    if (!rc) rc = gauss_adjoint_stage(model, model->d_kbuf, -1, 0.0, nullptr, nullptr, out,
+                                      nullptr, nullptr, s);
+    if (!rc) rc = launch_transpose(out, reinterpret_cast<double2*>(ybar), model->L + 1,
+                                   model->nstate * model->nmem, s);
+    return rc;
 }
 
 int aeo_model_profile(aeo_model_t model, int kernel_class, int max_launches) {

@@ -312,6 +312,53 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
                   threadIdx.x, kGridThreads);
 }
 
+// Adjoint of the BVE Jacobian on one north/south row pair.  E/O columns
+// (member, 5 fields): psi, S psi, eta, S eta of the forward stage and the
+// Jacobian cotangent input jb; grid rows after the inverse FFT
+//   0 a_l, 1 a_s, 2 b_l, 3 b_s (as in grid_jac_kernel), 4 jt.
+// The cotangents of the four rows are jt times the partial derivatives of
+// J = (a_s b_l - a_l b_s)/c^2 (the quadrature weight of Ana^T cancels the
+// unweighted Syn^T, leaving a weighted analysis):
+//   0: -jt b_s/c^2,  1: jt b_l/c^2,  2: jt a_s/c^2,  3: -jt a_l/c^2,
+// analysed for m <= L into S/D columns (member, 4).  grid (nh, members).
+__global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, const double2* E,
+                                                               const double2* O, double R,
+                                                               int nmem, double2* S,
+                                                               double2* D) {
+    extern __shared__ double2 lds[];
+    const int jp = blockIdx.x, b = blockIdx.y;
+    const int n = g.nlon, L = g.L;
+    const int jN = jp, jS = g.nlat - 1 - jp;
+    const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
+    const size_t ms = (size_t)nmem * 5;
+    for (int r = 0; r < 5; ++r) {
+        const int mode = (r == 4) ? kSpecPlain : ((r & 1) ? kSpecScale : kSpecDlam);
+        pack_spectrum(lds + r * n, n, L, E + eo + r, O + eo + r, ms, mode, R, threadIdx.x,
+                      kGridThreads);
+    }
+    __syncthreads();
+    fft_rows<+1>(lds, 5, g.fft);
+    const double cN = clamp_coslat(g.coslat[jN]);
+    const double cS = clamp_coslat(g.coslat[jS]);
+    const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
+    for (int k = threadIdx.x; k < n; k += kGridThreads) {
+        const double2 al = lds[k], as = lds[n + k], bl = lds[2 * n + k], bs = lds[3 * n + k];
+        const double2 jt = lds[4 * n + k];
+        const double tN = jt.x * qN, tS = jt.y * qS;
+        lds[k] = c2(-tN * bs.x, -tS * bs.y);
+        lds[n + k] = c2(tN * bl.x, tS * bl.y);
+        lds[2 * n + k] = c2(tN * as.x, tS * as.y);
+        lds[3 * n + k] = c2(-tN * al.x, -tS * al.y);
+    }
+    __syncthreads();
+    fft_rows<-1>(lds, 4, g.fft);
+    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
+    const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
+    for (int r = 0; r < 4; ++r)
+        unpack_sd(lds + r * n, n, L, w, S + so + r, D + so + r, (size_t)nmem * 4, threadIdx.x,
+                  kGridThreads);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -553,6 +600,77 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                    rk};
         rc = stage_ana<KindSwe>(md, sa, ep, syn_next, s);
     }
+    pr.end(kClsAnalysis, s);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// BVE adjoint launchers
+// ---------------------------------------------------------------------------
+int gauss_adjoint_prime(aeo_model_s* md, const double2* yfwd, const double2* kb, hipStream_t s) {
+    aeo_plan_s* p = md->plan;
+    const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
+    const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
+    StageArgs sa{};
+    sa.ys = yfwd;
+    sa.ys_mmajor = 1;
+    sa.ys_stride = fs;
+    sa.ys_aux = kb;
+    sa.aux_stride = fs;
+    sa.nmem = md->nmem;
+    sa.nstate_valid = 2;
+    sa.E = md->d_aE;
+    sa.O = md->d_aO;
+    sa.k = col_consts(p, md->R, md->d_lap, f10);
+    ProfRec& pr = md->prof;
+    pr.begin(kClsSynthesis, s);
+    int rc = launch_stage<KindBveAdj, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
+                                                             md->nslices);
+    pr.end(kClsSynthesis, s);
+    return rc;
+}
+
+int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt,
+                        const double2* a, double2* anext, double2* kout, double2* knext,
+                        const double2* fwd_next, hipStream_t s) {
+    aeo_plan_s* p = md->plan;
+    const int L = p->L;
+    const size_t fs = (size_t)(L + 1) * (L + 1);
+    const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
+    ProfRec& pr = md->prof;
+    const size_t lds = sizeof(double2) * 5 * p->nlon;
+    if (lds > 65536) {
+        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)grid_adj_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    pr.begin(kClsGrid, s);
+    hipLaunchKernelGGL(grid_adj_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
+                       p->gdev(), md->d_aE, md->d_aO, md->R, md->nmem, md->d_aS, md->d_aD);
+    pr.end(kClsGrid, s);
+    AEO_LAUNCH_CHECK();
+    StageArgs sa{};
+    sa.ys = nullptr;              // column 0 is filled by the epilogue (fwd_next)
+    sa.ys_mmajor = 1;
+    sa.ys_stride = fs;
+    sa.ys_aux = kb;
+    sa.aux_stride = fs;
+    sa.nmem = md->nmem;
+    sa.nstate_valid = 2;
+    sa.S = md->d_aS;
+    sa.D = md->d_aD;
+    sa.lmax_a = L;
+    sa.mmax_a = L;
+    sa.E = md->d_aE;
+    sa.O = md->d_aO;
+    sa.k = col_consts(p, md->R, md->d_lap, f10);
+    Rk4Adj rk{stage, dt, a, md->d_asum, anext, kout, knext};
+    EpiBveAdjS ep{L, md->R, fs, md->d_nulap, md->d_lap, p->d_Cp, p->d_Cm, fwd_next, rk};
+    int rc;
+    pr.begin(kClsAnalysis, s);
+    if (fwd_next)
+        rc = launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, md->mb, sa, ep, s, md->nslices);
+    else
+        rc = launch_stage<KindBveAdj, EpiBveAdjS, true, false>(p, md->mb, sa, ep, s, md->nslices);
     pr.end(kClsAnalysis, s);
     return rc;
 }

@@ -108,6 +108,25 @@ struct KindSwe {
     }
 };
 
+// BVE adjoint: the forward fields of the stage state (column 0; psi, S psi,
+// eta, S eta as in KindBve) and, from the tendency cotangent kb (column 1),
+// the synthesis input of the Jacobian cotangent
+//     jb_lm = -kb_lm / c_m   for 1 <= l <= cut, m <= cut,   0 elsewhere
+// (c_0 = 1, c_m = 2, cut = 2L/3).  Synthesising jb (which applies c_m) gives
+// the grid field jt with Ana^T(-trunc kb) = w jt, w the quadrature weight.
+struct KindBveAdj {
+    static constexpr int NSTATE = 2, NF = 5;
+    static constexpr int SF[5] = {-1, 0, -1, 2, -1};
+    __device__ static double2 base(int f, const double2* st, int l, int m, const ColK& k) {
+        if (f < 4) return KindBve::base(f, st, l, m, k);
+        const int cut = (2 * k.L) / 3;
+        if (l == 0 || l > cut || m > cut) return c2(0.0, 0.0);
+        const double s = (m == 0) ? -1.0 : -0.5;
+        const double2 v = st[(k.L + 1) + l];
+        return c2(s * v.x, s * v.y);
+    }
+};
+
 // Fill cols[f][i] (i = l - m) for the member's state column st.
 template <class KS>
 __device__ inline void derive_columns(const double2* st, int m, const ColK& k, double2* cols) {
@@ -248,12 +267,117 @@ struct EpiSweS {
     }
 };
 
+// Reverse-mode RK4 bookkeeping on one entry.  With a = cotangent of the step
+// output and yb_i = T'(y_i)^T kb_i the stage results (adjoint stage r runs
+// forward stage 3 - r):
+//   kb_4 = dt/6 a                       (set before stage 0)
+//   r=0: asum  = a + yb_4,  kb_3 = dt/3 a + dt yb_4
+//   r=1: asum += yb_3,      kb_2 = dt/3 a + dt/2 yb_3
+//   r=2: asum += yb_2,      kb_1 = dt/6 a + dt/2 yb_2
+//   r=3: anext = asum + yb_1 (cotangent of the step input), kb_4' = dt/6 anext
+// Only the writer tile touches asum in r = 0..2; r = 3 reads it in every tile
+// and writes nothing but anext.  Returns the next stage's kb.
+struct Rk4Adj {
+    int stage;          // 0..3; -1: tendency adjoint only (kout = yb)
+    double dt;
+    const double2* a;
+    double2* asum;
+    double2* anext;
+    double2* kout;
+    double2* knext;     // next kb (global copy for the next launch's epilogue)
+    __device__ double2 apply(size_t idx, double2 yb, bool writer) const {
+        if (stage < 0) {
+            if (writer) kout[idx] = yb;
+            return c2(0.0, 0.0);
+        }
+        const double2 av = a[idx];
+        double2 nx;
+        if (stage == 0) {
+            if (writer) asum[idx] = cadd(av, yb);
+            nx = cadd(cscale(av, dt / 3.0), cscale(yb, dt));
+        } else if (stage == 1) {
+            if (writer) asum[idx] = cadd(asum[idx], yb);
+            nx = cadd(cscale(av, dt / 3.0), cscale(yb, 0.5 * dt));
+        } else if (stage == 2) {
+            if (writer) asum[idx] = cadd(asum[idx], yb);
+            nx = cadd(cscale(av, dt / 6.0), cscale(yb, 0.5 * dt));
+        } else {
+            const double2 tot = cadd(asum[idx], yb);
+            if (writer) anext[idx] = tot;
+            nx = cscale(tot, dt / 6.0);
+        }
+        if (writer && knext) knext[idx] = nx;
+        return nx;
+    }
+};
+
+// Transpose of one BVE stage k = -trunc Ana(J(psi, eta)) + nu lap' zeta,
+// row 0 pinned, psi = zeta / lap (l >= 1), eta = zeta + f.  The four analysed
+// columns are weighted analyses of the cotangents of the Jacobian's grid
+// inputs (acc 0: a_l, 1: a_s, 2: b_l, 3: b_s); then
+//   psib = c_m ( -(i m / R) A_l + S^T A_s / R ),
+//   etab = c_m ( -(i m / R) B_l + S^T B_s / R ),
+//   zb   = nu lap' kb' + psib / lap (l >= 1) + etab,     kb' = kb with row 0 = 0,
+// with (S^T x)_l = C+_l x_{l+1} + C-_l x_{l-1} (l-1 >= m).  State column 0
+// receives the forward state of the next adjoint stage, column 1 its kb.
+template <class T, class = void>
+struct epi_column { static constexpr bool value = false; };
+template <class T>
+struct epi_column<T, decltype((void)T::COLUMN)> { static constexpr bool value = T::COLUMN; };
+
+struct EpiBveAdjS {
+    static constexpr int NA = 4, NSTATE = 2;
+    static constexpr bool COLUMN = true;
+    int L;
+    double R;
+    size_t member_stride;
+    const double* nulap;
+    const double* lap;
+    const double* Cp;
+    const double* Cm;
+    const double2* fwd_next;   // forward stage state of the next adjoint stage, or nullptr
+    Rk4Adj rk;
+    __device__ double2 st_term(const double2* x, int l, int m) const {
+        const int n = L + 1;
+        double2 v = c2(0.0, 0.0);
+        if (l >= m) {
+            if (l + 1 <= L) v = cadd(v, cscale(x[l + 1], Cp[l * n + m]));
+            if (l - 1 >= m) v = cadd(v, cscale(x[l - 1], Cm[l * n + m]));
+        }
+        return v;
+    }
+    // acc: the member's analysed columns [NA][n]; st: its state columns [2][n]
+    __device__ void apply_col(int b, int l, int m, const double2* acc, double2* st,
+                              bool writer) const {
+        const int n = L + 1;
+        const size_t idx = b * member_stride + (size_t)m * n + l;   // internal [m][l]
+        const double q = (double)m / R, invR = 1.0 / R;
+        const double cm = (m == 0) ? 1.0 : 2.0;
+        const double2 al = acc[l], bl = acc[2 * n + l];
+        const double2 sa = st_term(acc + n, l, m), sb = st_term(acc + 3 * n, l, m);
+        // -(i q) x = (q x.y, -q x.x)
+        const double2 psib = cscale(c2(q * al.y + invR * sa.x, -q * al.x + invR * sa.y), cm);
+        const double2 etab = cscale(c2(q * bl.y + invR * sb.x, -q * bl.x + invR * sb.y), cm);
+        double2 zb = etab;
+        if (l > 0) {
+            const double d = lap[l];
+            zb = cadd(zb, c2(psib.x / d, psib.y / d));
+            zb = cadd(zb, cscale(st[n + l], nulap[l]));
+        }
+        const double2 kn = rk.apply(idx, zb, writer);
+        st[l] = fwd_next ? fwd_next[idx] : c2(0.0, 0.0);
+        st[n + l] = kn;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // The stage kernel.
 // ---------------------------------------------------------------------------
 struct StageArgs {
-    const double2* ys;      // stage input states [member][NSTATE][n][n]
+    const double2* ys;      // stage input states [member][NSTATE][n][n] (nullptr: zeros)
     size_t ys_stride;       // complex entries per member
+    const double2* ys_aux;  // if set: state fields 1.. come from here instead
+    size_t aux_stride;      // complex entries per member of ys_aux
     int ys_mmajor;          // 1: ys is [..][m][l] (internal layout); 0: [..][l][m]
     int nmem;
     int nstate_valid;       // state fields present in ys (<= NSTATE)
@@ -319,9 +443,13 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         const int s = r / n, l = r - s * n;
         const int b = b0 + bb;
         double2 v = c2(0.0, 0.0);
-        if (b < a.nmem && s < a.nstate_valid)
-            v = a.ys[b * a.ys_stride + (size_t)s * n * n +
-                     (a.ys_mmajor ? (size_t)m * n + l : (size_t)l * n + m)];
+        if (b < a.nmem && s < a.nstate_valid) {
+            const size_t e = a.ys_mmajor ? (size_t)m * n + l : (size_t)l * n + m;
+            if (s >= 1 && a.ys_aux)
+                v = a.ys_aux[b * a.aux_stride + (size_t)(s - 1) * n * n + e];
+            else if (a.ys)
+                v = a.ys[b * a.ys_stride + (size_t)s * n * n + e];
+        }
         st[t] = v;
     }
     __syncthreads();
@@ -491,10 +619,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             const int bb = t / n, l = t - bb * n;
             const int b = b0 + bb;
             if (b >= a.nmem) continue;
-            double2 acc[NA];
+            if constexpr (epi_column<Epi>::value) {
+                ep.apply_col(b, l, m, accl + (size_t)bb * NA * n, st + (size_t)bb * NST * n,
+                             writer);
+            } else {
+                double2 acc[NA];
 #pragma unroll
-            for (int f = 0; f < NA; ++f) acc[f] = accl[(bb * NA + f) * n + l];
-            ep.apply(b, l, m, acc, st + (size_t)bb * NST * n, writer);
+                for (int f = 0; f < NA; ++f) acc[f] = accl[(bb * NA + f) * n + l];
+                ep.apply(b, l, m, acc, st + (size_t)bb * NST * n, writer);
+            }
         }
         __syncthreads();
     }

@@ -146,6 +146,14 @@ struct aeo_model_s {
     double2* d_ybuf = nullptr;     // 2 slots (ping-pong between steps)
     double2* d_ksum = nullptr;
     double2* d_fint = nullptr;     // forcing in the internal [m][l] layout
+    // adjoint workspaces (allocated on the first adjoint call)
+    double2* d_aE = nullptr;       // 5 synthesis columns per member
+    double2* d_aO = nullptr;
+    double2* d_aS = nullptr;       // 4 analysis columns per member
+    double2* d_aD = nullptr;
+    double2* d_asum = nullptr;
+    double2* d_abuf = nullptr;     // 2 slots: step-end / step-start cotangents
+    double2* d_kbuf = nullptr;     // 2 slots: stage tendency cotangents
     int nmem = 1;                  // members (independent states) in the group
     int mb = 1;                    // members per workgroup in the Legendre stage kernel
     int nslices = 0;               // 0: FP64 contractions; > 0: split BF16 MFMA digits
@@ -170,6 +178,16 @@ int gauss_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
                       double2* ynext, bool syn_next, hipStream_t s);
 
+// Adjoint of the BVE stage (Gauss plans).  prime: synthesis of the forward
+// fields of yfwd and of the tendency cotangent kb into the adjoint E/O
+// workspace.  stage: Fourier adjoint + analysis and the reverse RK4
+// bookkeeping of adjoint stage r (0..3; -1: yb of one tendency into kout);
+// with fwd_next set the same launch synthesises the next adjoint stage.
+int gauss_adjoint_prime(aeo_model_s* m, const double2* yfwd, const double2* kb, hipStream_t s);
+int gauss_adjoint_stage(aeo_model_s* m, const double2* kb, int stage, double dt,
+                        const double2* a, double2* anext, double2* kout, double2* knext,
+                        const double2* fwd_next, hipStream_t s);
+
 int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s);
 int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
 int points_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
@@ -181,6 +199,8 @@ int points_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
 
 int launch_spec_op(int op, int L, double R, const double2* in, double2* out, int nf,
                    hipStream_t s);
+// out = alpha * in, n complex entries.
+int launch_scale(const double2* in, double2* out, double alpha, size_t n, hipStream_t s);
 // out[k] = transpose(in[k]) for nmat (n x n) complex matrices (in != out).
 int launch_transpose(const double2* in, double2* out, int n, int nmat, hipStream_t s);
 int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf, double* out,

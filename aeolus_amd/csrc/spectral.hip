@@ -211,3 +211,23 @@ int launch_transpose(const double2* in, double2* out, int n, int nmat, hipStream
 }
 
 }  // namespace aeo
+
+namespace aeo {
+
+__global__ void scale_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                             double alpha, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = cscale(in[i], alpha);
+}
+
+int launch_scale(const double2* in, double2* out, double alpha, size_t n, hipStream_t s) {
+    if (n == 0) return AEO_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, s, in, out, alpha, n);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+}  // namespace aeo

@@ -120,6 +120,53 @@ class BarotropicVorticity:
                                           y.data_ptr(), nat.stream()), "rk4")
         return y
 
+    # -- adjoint (no reference counterpart) ----------------------------------
+    def checkpoint_elems(self, nsteps: int) -> int:
+        """complex128 entries of the stage checkpoints of an ``nsteps`` run."""
+        n = ctypes.c_longlong()
+        nat.check(nat.lib().aeo_model_checkpoint_elems(self._model, int(nsteps), ctypes.byref(n)),
+                  "checkpoint_elems")
+        return int(n.value)
+
+    def rk4_record(self, coeffs, dt: float, nsteps: int = 1, forcing_coeffs=None,
+                   out: torch.Tensor | None = None):
+        """:meth:`rk4` (bit-identical result) that also keeps the input state of
+        every RK4 stage; returns ``(y, checkpoints)`` for :meth:`rk4_adjoint`."""
+        z = self._coeffs(coeffs)
+        y = torch.empty_like(z) if out is None else out
+        ck = torch.empty(max(self.checkpoint_elems(nsteps), 1), dtype=torch.complex128,
+                         device=self.device)
+        frc = None if forcing_coeffs is None else self._coeffs(forcing_coeffs)
+        nat.check(nat.lib().aeo_model_rk4_record(self._model, z.data_ptr(), float(dt), int(nsteps),
+                                                 None if frc is None else frc.data_ptr(),
+                                                 y.data_ptr(), ck.data_ptr(), nat.stream()),
+                  "rk4_record")
+        return y, ck
+
+    def rk4_adjoint(self, checkpoints: torch.Tensor, dt: float, nsteps: int, ybar_out,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+        """Cotangent of the initial state of a recorded run from the cotangent
+        ``ybar_out`` of its final state (real inner product of the coefficient
+        arrays, the torch.autograd convention for complex tensors)."""
+        if checkpoints.numel() < self.checkpoint_elems(nsteps):
+            raise ValueError("rk4_adjoint: checkpoint buffer is smaller than the recorded run")
+        g = self._coeffs(ybar_out)
+        res = torch.empty_like(g) if out is None else out
+        nat.check(nat.lib().aeo_model_rk4_adjoint(self._model, checkpoints.data_ptr(), float(dt),
+                                                  int(nsteps), g.data_ptr(), res.data_ptr(),
+                                                  nat.stream()), "rk4_adjoint")
+        return res
+
+    def tendency_adjoint(self, coeffs, kbar) -> torch.Tensor:
+        """``T'(coeffs)^T kbar`` for one tendency evaluation."""
+        z = self._coeffs(coeffs)
+        k = self._coeffs(kbar)
+        out = torch.empty_like(z)
+        nat.check(nat.lib().aeo_model_tendency_adjoint(self._model, z.data_ptr(), k.data_ptr(),
+                                                       out.data_ptr(), nat.stream()),
+                  "tendency_adjoint")
+        return out
+
 
 class BarotropicBatch(BarotropicVorticity):
     """``nmembers`` independent BVE states on one Gauss product space, stepped

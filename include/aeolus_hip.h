@@ -184,6 +184,34 @@ int aeo_model_tendency(aeo_model_t model, const double* y, const double* forcing
 int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
                   const double* forcing, double* y_out, void* stream);
 
+/* Adjoint of the BVE RK4 step (Gauss plans; models from aeo_bve_create or
+ * aeo_bve_create_batch).  No counterpart in the reference: its only
+ * adjoint-flavoured primitive is adjoint_sin_theta_d_theta_coeffs
+ * (num/spectral_operators.py:243-270), which these kernels apply in place.
+ *
+ * Cotangents use the real Euclidean inner product of the dense (L+1, L+1)
+ * complex128 arrays, <x, y> = sum Re(conj(x) y) (the convention of
+ * torch.autograd for complex tensors).
+ *
+ * aeo_model_checkpoint_elems: complex128 entries a checkpoint buffer for
+ * nsteps steps needs (nsteps * 4 * members * (L+1)^2).
+ * aeo_model_rk4_record: aeo_model_rk4 (bit-identical y_out) that also writes
+ * the input state of every RK4 stage into `checkpoints` (device, caller
+ * owned, opaque internal layout).
+ * aeo_model_rk4_adjoint: given ybar_out = dJ/dy_out, ybar = dJ/dy for the
+ * recorded run (same model, dt, nsteps; the constant forcing drops out).
+ * ybar may equal ybar_out.
+ * aeo_model_tendency_adjoint: ybar = T'(y)^T kbar for one tendency call
+ * (forcing drops out). */
+int aeo_model_checkpoint_elems(aeo_model_t model, int nsteps, long long* out);
+int aeo_model_rk4_record(aeo_model_t model, const double* y, double dt, int nsteps,
+                         const double* forcing, double* y_out, double* checkpoints,
+                         void* stream);
+int aeo_model_rk4_adjoint(aeo_model_t model, const double* checkpoints, double dt, int nsteps,
+                          const double* ybar_out, double* ybar, void* stream);
+int aeo_model_tendency_adjoint(aeo_model_t model, const double* y, const double* kbar,
+                               double* ybar, void* stream);
+
 /* Bracket every launch of one kernel class of the model's stages with a
  * pair of HIP events on the launch stream (0 = Legendre synthesis, 1 = FFT
  * grid stage with the pointwise products, 2 = Legendre analysis with the

@@ -1,0 +1,216 @@
+"""HIP adjoint of the BVE RK4 step against the oracle's adjoint, dot-product
+tests against the tangent-linear step, and the autograd gradient against
+central finite differences.
+
+Bars: FP64 adjoint tendency / step within 1e-12 relative L2 of the oracle
+(itself pinned on the CPU); FP64 dot-product identities within 1e-12; the
+gradient of J = 1/2 ||zeta(T) - target||^2 within 1e-6 of central
+differences (eps = 1e-7 relative to ||zeta(0)||) along 8 seeded unit
+directions, FP64 and split precision.
+"""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+R = 6.371e6
+OMEGA = 2.0 * math.pi / 86400.0
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _planets(L, nlat, nlon, oracle=True):
+    from aeolus_amd.planet import Planet
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R, OMEGA, grid_type="latlon", nlat=nlat, nlon=nlon, l_max=L,
+                                product_quadrature="fine")
+    if not oracle:
+        return pl, None
+    from oracle.models import Planet as OPlanet
+    return pl, OPlanet(l_max=L, radius=R, omega=OMEGA, grid_type="latlon", nlat=nlat, nlon=nlon)
+
+
+def _field(L, seed, rms=5e-5, full=False):
+    rng = np.random.default_rng(seed)
+    l = np.arange(L + 1, dtype=np.float64)
+    amp = np.zeros(L + 1)
+    amp[1:] = l[1:] ** -1.5
+    a = amp[:, None] * np.exp(1j * rng.uniform(0, 2 * math.pi, (L + 1, L + 1)))
+    if full:   # every entry, including m > l and Im(a_l0)
+        a = a + 1e-2 * (rng.standard_normal(a.shape) + 1j * rng.standard_normal(a.shape))
+    else:
+        a[:, 0] = a[:, 0].real
+        li, mi = np.indices(a.shape)
+        a = np.where(mi <= li, a, 0.0)
+        a[0, :] = 0.0
+    return a * (rms / np.sqrt(np.sum(np.abs(a) ** 2)))
+
+
+def _inner(x, y):
+    import torch
+    return float(torch.sum((torch.conj(x) * y).real))
+
+
+@pytest.mark.parametrize("L,nlat,nlon,nu", [(21, 32, 64, 0.0), (42, 64, 128, 1e5)])
+def test_tendency_adjoint_matches_oracle(L, nlat, nlon, nu):
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from oracle.adjoint import bve_tendency_T
+    print(f"case tendency adjoint L={L}", flush=True)
+    pl, op = _planets(L, nlat, nlon)
+    model = BarotropicVorticity(pl, viscosity=nu)
+    z = _field(L, 1)
+    kb = _field(L, 2, full=True)
+    got = _np(model.tendency_adjoint(torch.as_tensor(z, device="cuda"),
+                                     torch.as_tensor(kb, device="cuda")))
+    ref = bve_tendency_T(op, z, kb, nu)
+    assert rel_l2(got, ref) < 1e-12
+
+
+def test_record_is_bit_identical_and_adjoint_matches_oracle():
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from oracle import adjoint as adj
+    print("case rk4 adjoint L=21 3 steps", flush=True)
+    L, nu, dt, ns = 21, 1e5, 900.0, 3
+    pl, op = _planets(L, 32, 64)
+    model = BarotropicVorticity(pl, viscosity=nu)
+    z = _field(L, 3)
+    zt = torch.as_tensor(z, device="cuda")
+    y_plain = model.rk4(zt, dt, ns)
+    y_rec, ck = model.rk4_record(zt, dt, ns)
+    assert torch.equal(y_plain, y_rec)
+    yb = _field(L, 4, full=True)
+    got = _np(model.rk4_adjoint(ck, dt, ns, torch.as_tensor(yb, device="cuda")))
+    T, _, TT = adj.tendency_funcs(op, nu)
+    y, stages = z, []
+    for _ in range(ns):
+        y, s = adj.rk4_step_record(T, y, dt)
+        stages.append(s)
+    ref = yb
+    for s in reversed(stages):
+        ref = adj.rk4_step_T(TT, s, ref, dt)
+    assert rel_l2(got, ref) < 1e-12
+
+
+def _tl_step(model, y, d, dt):
+    """Tangent-linear RK4 step from the device tendency: the BVE tendency is
+    quadratic plus linear, so (T(y + s d) - T(y - s d)) / (2 s) = T'(y) d
+    holds exactly up to rounding for any s."""
+    from aeolus_amd.physics import BarotropicState
+
+    def T(x):
+        return model.tendency(BarotropicState(x))
+
+    def Tl(x, e):
+        s = float(x.abs().norm() / max(float(e.abs().norm()), 1e-300))
+        return (T(x + s * e) - T(x - s * e)) / (2.0 * s)
+
+    k1, d1 = T(y), Tl(y, d)
+    y2, e2 = y + 0.5 * dt * k1, d + 0.5 * dt * d1
+    k2, d2 = T(y2), Tl(y2, e2)
+    y3, e3 = y + 0.5 * dt * k2, d + 0.5 * dt * d2
+    k3, d3 = T(y3), Tl(y3, e3)
+    y4, e4 = y + dt * k3, d + dt * d3
+    k4, d4 = T(y4), Tl(y4, e4)
+    return (y + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4),
+            d + (dt / 6.0) * (d1 + 2 * d2 + 2 * d3 + d4))
+
+
+@pytest.mark.parametrize("L,nlat,nlon", [(42, 64, 128), (85, 128, 256)])
+def test_dot_product_tendency_and_step(L, nlat, nlon):
+    import torch
+    from aeolus_amd.physics import BarotropicState, BarotropicVorticity
+    print(f"case dot product L={L}", flush=True)
+    pl, _ = _planets(L, nlat, nlon, oracle=False)
+    model = BarotropicVorticity(pl, viscosity=5e4)
+    dev = "cuda"
+    y = torch.as_tensor(_field(L, 5), device=dev)
+    d = torch.as_tensor(_field(L, 6), device=dev)
+    kb = torch.as_tensor(_field(L, 7, full=True), device=dev)
+    T = lambda x: model.tendency(BarotropicState(x))
+    s = float(y.abs().norm() / d.abs().norm())
+    tl = (T(y + s * d) - T(y - s * d)) / (2.0 * s)
+    lhs, rhs = _inner(tl, kb), _inner(d, model.tendency_adjoint(y, kb))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+    dt, ns = 600.0, 2
+    yy, dd = y, d
+    for _ in range(ns):
+        yy, dd = _tl_step(model, yy, dd, dt)
+    _, ck = model.rk4_record(y, dt, ns)
+    yb = torch.as_tensor(_field(L, 8, full=True), device=dev)
+    lhs, rhs = _inner(dd, yb), _inner(d, model.rk4_adjoint(ck, dt, ns, yb))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+
+
+def _fd_check(model, L, dt, nsteps, ndir=8):
+    import torch
+    from aeolus_amd.run.adjoint import rollout, sphere_misfit
+    dev = "cuda"
+    z0 = torch.as_tensor(_field(L, 11), device=dev)
+    target = torch.as_tensor(_field(L, 12), device=dev)
+    z = z0.clone().requires_grad_(True)
+    J = sphere_misfit(rollout(model, z, dt, nsteps), target)
+    J.backward()
+    g = z.grad.detach()
+    eps = 1e-7 * float(z0.abs().norm())
+    worst = 0.0
+    for k in range(ndir):
+        d = torch.as_tensor(_field(L, 100 + k), device=dev)
+        d = d / d.abs().norm()
+        jp = float(sphere_misfit(model.rk4(z0 + eps * d, dt, nsteps), target))
+        jm = float(sphere_misfit(model.rk4(z0 - eps * d, dt, nsteps), target))
+        fd = (jp - jm) / (2.0 * eps)
+        ad = _inner(g, d)
+        worst = max(worst, abs(fd - ad) / max(abs(fd), abs(ad)))
+    return worst
+
+
+@pytest.mark.parametrize("nslices", [0, 6])
+def test_autograd_gradient_matches_finite_differences(nslices):
+    from aeolus_amd.physics import BarotropicVorticity
+    print(f"case autograd fd L=42 nslices={nslices}", flush=True)
+    pl, _ = _planets(42, 64, 128, oracle=False)
+    model = BarotropicVorticity(pl)
+    model.set_precision(nslices)
+    worst = _fd_check(model, 42, 600.0, 12)
+    print(f"worst fd mismatch {worst:.3e}", flush=True)
+    assert worst < 1e-6
+
+
+def test_c5_adjoint_gradient_l170_split():
+    """L = 170 on Gauss 256x512, split-precision transforms with the FP64
+    state, 48 RK4 steps of 300 s, 8 seeded directions."""
+    from aeolus_amd.physics import BarotropicVorticity
+    print("case autograd fd L=170 nslices=6 48 steps", flush=True)
+    pl, _ = _planets(170, 256, 512, oracle=False)
+    model = BarotropicVorticity(pl)
+    model.set_precision(6)
+    worst = _fd_check(model, 170, 300.0, 48)
+    print(f"worst fd mismatch {worst:.3e}", flush=True)
+    assert worst < 1e-6
+
+
+def test_batch_adjoint_matches_single_members():
+    import torch
+    from aeolus_amd.physics import BarotropicBatch, BarotropicVorticity
+    print("case batch adjoint nmem=3", flush=True)
+    L, dt, ns = 42, 600.0, 3
+    pl, _ = _planets(L, 64, 128, oracle=False)
+    batch = BarotropicBatch(pl, 3, viscosity=1e5)
+    single = BarotropicVorticity(pl, viscosity=1e5)
+    z = torch.as_tensor(np.stack([_field(L, 20 + k) for k in range(3)]), device="cuda")
+    yb = torch.as_tensor(np.stack([_field(L, 30 + k, full=True) for k in range(3)]), device="cuda")
+    _, ck = batch.rk4_record(z, dt, ns)
+    g = batch.rk4_adjoint(ck, dt, ns, yb)
+    for k in range(3):
+        _, ck1 = single.rk4_record(z[k], dt, ns)
+        assert torch.equal(g[k], single.rk4_adjoint(ck1, dt, ns, yb[k]))
