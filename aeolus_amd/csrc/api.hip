@@ -194,6 +194,12 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
         p->fft.nstages = nst;
         for (int i = 0; i < nst; ++i) p->fft.radix[i] = radices[i];
         p->fft.tw = p->d_tw;
+        p->fft.lgn = -1;
+        if ((nlon & (nlon - 1)) == 0) {
+            int lg = 0;
+            while ((1 << lg) < nlon) ++lg;
+            p->fft.lgn = lg;
+        }
         if ((rc = gauss_build_tables(p, d_x, nullptr))) break;
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) {

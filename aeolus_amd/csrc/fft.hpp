@@ -22,11 +22,17 @@ namespace aeo {
 
 constexpr int kChunkElems = 2048;  // complex values per chunk (256 threads x 8)
 
+// LDS rows are padded by one element per 8 (element i of a row at i + i/8,
+// row stride n + n/8): Stockham writes at stride 8 then spread over all banks.
+__host__ __device__ inline int lds_row_stride(int n) { return n + (n >> 3); }
+__device__ inline int lp(int i) { return i + (i >> 3); }
+
 struct FftDev {
     int n;
     int nstages;
     int radix[kMaxRadices];
     const double2* tw;  // tw[t] = exp(-2 pi i t / n), t in [0, n)
+    int lgn;            // log2(n) when n is a power of two (shift/mask stages), else -1
 };
 
 template <int DIR>
@@ -64,6 +70,25 @@ __device__ inline void butterfly(double2* v) {
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, t);
         v[2] = csub(m, t);
+    } else if constexpr (R == 8) {
+        // two 4-point DFTs (even / odd inputs) joined with w^q, w = e^{DIR 2 pi i / 8}
+        const double h = 0.70710678118654752440;  // 1/sqrt(2)
+        double2 e[4] = {v[0], v[2], v[4], v[6]};
+        double2 o[4] = {v[1], v[3], v[5], v[7]};
+        butterfly<DIR, 4>(e);
+        butterfly<DIR, 4>(o);
+        // o1 (1 + DIR i)/sqrt2, o2 DIR i, o3 (-1 + DIR i)/sqrt2
+        const double2 t1 = c2(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+        const double2 t2 = mul_i<DIR>(o[2]);
+        const double2 t3 = c2(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], t1);
+        v[5] = csub(e[1], t1);
+        v[2] = cadd(e[2], t2);
+        v[6] = csub(e[2], t2);
+        v[3] = cadd(e[3], t3);
+        v[7] = csub(e[3], t3);
     } else if constexpr (R == 5) {
         const double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
         const double c2_ = -0.80901699437494742410; // cos(4 pi / 5)
@@ -99,10 +124,10 @@ __device__ inline void fft_stage(double2* buf, int nrows, int n, int Ns, const d
             const int row = t / nb;
             const int j = t - row * nb;
             const int k = j % Ns;
-            const double2* rb = buf + row * n;
+            const double2* rb = buf + row * lds_row_stride(n);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = rb[j + r * nb];
+                double2 x = rb[lp(j + r * nb)];
                 if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
                 v[b][r] = x;
             }
@@ -117,10 +142,10 @@ __device__ inline void fft_stage(double2* buf, int nrows, int n, int Ns, const d
             const int j = t - row * nb;
             const int k = j % Ns;
             butterfly<DIR, R>(v[b]);
-            double2* rb = buf + row * n;
+            double2* rb = buf + row * lds_row_stride(n);
             const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) rb[base + q * Ns] = v[b][q];
+            for (int q = 0; q < R; ++q) rb[lp(base + q * Ns)] = v[b][q];
         }
     }
     __syncthreads();
@@ -145,9 +170,9 @@ __device__ inline void fft_stage_generic(double2* buf, int nrows, int n, int Ns,
             const int k = p % Ns;
             const int q = (p / Ns) % R;
             const int j = (p / (Ns * R)) * Ns + k;
-            const double2* rb = buf + row * n;
+            const double2* rb = buf + row * lds_row_stride(n);
             for (int r = 0; r < R; ++r) {
-                double2 x = rb[j + r * nb];
+                double2 x = rb[lp(j + r * nb)];
                 if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
                 const int e = ((r * q) % R) * nb;  // W_R^{rq} = W_n^{(rq mod R) n/R}
                 acc = cadd(acc, e ? cmul(x, twiddle<DIR>(tw, e)) : x);
@@ -159,34 +184,112 @@ __device__ inline void fft_stage_generic(double2* buf, int nrows, int n, int Ns,
 #pragma unroll
     for (int b = 0; b < MAXO; ++b) {
         const int t = threadIdx.x + b * kGridThreads;
-        if (t < total) buf[t] = out[b];
+        if (t < total) {
+            const int row = t / n;
+            buf[row * lds_row_stride(n) + lp(t - row * n)] = out[b];
+        }
     }
     __syncthreads();
 }
 
-// Transform `nrows` contiguous rows of length fp.n in LDS (in place).
+// Power-of-two stage: n = 2^lgn, Ns = 2^lgNs, all index arithmetic by shifts
+// and masks; rows * n / R butterflies spread over the workgroup.
+template <int DIR, int R, int LGR>
+__device__ inline void fft_stage_p2(double2* buf, int nrows, int lgn, int lgNs, const double2* tw) {
+    constexpr int MAXB = (kChunkElems / R + kGridThreads - 1) / kGridThreads;
+    const int lgnb = lgn - LGR;
+    const int total = nrows << lgnb;
+    const int nmask = (1 << lgnb) - 1, kmask = (1 << lgNs) - 1;
+    const int lgspan = lgn - lgNs - LGR;
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < total) {
+            const int row = t >> lgnb;
+            const int j = t & nmask;
+            const int k = j & kmask;
+            const double2* rb = buf + row * lds_row_stride(1 << lgn);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = rb[lp(j + (r << lgnb))];
+                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, (r * k) << lgspan));
+                v[b][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < total) {
+            const int row = t >> lgnb;
+            const int j = t & nmask;
+            const int k = j & kmask;
+            butterfly<DIR, R>(v[b]);
+            double2* rb = buf + row * lds_row_stride(1 << lgn);
+            const int base = ((j >> lgNs) << (lgNs + LGR)) + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) rb[lp(base + (q << lgNs))] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Copy the twiddle table into LDS (n entries at tw_lds), so no stage waits
+// on a global load; pass tw_lds to fft_rows.  The caller synchronises the
+// workgroup before the first stage.
+__device__ inline void fft_twiddles_to_lds(const FftDev& fp, double2* tw_lds) {
+    for (int t = threadIdx.x; t < fp.n; t += kGridThreads) tw_lds[t] = fp.tw[t];
+}
+
+// Transform `nrows` rows of length fp.n in LDS (in place), stored with the
+// padded layout above (row r, element i at r * lds_row_stride(n) + lp(i));
+// tw: the twiddle table (global or an LDS copy).
 // DIR = -1: X_m = sum_k x_k e^{-2 pi i m k / n};  DIR = +1: unnormalised inverse.
 // Must be called by all threads of the (kGridThreads-wide) workgroup.
-template <int DIR>
-__device__ void fft_rows(double2* buf, int nrows, const FftDev& fp) {
+// P2: n is a power of two <= kChunkElems (fp.lgn >= 0), compiled without the
+// general-radix stages.
+template <int DIR, bool P2>
+__device__ void fft_rows(double2* buf, int nrows, const FftDev& fp, const double2* tw) {
     const int n = fp.n;
     const int cb = (n <= kChunkElems) ? (kChunkElems / n) : 1;
+    if constexpr (P2) {
+        // radix-8 stages, then one radix-4 or radix-2 stage for the remainder
+        for (int r0 = 0; r0 < nrows; r0 += cb) {
+            const int nr = (nrows - r0 < cb) ? (nrows - r0) : cb;
+            double2* b = buf + r0 * lds_row_stride(n);
+            int lgNs = 0;
+            while (fp.lgn - lgNs >= 3) {
+                fft_stage_p2<DIR, 8, 3>(b, nr, fp.lgn, lgNs, tw);
+                lgNs += 3;
+            }
+            if (fp.lgn - lgNs == 2) fft_stage_p2<DIR, 4, 2>(b, nr, fp.lgn, lgNs, tw);
+            if (fp.lgn - lgNs == 1) fft_stage_p2<DIR, 2, 1>(b, nr, fp.lgn, lgNs, tw);
+        }
+        return;
+    } else {
     for (int r0 = 0; r0 < nrows; r0 += cb) {
         const int nr = (nrows - r0 < cb) ? (nrows - r0) : cb;
-        double2* b = buf + r0 * n;
+        double2* b = buf + r0 * lds_row_stride(n);
         int Ns = 1;
         for (int s = 0; s < fp.nstages; ++s) {
             const int R = fp.radix[s];
             switch (R) {
-                case 2: fft_stage<DIR, 2>(b, nr, n, Ns, fp.tw); break;
-                case 3: fft_stage<DIR, 3>(b, nr, n, Ns, fp.tw); break;
-                case 4: fft_stage<DIR, 4>(b, nr, n, Ns, fp.tw); break;
-                case 5: fft_stage<DIR, 5>(b, nr, n, Ns, fp.tw); break;
-                default: fft_stage_generic<DIR>(b, nr, n, Ns, R, fp.tw); break;
+                case 2: fft_stage<DIR, 2>(b, nr, n, Ns, tw); break;
+                case 3: fft_stage<DIR, 3>(b, nr, n, Ns, tw); break;
+                case 4: fft_stage<DIR, 4>(b, nr, n, Ns, tw); break;
+                case 5: fft_stage<DIR, 5>(b, nr, n, Ns, tw); break;
+                default: fft_stage_generic<DIR>(b, nr, n, Ns, R, tw); break;
             }
             Ns *= R;
         }
     }
+    }
+}
+
+__host__ __device__ inline bool fft_p2(const FftDev& fp) {
+    return fp.lgn >= 0 && fp.n <= kChunkElems;
 }
 
 }  // namespace aeo

@@ -22,6 +22,12 @@ namespace aeo {
 
 constexpr int kRowElems = 4096;   // complex values per LDS pass of the generic grid kernels
 
+// Rows per LDS pass of the generic grid kernels (the twiddle table takes one row).
+__host__ __device__ inline int row_cap(int n) {
+    const int c = (kRowElems - n) / lds_row_stride(n);
+    return c > 0 ? c : 1;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 normalised associated-Legendre recurrence (one thread per (m, jp)).
 //   Pbar_00 = 1/sqrt(4 pi)
@@ -116,7 +122,8 @@ __global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
 }
 
 // ---------------------------------------------------------------------------
-// Fourier stage helpers.
+// Fourier stage helpers.  Grid rows live in LDS with the padded layout of
+// fft.hpp: row r, longitude k at r * lds_row_stride(n) + lp(k).
 // ---------------------------------------------------------------------------
 enum SpecMode { kSpecPlain = 0, kSpecDlam = 1, kSpecScale = 2 };
 
@@ -148,7 +155,7 @@ __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er
             else if (mode == kSpecScale) { gn = cscale(gn, invR); gs = cscale(gs, invR); }
             z = cadd(z, c2(gn.x + gs.y, -gn.y + gs.x));
         }
-        Z[b] = z;
+        Z[lp(b)] = z;
     }
 }
 
@@ -159,8 +166,8 @@ __device__ inline void unpack_sd(const double2* Z, int n, int mmax, double w, do
     for (int m = t0; m <= mmax; m += tstride) {
         const int b = m % n;
         const int bn = (n - b) % n;
-        const double2 zb = Z[b];
-        const double2 zn = cconj(Z[bn]);
+        const double2 zb = Z[lp(b)];
+        const double2 zn = cconj(Z[lp(bn)]);
         const double2 fn = cscale(cadd(zb, zn), 0.5);
         const double2 dd = csub(zb, zn);
         const double2 fs = c2(0.5 * dd.y, -0.5 * dd.x);  // (zb - zn) / (2i)
@@ -172,28 +179,67 @@ __device__ inline void unpack_sd(const double2* Z, int n, int mmax, double w, do
 // Intermediate layout of every grid kernel: E/O/S/D[jp][m][column], one
 // column per (member, field) pair, `ncol` columns in all.
 
+// Prologue of the fused grid kernels: the twiddle table (n entries at twl)
+// and the NF even/odd columns of one member of a row pair (sE/sO[m * NF + f],
+// m <= L) into LDS.  Each thread issues all of its loads before its first
+// LDS store, so the gather costs one memory latency.
+template <int NF>
+__device__ inline void grid_prologue(const GaussDev& g, const double2* __restrict__ E,
+                                       const double2* __restrict__ O, size_t eo, size_t ms,
+                                       double2* twl, double2* sE, double2* sO) {
+    // loads are unconditional (indices clamped in range) so the values stay
+    // in registers; only the LDS stores are predicated
+    constexpr int PER = 4;
+    const int n = g.nlon;
+    const int tot = (g.L + 1) * NF;
+    for (int t0 = 0; t0 < tot || t0 < n; t0 += PER * kGridThreads) {
+        double2 ve[PER], vo[PER], vt[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = t0 + threadIdx.x + u * kGridThreads;
+            const int te = (t < tot) ? t : tot - 1;
+            const int m = te / NF, f = te - m * NF;
+            ve[u] = E[eo + m * ms + f];
+            vo[u] = O[eo + m * ms + f];
+            vt[u] = g.fft.tw[(t < n) ? t : n - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = t0 + threadIdx.x + u * kGridThreads;
+            if (t < tot) {
+                sE[t] = ve[u];
+                sO[t] = vo[u];
+            }
+            if (t < n) twl[t] = vt[u];
+        }
+    }
+}
+
 // Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh); block 256.
+template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
                                                                const double2* __restrict__ E,
                                                                const double2* __restrict__ O,
                                                                int nf, double* __restrict__ f_out) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L;
+    const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t row = (size_t)jp * (L + 1) * nf;
-    const int cap = kRowElems / n > 0 ? kRowElems / n : 1;  // rows per LDS pass
+    const int cap = row_cap(n);
+    const double2* twl = lds + (size_t)cap * rs;
+    fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
     for (int f0 = 0; f0 < nf; f0 += cap) {
         const int nr = (nf - f0 < cap) ? nf - f0 : cap;
         for (int r = 0; r < nr; ++r) {
-            pack_spectrum(lds + r * n, n, L, E + row + f0 + r, O + row + f0 + r, nf, kSpecPlain,
+            pack_spectrum(lds + r * rs, n, L, E + row + f0 + r, O + row + f0 + r, nf, kSpecPlain,
                           1.0, threadIdx.x, kGridThreads);
         }
         __syncthreads();
-        fft_rows<+1>(lds, nr, g.fft);
+        fft_rows<+1, P2>(lds, nr, g.fft, twl);
         for (int t = threadIdx.x; t < nr * n; t += kGridThreads) {
             const int r = t / n, k = t - r * n;
-            const double2 z = lds[t];
+            const double2 z = lds[r * rs + lp(k)];
             double* fo = f_out + (size_t)(f0 + r) * g.nlat * n;
             fo[(size_t)jN * n + k] = z.x;
             if (jS != jN) fo[(size_t)jS * n + k] = z.y;
@@ -203,6 +249,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
 }
 
 // Analysis of nf grid fields: rows -> S/D (m <= mmax).  grid (nh); block 256.
+template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
                                                                const double* __restrict__ f_in,
                                                                int nf, int mmax,
@@ -210,22 +257,24 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
                                                                double2* __restrict__ D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
-    const int n = g.nlon, L = g.L;
+    const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t row = (size_t)jp * (L + 1) * nf;
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
-    const int cap = kRowElems / n > 0 ? kRowElems / n : 1;
+    const int cap = row_cap(n);
+    const double2* twl = lds + (size_t)cap * rs;
+    fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
     for (int f0 = 0; f0 < nf; f0 += cap) {
         const int nr = (nf - f0 < cap) ? nf - f0 : cap;
         for (int t = threadIdx.x; t < nr * n; t += kGridThreads) {
             const int r = t / n, k = t - r * n;
             const double* fi = f_in + (size_t)(f0 + r) * g.nlat * n;
-            lds[t] = c2(fi[(size_t)jN * n + k], fi[(size_t)jS * n + k]);
+            lds[r * rs + lp(k)] = c2(fi[(size_t)jN * n + k], fi[(size_t)jS * n + k]);
         }
         __syncthreads();
-        fft_rows<-1>(lds, nr, g.fft);
+        fft_rows<-1, P2>(lds, nr, g.fft, twl);
         for (int r = 0; r < nr; ++r)
-            unpack_sd(lds + r * n, n, mmax, w, S + row + f0 + r, D + row + f0 + r, nf,
+            unpack_sd(lds + r * rs, n, mmax, w, S + row + f0 + r, D + row + f0 + r, nf,
                       threadIdx.x, kGridThreads);
         __syncthreads();
     }
@@ -233,33 +282,41 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
 
 // Jacobian: fields (a, S a, b, S b) -> J = (a_s b_l - a_l b_s)/c^2 -> S/D.
 // grid (nh, members); E/O columns (member, 4 fields), S/D columns (member).
+template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, const double2* E,
                                                                const double2* O, double R,
                                                                int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x, b = blockIdx.y;
-    const int n = g.nlon, L = g.L;
+    const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
     const size_t ms = (size_t)nmem * 4;
+    // LDS: rows [4][rs] | twiddles [n] | E, O columns [L+1][4] each
+    double2* sE = lds + 4 * rs + n;
+    double2* sO = sE + (L + 1) * 4;
+    const double2* twl = lds + 4 * rs;
+    grid_prologue<4>(g, E, O, eo, ms, lds + 4 * rs, sE, sO);
+    __syncthreads();
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
-        pack_spectrum(lds + r * n, n, L, E + eo + r, O + eo + r, ms,
+        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 4,
                       (r & 1) ? kSpecScale : kSpecDlam, R, threadIdx.x, kGridThreads);
     }
     __syncthreads();
-    fft_rows<+1>(lds, 4, g.fft);
+    fft_rows<+1, P2>(lds, 4, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
-        const double2 al = lds[k], as = lds[n + k], bl = lds[2 * n + k], bs = lds[3 * n + k];
+        const int e = lp(k);
+        const double2 al = lds[e], as = lds[rs + e], bl = lds[2 * rs + e], bs = lds[3 * rs + e];
         const double jn = jac_point(al.x, as.x, bl.x, bs.x, cN);
         const double js = jac_point(al.y, as.y, bl.y, bs.y, cS);
-        lds[k] = c2(jn, js);
+        lds[e] = c2(jn, js);
     }
     __syncthreads();
-    fft_rows<-1>(lds, 1, g.fft);
+    fft_rows<-1, P2>(lds, 1, g.fft, twl);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = (size_t)jp * (L + 1) * nmem + b;
     unpack_sd(lds, n, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
@@ -267,13 +324,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
 
 // Shallow water: 10 Legendre fields -> 12 grid rows -> 4 products -> S/D.
 // grid (nh, members); E/O columns (member, 10 fields), S/D columns (member, 4).
+template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, const double2* E,
                                                                const double2* O, double R,
                                                                int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x, b = blockIdx.y;
-    const int n = g.nlon, L = g.L;
+    const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 10;
     const size_t ms = (size_t)nmem * 10;
@@ -282,33 +340,40 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
                               kSpecScale, kSpecPlain, kSpecDlam, kSpecScale, kSpecPlain,
                               kSpecPlain, kSpecPlain};
+    // LDS: rows [12][rs] | twiddles [n] | E, O columns [L+1][10] each
+    double2* sE = lds + 12 * rs + n;
+    double2* sO = sE + (L + 1) * 10;
+    const double2* twl = lds + 12 * rs;
+    grid_prologue<10>(g, E, O, eo, ms, lds + 12 * rs, sE, sO);
+    __syncthreads();
     for (int r = 0; r < 12; ++r) {
-        pack_spectrum(lds + r * n, n, L, E + eo + src[r], O + eo + src[r], ms, mode[r], R,
+        pack_spectrum(lds + r * rs, n, L, sE + src[r], sO + src[r], 10, mode[r], R,
                       threadIdx.x, kGridThreads);
     }
     __syncthreads();
-    fft_rows<+1>(lds, 12, g.fft);
+    fft_rows<+1, P2>(lds, 12, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
+        const int e = lp(k);
         double gn[12], gs[12], on[4], os[4];
 #pragma unroll
         for (int r = 0; r < 12; ++r) {
-            const double2 z = lds[r * n + k];
+            const double2 z = lds[r * rs + e];
             gn[r] = z.x;
             gs[r] = z.y;
         }
         swe_point(gn, cN, on);
         swe_point(gs, cS, os);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) lds[r * n + k] = c2(on[r], os[r]);
+        for (int r = 0; r < 4; ++r) lds[r * rs + e] = c2(on[r], os[r]);
     }
     __syncthreads();
-    fft_rows<-1>(lds, 4, g.fft);
+    fft_rows<-1, P2>(lds, 4, g.fft, twl);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
     for (int r = 0; r < 4; ++r)
-        unpack_sd(lds + r * n, n, mmax, w, S + so + r, D + so + r, (size_t)nmem * 4,
+        unpack_sd(lds + r * rs, n, mmax, w, S + so + r, D + so + r, (size_t)nmem * 4,
                   threadIdx.x, kGridThreads);
 }
 
@@ -321,43 +386,62 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
 // unweighted Syn^T, leaving a weighted analysis):
 //   0: -jt b_s/c^2,  1: jt b_l/c^2,  2: jt a_s/c^2,  3: -jt a_l/c^2,
 // analysed for m <= L into S/D columns (member, 4).  grid (nh, members).
+template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, const double2* E,
                                                                const double2* O, double R,
                                                                int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x, b = blockIdx.y;
-    const int n = g.nlon, L = g.L;
+    const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
     const size_t ms = (size_t)nmem * 5;
+    // LDS: rows [5][rs] | twiddles [n] | E, O columns [L+1][5] each
+    double2* sE = lds + 5 * rs + n;
+    double2* sO = sE + (L + 1) * 5;
+    const double2* twl = lds + 5 * rs;
+    grid_prologue<5>(g, E, O, eo, ms, lds + 5 * rs, sE, sO);
+    __syncthreads();
     for (int r = 0; r < 5; ++r) {
         const int mode = (r == 4) ? kSpecPlain : ((r & 1) ? kSpecScale : kSpecDlam);
-        pack_spectrum(lds + r * n, n, L, E + eo + r, O + eo + r, ms, mode, R, threadIdx.x,
-                      kGridThreads);
+        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 5, mode, R, threadIdx.x, kGridThreads);
     }
     __syncthreads();
-    fft_rows<+1>(lds, 5, g.fft);
+    fft_rows<+1, P2>(lds, 5, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
-        const double2 al = lds[k], as = lds[n + k], bl = lds[2 * n + k], bs = lds[3 * n + k];
-        const double2 jt = lds[4 * n + k];
+        const int e = lp(k);
+        const double2 al = lds[e], as = lds[rs + e], bl = lds[2 * rs + e], bs = lds[3 * rs + e];
+        const double2 jt = lds[4 * rs + e];
         const double tN = jt.x * qN, tS = jt.y * qS;
-        lds[k] = c2(-tN * bs.x, -tS * bs.y);
-        lds[n + k] = c2(tN * bl.x, tS * bl.y);
-        lds[2 * n + k] = c2(tN * as.x, tS * as.y);
-        lds[3 * n + k] = c2(-tN * al.x, -tS * al.y);
+        lds[e] = c2(-tN * bs.x, -tS * bs.y);
+        lds[rs + e] = c2(tN * bl.x, tS * bl.y);
+        lds[2 * rs + e] = c2(tN * as.x, tS * as.y);
+        lds[3 * rs + e] = c2(-tN * al.x, -tS * al.y);
     }
     __syncthreads();
-    fft_rows<-1>(lds, 4, g.fft);
+    fft_rows<-1, P2>(lds, 4, g.fft, twl);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
     for (int r = 0; r < 4; ++r)
-        unpack_sd(lds + r * n, n, L, w, S + so + r, D + so + r, (size_t)nmem * 4, threadIdx.x,
+        unpack_sd(lds + r * rs, n, L, w, S + so + r, D + so + r, (size_t)nmem * 4, threadIdx.x,
                   kGridThreads);
 }
+
+// Kernel instance for the plan's FFT length (power-of-two stages or general radices).
+#define AEO_GRID_PICK(name)                                                        \
+    static decltype(&name<true>) name##_for(const aeo_plan_s* p) {                 \
+        return fft_p2(p->fft) ? &name<true> : &name<false>;                        \
+    }
+AEO_GRID_PICK(grid_syn_kernel)
+AEO_GRID_PICK(grid_ana_kernel)
+AEO_GRID_PICK(grid_jac_kernel)
+AEO_GRID_PICK(grid_swe_kernel)
+AEO_GRID_PICK(grid_adj_kernel)
+#undef AEO_GRID_PICK
 
 // ---------------------------------------------------------------------------
 // Launchers
@@ -433,8 +517,27 @@ int gauss_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s) {
     return AEO_OK;
 }
 
+// Dynamic LDS beyond 64 KiB needs the kernel attribute.
+static int allow_lds(const void* kern, size_t bytes) {
+    AEO_REQUIRE(bytes <= 163840, AEO_ERR_UNSUPPORTED, "grid kernel: LDS demand exceeds 160 KiB");
+    if (bytes > 65536)
+        AEO_CHECK_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)bytes));
+    return AEO_OK;
+}
+
+// LDS of a fused grid kernel: `rows` padded rows, the twiddle table and the
+// staged E/O columns (nf fields each).
+static size_t grid_lds(const aeo_plan_s* p, int rows, int nf) {
+    return sizeof(double2) * ((size_t)rows * lds_row_stride(p->nlon) + p->nlon +
+                              2 * (size_t)nf * (p->L + 1));
+}
+
+static size_t jac_lds(const aeo_plan_s* p) { return grid_lds(p, 4, 4); }
+
 static size_t row_lds(const aeo_plan_s* p) {
-    return sizeof(double2) * (size_t)p->nlon * ((kRowElems / p->nlon > 0) ? kRowElems / p->nlon : 1);
+    // padded rows of one LDS pass + the twiddle table
+    return sizeof(double2) * ((size_t)row_cap(p->nlon) * lds_row_stride(p->nlon) + p->nlon);
 }
 
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
@@ -452,7 +555,9 @@ int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStrea
         sa.k = col_consts(p, 1.0, nullptr, 0.0);
         int rc = launch_stage<KindRaw, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(grid_syn_kernel, dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+        int rc2 = allow_lds((const void*)grid_syn_kernel_for(p), row_lds(p));
+        if (rc2) return rc2;
+        hipLaunchKernelGGL(grid_syn_kernel_for(p), dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
                            p->gdev(), p->d_E, p->d_O, nb, f + (size_t)f0 * p->nlat * p->nlon);
         AEO_LAUNCH_CHECK();
     }
@@ -464,7 +569,9 @@ int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream
     const size_t fs = (size_t)(L + 1) * (L + 1);
     for (int f0 = 0; f0 < nf; f0 += p->max_fields) {
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
-        hipLaunchKernelGGL(grid_ana_kernel, dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+        int rc2 = allow_lds((const void*)grid_ana_kernel_for(p), row_lds(p));
+        if (rc2) return rc2;
+        hipLaunchKernelGGL(grid_ana_kernel_for(p), dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
                            p->gdev(), f + (size_t)f0 * p->nlat * p->nlon, nb, L, p->d_S, p->d_D);
         AEO_LAUNCH_CHECK();
         StageArgs sa{};
@@ -499,8 +606,8 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     int rc = launch_stage<KindJac, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
-    hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, 1), dim3(kGridThreads),
-                       sizeof(double2) * 4 * p->nlon, s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
+    if ((rc = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p)))) return rc;
+    hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh, 1), dim3(kGridThreads), jac_lds(p), s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
                        p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
@@ -558,17 +665,15 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     ProfRec& pr = md->prof;
     pr.begin(kClsGrid, s);
     if (md->kind == kBVE) {
-        hipLaunchKernelGGL(grid_jac_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads),
-                           sizeof(double2) * 4 * p->nlon, s, p->gdev(), md->d_E, md->d_O,
+        int rc0 = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p));
+        if (rc0) return rc0;
+        hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), jac_lds(p), s, p->gdev(), md->d_E, md->d_O,
                            md->R, cut, md->nmem, md->d_S, md->d_D);
     } else {
-        const size_t swe_lds = sizeof(double2) * 12 * p->nlon;
-        if (swe_lds > 65536) {
-            AEO_CHECK_HIP(hipFuncSetAttribute((const void*)grid_swe_kernel,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)swe_lds));
-        }
-        hipLaunchKernelGGL(grid_swe_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds,
+        const size_t swe_lds = grid_lds(p, 12, 10);
+        int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);
+        if (rc0) return rc0;
+        hipLaunchKernelGGL(grid_swe_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds,
                            s, p->gdev(), md->d_E, md->d_O, md->R, cut, md->nmem, md->d_S,
                            md->d_D);
     }
@@ -638,13 +743,11 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     const size_t fs = (size_t)(L + 1) * (L + 1);
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
-    const size_t lds = sizeof(double2) * 5 * p->nlon;
-    if (lds > 65536) {
-        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)grid_adj_kernel,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
+    const size_t lds = grid_lds(p, 5, 5);
+    int rc0 = allow_lds((const void*)grid_adj_kernel_for(p), lds);
+    if (rc0) return rc0;
     pr.begin(kClsGrid, s);
-    hipLaunchKernelGGL(grid_adj_kernel, dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
+    hipLaunchKernelGGL(grid_adj_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
                        p->gdev(), md->d_aE, md->d_aO, md->R, md->nmem, md->d_aS, md->d_aD);
     pr.end(kClsGrid, s);
     AEO_LAUNCH_CHECK();

@@ -24,15 +24,27 @@
 
 namespace aeo {
 
-constexpr int kStageThreads = 256;
+constexpr int kStageThreads = 512;
+constexpr int kStageWaves = kStageThreads / 64;
+// FP64 table loads issued ahead per wave (analysis: per 64 degrees, for the
+// first PC chunks of 64; synthesis: rows), smaller for wide member/field tiles
+template <int MB, int NA, int NF>
+struct Prefetch {
+    static constexpr int PA = (MB * NA <= 2) ? 16 : 8;
+    static constexpr int PC = (MB * NA <= 2) ? 2 : 1;
+    static constexpr int PS = (MB * NF <= 5) ? 24 : 12;
+};
 
 struct ColK {
     int L;
     double R;
-    const double* Cp;   // (L+1)^2
+    const double* Cp;   // (L+1)^2, row-major [l][m]
     const double* Cm;   // (L+1)^2
-    const double* lap;  // -l(l+1)/R^2 (l = 0 entry 0)
+    const double* lap;  // -l(l+1)/R^2 (l = 0 entry 0), or nullptr
     double f10;         // f at (1, 0)
+    // set inside the stage kernel: LDS copies of column m of Cp / Cm ([l])
+    const double* cpc = nullptr;
+    const double* cmc = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -131,7 +143,6 @@ struct KindBveAdj {
 template <class KS>
 __device__ inline void derive_columns(const double2* st, int m, const ColK& k, double2* cols) {
     const int K = k.L - m + 1;
-    const int n = k.L + 1;
     for (int t = threadIdx.x; t < KS::NF * K; t += kStageThreads) {
         const int f = t / K, i = t - f * K;
         if (KS::SF[f] < 0) cols[t] = KS::base(f, st, m + i, m, k);
@@ -144,8 +155,8 @@ __device__ inline void derive_columns(const double2* st, int m, const ColK& k, d
             const int l = m + i;
             const double2* s = cols + src * K;
             double2 v = c2(0.0, 0.0);
-            if (i >= 1) v = cadd(v, cscale(s[i - 1], k.Cp[(l - 1) * n + m]));
-            if (l <= k.L - 1) v = cadd(v, cscale(s[i + 1], k.Cm[(l + 1) * n + m]));
+            if (i >= 1) v = cadd(v, cscale(s[i - 1], k.cpc[l - 1]));
+            if (l <= k.L - 1) v = cadd(v, cscale(s[i + 1], k.cmc[l + 1]));
             cols[t] = v;
         }
     }
@@ -389,6 +400,9 @@ struct StageArgs {
     ColK k;
 };
 
+// double2 slots holding the three FP64 coefficient columns (C+, C-, lap) of m
+__host__ __device__ inline int coef_slots(int n) { return (3 * n + 1) / 2; }
+
 template <class KS, class Epi, int MB, int NS>
 struct StageLds {
     static constexpr int NST = (KS::NSTATE > Epi::NSTATE) ? KS::NSTATE : Epi::NSTATE;
@@ -396,15 +410,17 @@ struct StageLds {
     static constexpr int NCS = MB * KS::NF * 2, NTS = (NCS + 15) / 16;
     static size_t bytes(int L, int nh, bool ana, bool syn) {
         const int n = L + 1;
-        size_t s = sizeof(double2) * (size_t)MB * NST * n;     // state columns
+        size_t s = sizeof(double2) * ((size_t)MB * NST * n + coef_slots(n));  // state + C+/C-/lap
         size_t a = 0, c = 0;
         if (ana) {
-            a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) + 4 * MB * Epi::NA * 64);
+            a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) +
+                                   (size_t)kStageWaves * MB * Epi::NA * 64);
             if (NS > 0) a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n)) +
                             sizeof(int) * 2 * NTA * 16 + 2 * sizeof(__bf16) * NS * NTA * 16 * rup(nh, 32) + 64;
         }
         if (syn) {
-            c = sizeof(double2) * ((size_t)MB * KS::NF * n + 2 * (size_t)MB * KS::NF * 64);
+            c = sizeof(double2) * ((size_t)MB * KS::NF * n +
+                                   (size_t)(kStageWaves - 2) * MB * KS::NF * 64);
             if (NS > 0) c = sizeof(double2) * ((size_t)MB * KS::NF * n) + sizeof(int) * NTS * 16 +
                             2 * sizeof(__bf16) * NS * NTS * 16 * rup(split_kp(L, 0), 32) + 64;
         }
@@ -413,6 +429,25 @@ struct StageLds {
 };
 
 __device__ inline double cpart(double2 v, int ri) { return ri ? v.y : v.x; }
+
+// ex[c] = exp_above(max_i |val(c, i)|) for c < ncol, i < len: every thread of
+// the workgroup cooperates, tpc (a power of two <= 64) threads per column,
+// reduced with lane shuffles.
+template <class F>
+__device__ inline void col_max_exp(int ncol, int len, int* ex, const F& val) {
+    int tpc = 64;
+    while (tpc > 1 && tpc * ncol > kStageThreads) tpc >>= 1;
+    const int cpb = kStageThreads / tpc;
+    const int sub = threadIdx.x & (tpc - 1);
+    for (int c0 = 0; c0 < ncol; c0 += cpb) {
+        const int c = c0 + threadIdx.x / tpc;
+        double mx = 0.0;
+        if (c < ncol)
+            for (int i = sub; i < len; i += tpc) mx = fmax(mx, fabs(val(c, i)));
+        for (int off = tpc >> 1; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if (sub == 0 && c < ncol) ex[c] = exp_above(mx);
+    }
+}
 
 // 16-byte aligned pointer at or after p
 template <class T>
@@ -426,6 +461,9 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     constexpr int NST = StageLds<KS, Epi, MB, NS>::NST;
     constexpr int NA = Epi::NA;
     constexpr int NF = KS::NF;
+    constexpr int kPre = Prefetch<MB, NA, NF>::PA;
+    constexpr int kPC = Prefetch<MB, NA, NF>::PC;
+    constexpr int kPreS = Prefetch<MB, NA, NF>::PS;
     extern __shared__ double2 lds[];
     const int m = blockIdx.x;
     const int jt = blockIdx.y;
@@ -433,70 +471,134 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     const int L = g.L, nh = g.nh, n = L + 1;
     const int K = L - m + 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int top = (a.lmax_a < L) ? a.lmax_a : L;      // analysed degrees l <= top
+    const bool active = ANA && (m <= a.mmax_a);
     double2* st = lds;                                  // [MB][NST][n]
-    double2* work = lds + (size_t)MB * NST * n;
+    double* cpc = reinterpret_cast<double*>(lds + (size_t)MB * NST * n);   // C+ column m
+    double* cmc = cpc + n;                                                 // C- column m
+    double* lapc = cmc + n;                                                // lap
+    double2* work = lds + (size_t)MB * NST * n + coef_slots(n);
+    ColK kc = a.k;
+    kc.cpc = cpc;
+    kc.cmc = cmc;
+    if (a.k.lap) kc.lap = lapc;
 
-    // -- state columns of this m --------------------------------------------
-    for (int t = threadIdx.x; t < MB * NST * n; t += kStageThreads) {
-        const int bb = t / (NST * n);
-        const int r = t - bb * NST * n;
-        const int s = r / n, l = r - s * n;
-        const int b = b0 + bb;
-        double2 v = c2(0.0, 0.0);
-        if (b < a.nmem && s < a.nstate_valid) {
-            const size_t e = a.ys_mmajor ? (size_t)m * n + l : (size_t)l * n + m;
-            if (s >= 1 && a.ys_aux)
-                v = a.ys_aux[b * a.aux_stride + (size_t)(s - 1) * n * n + e];
-            else if (a.ys)
-                v = a.ys[b * a.ys_stride + (size_t)s * n * n + e];
-        }
-        st[t] = v;
+    // -- prologue: every load that does not depend on another workgroup's
+    //    results of this launch is issued before the first wait.  FP64 table
+    //    rows go to registers (pa: first 64 kPC analysed degrees, ps: first
+    //    kPreS synthesis rows of this wave); state, S/D and coefficient
+    //    columns go through registers into LDS.
+    double pa[kPC][kPre];
+    double ps[kPreS];
+    if constexpr (ANA && NS == 0) {
+        const double* Pa = g.Pa + g.offs[m];
+#pragma unroll
+        for (int c = 0; c < kPC; ++c)
+#pragma unroll
+            for (int u = 0; u < kPre; ++u) {
+                const int l = m + 64 * c + lane, jp = wave + kStageWaves * u;
+                pa[c][u] = (active && l <= top && jp < nh) ? Pa[(size_t)jp * K + (l - m)] : 0.0;
+            }
     }
+    // synthesis rows: issued here for synthesis-only launches, otherwise just
+    // before the epilogue (overlapping its loads, not held through the analysis)
+    auto load_ps = [&]() {
+        const int jps = jt * 64 + lane;
+        const double* Ps = g.Ps + g.offs[m] + jps;
+#pragma unroll
+        for (int u = 0; u < kPreS; ++u) {
+            const int i = wave + kStageWaves * u;
+            ps[u] = (jps < nh && i < K) ? Ps[(size_t)i * nh] : 0.0;
+        }
+    };
+    if constexpr (SYN && NS == 0 && !ANA) load_ps();
+    double2* X = work;                                  // [MB][NA][2][nh]  (S then D)
+    double2* accl = X + (size_t)MB * NA * 2 * nh;       // [MB][NA][n]
+    {
+        const int tot_s = MB * NST * n;
+        const int tot_x = active ? MB * NA * nh : 0;
+        const int tot = (tot_s > tot_x ? tot_s : tot_x) > n ? (tot_s > tot_x ? tot_s : tot_x) : n;
+        // loads are unconditional where possible (indices clamped in range) so
+        // the values stay in registers; only the LDS stores are predicated
+        for (int t0 = 0; t0 < tot; t0 += 2 * kStageThreads) {
+            double2 vs[2], vx[2][2];
+            double vc[2][3];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = t0 + threadIdx.x + u * kStageThreads;
+                vs[u] = c2(0.0, 0.0);
+                {
+                    const int ts = (t < tot_s) ? t : tot_s - 1;
+                    const int bb = ts / (NST * n);
+                    const int r = ts - bb * NST * n;
+                    const int sf = r / n, l = r - sf * n;
+                    const int b = b0 + bb;
+                    if (b < a.nmem && sf < a.nstate_valid) {
+                        const size_t e = a.ys_mmajor ? (size_t)m * n + l : (size_t)l * n + m;
+                        if (sf >= 1 && a.ys_aux)
+                            vs[u] = a.ys_aux[b * a.aux_stride + (size_t)(sf - 1) * n * n + e];
+                        else if (a.ys)
+                            vs[u] = a.ys[b * a.ys_stride + (size_t)sf * n * n + e];
+                    }
+                }
+                vx[u][0] = vx[u][1] = c2(0.0, 0.0);
+                if (tot_x > 0) {
+                    const int tx = (t < tot_x) ? t : tot_x - 1;
+                    const int f = tx % NA;
+                    const int bb = (tx / NA) % MB;
+                    const int jp = tx / (NA * MB);
+                    const int b = b0 + bb;
+                    if (b < a.nmem) {
+                        const size_t src = (((size_t)jp * n + m) * a.nmem + b) * NA + f;
+                        vx[u][0] = a.S[src];
+                        vx[u][1] = a.D[src];
+                    }
+                }
+                const int tc = (t < n) ? t : n - 1;
+                vc[u][0] = a.k.Cp[(size_t)tc * n + m];
+                vc[u][1] = a.k.Cm[(size_t)tc * n + m];
+                vc[u][2] = a.k.lap ? a.k.lap[tc] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = t0 + threadIdx.x + u * kStageThreads;
+                if (t < tot_s) st[t] = vs[u];
+                if (t < tot_x) {
+                    const int f = t % NA;
+                    const int bb = (t / NA) % MB;
+                    const int jp = t / (NA * MB);
+                    X[((bb * NA + f) * 2 + 0) * nh + jp] = vx[u][0];
+                    X[((bb * NA + f) * 2 + 1) * nh + jp] = vx[u][1];
+                }
+                if (t < n) {
+                    cpc[t] = vc[u][0];
+                    cmc[t] = vc[u][1];
+                    lapc[t] = vc[u][2];
+                }
+            }
+        }
+    }
+    if constexpr (ANA)
+        for (int t = threadIdx.x; t < MB * NA * n; t += kStageThreads) accl[t] = c2(0.0, 0.0);
     __syncthreads();
 
     if constexpr (ANA) {
-        double2* X = work;                                  // [MB][NA][2][nh]  (S then D)
-        double2* accl = X + (size_t)MB * NA * 2 * nh;       // [MB][NA][n]
-        double2* red = accl + (size_t)MB * NA * n;          // [4][MB][NA][64]
-        const bool active = (m <= a.mmax_a);
-        if (active) {
-            for (int t = threadIdx.x; t < MB * NA * nh; t += kStageThreads) {
-                const int f = t % NA;
-                const int bb = (t / NA) % MB;
-                const int jp = t / (NA * MB);
-                const int b = b0 + bb;
-                double2 sv = c2(0.0, 0.0), dv = c2(0.0, 0.0);
-                if (b < a.nmem) {
-                    const size_t src = (((size_t)jp * n + m) * a.nmem + b) * NA + f;
-                    sv = a.S[src];
-                    dv = a.D[src];
-                }
-                X[((bb * NA + f) * 2 + 0) * nh + jp] = sv;
-                X[((bb * NA + f) * 2 + 1) * nh + jp] = dv;
-            }
-        }
-        for (int t = threadIdx.x; t < MB * NA * n; t += kStageThreads) accl[t] = c2(0.0, 0.0);
-        __syncthreads();
+        double2* red = accl + (size_t)MB * NA * n;          // [waves][MB][NA][64]
         if constexpr (NS > 0) if (active) {
             // ---- split-precision analysis: BF16 digit MFMA, exact FP32, FP64 recombination
             constexpr int NCA = StageLds<KS, Epi, MB, NS>::NCA;
             constexpr int NTA = StageLds<KS, Epi, MB, NS>::NTA;
             constexpr int NCP = NTA * 16;
-            const int top = (a.lmax_a < L) ? a.lmax_a : L;
             const int nhk = rup(nh, 32);
             const int kp = split_kp(L, m), mpad = rup(kp, 16);
             int* gexp = reinterpret_cast<int*>(accl + (size_t)MB * NA * n);   // [2][NCP]
             __bf16* xs = align16<__bf16>(gexp + 2 * NCP);                      // [2][NS][NCP][nhk]
-            for (int t = threadIdx.x; t < 2 * NCP; t += kStageThreads) {
+            col_max_exp(2 * NCP, nh, gexp, [&](int t, int j) {
                 const int p = t / NCP, c = t - p * NCP;
-                double mx = 0.0;
-                if (c < NCA) {
-                    const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                    const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
-                    for (int j = 0; j < nh; ++j) mx = fmax(mx, fabs(cpart(xr[j], ri)));
-                }
-                gexp[t] = exp_above(mx);
-            }
+                if (c >= NCA) return 0.0;
+                const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                return cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
+            });
             __syncthreads();
             for (int t = threadIdx.x; t < 2 * NCP * nhk; t += kStageThreads) {
                 const int p = t / (NCP * nhk);
@@ -517,10 +619,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             const __bf16* Pt = g.Pasl + g.offs_sa[m];
             const int* er = g.eana + g.offs_ea[m];
             double* accd = reinterpret_cast<double*>(accl);
-            for (int p = 0; p < 2; ++p) {
-                const int Mv = (top >= m + p) ? (top - m - p) / 2 + 1 : 0;
-                const int MT = (Mv + 15) / 16;
-                for (int mt = wave; mt < MT; mt += 4) {
+            // 16-degree tiles of both parities spread over all waves
+            const int Mv0 = (top >= m) ? (top - m) / 2 + 1 : 0;
+            const int Mv1 = (top >= m + 1) ? (top - m - 1) / 2 + 1 : 0;
+            const int MT0 = (Mv0 + 15) / 16, MT1 = (Mv1 + 15) / 16;
+            for (int tt = wave; tt < MT0 + MT1; tt += kStageWaves) {
+                const int p = (tt < MT0) ? 0 : 1;
+                const int mt = (tt < MT0) ? tt : tt - MT0;
+                const int Mv = p ? Mv1 : Mv0;
+                {
                     floatx4 acc[NS][NTA];
 #pragma unroll
                     for (int q = 0; q < NS; ++q)
@@ -568,9 +675,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             __syncthreads();
         }
         if constexpr (NS == 0) if (active) {
-            const int top = (a.lmax_a < L) ? a.lmax_a : L;
             const double* Pa = g.Pa + g.offs[m];
-            for (int l0 = m; l0 <= top; l0 += 64) {
+            for (int l0 = m, c = 0; l0 <= top; l0 += 64, ++c) {
                 const int l = l0 + lane;
                 double2 acc[MB][NA];
 #pragma unroll
@@ -580,17 +686,32 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 if (l <= top) {
                     const int par = (l - m) & 1;
                     const double* P = Pa + (l - m);
-#pragma unroll 4
-                    for (int jp = wave; jp < nh; jp += 4) {
-                        const double p = P[(size_t)jp * K];
+                    for (int j0 = wave; j0 < nh; j0 += kStageWaves * kPre) {
+                        double pv[kPre];
+                        if (j0 == wave && c < kPC) {
 #pragma unroll
-                        for (int bb = 0; bb < MB; ++bb)
+                            for (int u = 0; u < kPre; ++u) pv[u] = (c == 0) ? pa[0][u] : pa[kPC - 1][u];
+                        } else {
 #pragma unroll
-                            for (int f = 0; f < NA; ++f) {
-                                const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
-                                acc[bb][f].x = fma(p, x.x, acc[bb][f].x);
-                                acc[bb][f].y = fma(p, x.y, acc[bb][f].y);
+                            for (int u = 0; u < kPre; ++u) {
+                                const int jp = j0 + kStageWaves * u;
+                                pv[u] = (jp < nh) ? P[(size_t)jp * K] : 0.0;
                             }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kPre; ++u) {
+                            const int jp = j0 + kStageWaves * u;
+                            if (jp < nh) {
+#pragma unroll
+                                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                                    for (int f = 0; f < NA; ++f) {
+                                        const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
+                                        acc[bb][f].x = fma(pv[u], x.x, acc[bb][f].x);
+                                        acc[bb][f].y = fma(pv[u], x.y, acc[bb][f].y);
+                                    }
+                            }
+                        }
                     }
                 }
 #pragma unroll
@@ -603,16 +724,17 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     const int ll = l0 + (t & 63);
                     if (ll <= top) {
                         const int q = t >> 6;  // (bb * NA + f)
-                        double2 s = red[(0 * MB * NA + q) * 64 + (t & 63)];
-                        s = cadd(s, red[(1 * MB * NA + q) * 64 + (t & 63)]);
-                        s = cadd(s, red[(2 * MB * NA + q) * 64 + (t & 63)]);
-                        s = cadd(s, red[(3 * MB * NA + q) * 64 + (t & 63)]);
+                        double2 s = red[q * 64 + (t & 63)];
+#pragma unroll
+                        for (int w = 1; w < kStageWaves; ++w)
+                            s = cadd(s, red[(w * MB * NA + q) * 64 + (t & 63)]);
                         accl[q * n + ll] = s;
                     }
                 }
                 __syncthreads();
             }
         }
+        if constexpr (SYN && NS == 0) load_ps();
         // epilogue over every (l, m) of the column and every member
         const bool writer = (jt == 0);
         for (int t = threadIdx.x; t < MB * n; t += kStageThreads) {
@@ -639,18 +761,13 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         constexpr int NCP = NTS * 16;
         double2* cols = work;                               // [MB][NF][K]
         for (int bb = 0; bb < MB; ++bb)
-            derive_columns<KS>(st + (size_t)bb * NST * n, m, a.k, cols + (size_t)bb * NF * K);
+            derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
         const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
         int* fexp = reinterpret_cast<int*>(cols + (size_t)MB * NF * n);       // [NCP]
         __bf16* qs = align16<__bf16>(fexp + NCP);                             // [2][NS][NCP][kpad]
-        for (int c = threadIdx.x; c < NCP; c += kStageThreads) {
-            double mx = 0.0;
-            if (c < NCS) {
-                const double2* cr = cols + (size_t)(c >> 1) * K;
-                for (int i = 0; i < K; ++i) mx = fmax(mx, fabs(cpart(cr[i], c & 1)));
-            }
-            fexp[c] = exp_above(mx);
-        }
+        col_max_exp(NCP, K, fexp, [&](int c, int i) {
+            return (c < NCS) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
+        });
         __syncthreads();
         for (int t = threadIdx.x; t < 2 * NCP * kpad; t += kStageThreads) {
             const int p = t / (NCP * kpad);
@@ -664,11 +781,14 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             for (int q = 0; q < NS; ++q) qs[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = (__bf16)d[q];
         }
         __syncthreads();
-        const int j0 = jt * 64 + wave * 16;
+        // waves 0-3 / 4-7: even / odd degrees of the same four 16-row tiles
+        static_assert(kStageWaves == 8, "split synthesis assigns one parity per 4 waves");
+        const int j0 = jt * 64 + (wave & 3) * 16;
         if (j0 < nh) {
             const __bf16* Pt = g.Psl + g.offs_ss[m];
             const int* er = g.esyn + (size_t)m * 2 * nhp;
-            for (int p = 0; p < 2; ++p) {
+            {
+                const int p = wave >> 2;
                 floatx4 acc[NS][NTS];
 #pragma unroll
                 for (int q = 0; q < NS; ++q)
@@ -721,7 +841,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         double2* cols = work;                               // [MB][NF][K]
         double2* red = cols + (size_t)MB * NF * n;          // [2][MB][NF][64]
         for (int bb = 0; bb < MB; ++bb)
-            derive_columns<KS>(st + (size_t)bb * NST * n, m, a.k, cols + (size_t)bb * NF * K);
+            derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
         const int jp = jt * 64 + lane;
         double2 acc[MB][NF];
 #pragma unroll
@@ -729,18 +849,34 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 #pragma unroll
             for (int f = 0; f < NF; ++f) acc[bb][f] = c2(0.0, 0.0);
         if (jp < nh) {
+            // kPre table rows in flight per wave before the multiply-adds
             const double* P = g.Ps + g.offs[m] + jp;
-#pragma unroll 4
-            for (int i = wave; i < K; i += 4) {
-                const double p = P[(size_t)i * nh];
+            for (int i0 = wave; i0 < K; i0 += kStageWaves * kPreS) {
+                double pv[kPreS];
+                if (i0 == wave) {
 #pragma unroll
-                for (int bb = 0; bb < MB; ++bb)
+                    for (int u = 0; u < kPreS; ++u) pv[u] = ps[u];
+                } else {
 #pragma unroll
-                    for (int f = 0; f < NF; ++f) {
-                        const double2 c = cols[(bb * NF + f) * K + i];
-                        acc[bb][f].x = fma(p, c.x, acc[bb][f].x);
-                        acc[bb][f].y = fma(p, c.y, acc[bb][f].y);
+                    for (int u = 0; u < kPreS; ++u) {
+                        const int i = i0 + kStageWaves * u;
+                        pv[u] = (i < K) ? P[(size_t)i * nh] : 0.0;
                     }
+                }
+#pragma unroll
+                for (int u = 0; u < kPreS; ++u) {
+                    const int i = i0 + kStageWaves * u;
+                    if (i < K) {
+#pragma unroll
+                        for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                            for (int f = 0; f < NF; ++f) {
+                                const double2 c = cols[(bb * NF + f) * K + i];
+                                acc[bb][f].x = fma(pv[u], c.x, acc[bb][f].x);
+                                acc[bb][f].y = fma(pv[u], c.y, acc[bb][f].y);
+                            }
+                    }
+                }
             }
         }
         if (wave >= 2) {
@@ -752,6 +888,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         }
         __syncthreads();
         if (wave < 2 && jp < nh) {
+            // waves of equal parity (wave mod 2) hold partial sums of E (even) / O (odd)
             double2* dst = (wave == 0) ? a.E : a.O;
 #pragma unroll
             for (int bb = 0; bb < MB; ++bb) {
@@ -759,8 +896,11 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 if (b >= a.nmem) continue;
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
-                    const double2 o = red[((wave * MB + bb) * NF + f) * 64 + lane];
-                    dst[(((size_t)jp * n + m) * a.nmem + b) * NF + f] = cadd(acc[bb][f], o);
+                    double2 v = acc[bb][f];
+#pragma unroll
+                    for (int w = wave + 2; w < kStageWaves; w += 2)
+                        v = cadd(v, red[(((w - 2) * MB + bb) * NF + f) * 64 + lane]);
+                    dst[(((size_t)jp * n + m) * a.nmem + b) * NF + f] = v;
                 }
             }
         }

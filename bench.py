@@ -1,23 +1,38 @@
 """Benchmark: spectral RK4 steps/s on one MI355X.
 
-Headline workload: barotropic-vorticity RK4 at L = 170 on the Gauss
-256 x 512 grid (the L = 170 member class, single member, FP64), fixed
-dt = 300 s, seeded l^-1.5 vorticity with rms 5e-5 s^-1.  Also measured in
-the same run: the shallow-water step at L = 85 on Gauss 128 x 256 with a
-2 km Gaussian mountain (dt = 300 s).  Inputs are resident in HBM before the
-timed region; a "step" is one full RK4 step (4 fused stages).
+Headline (``value``): barotropic-vorticity RK4 steps/s at L = 170 on the
+Gauss 256 x 512 grid (single member, dt = 300 s, seeded l^-1.5 vorticity
+with rms 5e-5 s^-1, R = 6.371e6 m, 24 h rotation) for the fastest precision
+mode whose vorticity after one simulated day (288 steps) stays within 1e-11
+relative L2 of the plain FP64 step: plain FP64 Legendre contractions, or
+split BF16-digit MFMA contractions with k in {4, 5, 6, 7} slices (FP64 state,
+recombination, tables and RK4 update).  The plain FP64 step itself matches
+the NumPy oracle to <= 1e-12 (tests).
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: the measurement
-covers one GPU.
+Measured in the same run (``secondary``):
+  * the same at L = 85 on Gauss 128 x 256;
+  * shallow water at L = 85 on Gauss 128 x 256 with a 2 km Gaussian mountain,
+    FP64 and split, per-field error after one day;
+  * the uneven-L ensemble: 64 members each at L = 21/42/85/170 on
+    32x64/64x128/128x256/256x512, one hipGraph replay per two steps,
+    autotuned tiles, FP64 and split, with the one-day error;
+  * the adjoint: L = 170, 48 steps, split transforms; cost of the
+    checkpointed forward + reverse sweep relative to the plain forward.
+
+Inputs are resident in HBM before every timed region; a "step" is one full
+RK4 step (4 fused stages).  Prints one JSON line.  ``--gpus N`` with N > 1
+exits: every figure is one GPU's.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
 import sys
 import time
+import warnings
 
 import numpy as np
 
@@ -27,6 +42,13 @@ sys.path.insert(0, ROOT)
 R_EARTH = 6.371e6
 OMEGA = 2.0 * math.pi / 86400.0
 HBM_PEAK_GBS = 8000.0
+DAY_STEPS = 288          # one simulated day at dt = 300 s
+DT = 300.0
+TOL = 1e-11
+SPLITS = (4, 5, 6, 7)
+KCLASS = {0: "synthesis prime (leg_stage_kernel, SYN only)",
+          1: "grid_jac_kernel (row-pair FFTs + Jacobian)",
+          2: "leg_stage_kernel<KindBve, EpiBveS> (analysis + RK4 epilogue + next synthesis)"}
 
 
 def _args():
@@ -36,101 +58,223 @@ def _args():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--headline-only", action="store_true",
+                   help="only the headline workload (for profiler runs)")
+    p.add_argument("--precision", type=int, default=None,
+                   help="force the headline precision mode (0 = FP64, k = split slices)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run 4 device copies of 256 MiB (scales profiler byte counters)")
     return p.parse_args()
 
 
-def _bve_planet(L, nlat, nlon):
-    from aeolus_amd.planet import Planet
-    return Planet.with_radius(R_EARTH, OMEGA, grid_type="latlon", nlat=nlat, nlon=nlon,
-                              l_max=L, product_quadrature="fine")
-
-
-def _timed_steps(model, y, dt, steps, warmup, prof_cls=None):
-    """Run warmup + timed fused steps; return (ms_per_step, y_out, kernel_ms, launches)."""
+def copy_calibration():
+    """Plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE per dispatch of
+    this kernel against 256 MiB gives the counters' scale."""
     import torch
-    from aeolus_amd import _native as nat
-    out = torch.empty_like(y)
-    if warmup:
-        model.rk4(y, dt, warmup, out=out)
+    n = 256 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    _sync()
+    print(json.dumps({"copy_bytes_per_dispatch": n * 8}))
+
+
+def _planet(L, nlat, nlon):
+    from aeolus_amd.planet import Planet
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return Planet.with_radius(R_EARTH, OMEGA, grid_type="latlon", nlat=nlat, nlon=nlon,
+                                  l_max=L, product_quadrature="fine")
+
+
+def _rel(a, b):
+    import torch
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def _sync():
+    import torch
     torch.cuda.synchronize()
-    if prof_cls is not None:
-        nat.check(nat.lib().aeo_model_profile(model._model, prof_cls, 4 * steps), "profile")
+
+
+def _timed(run, steps, warmup):
+    """ms per step of run(k) (k RK4 steps), events on the library's stream."""
+    import torch
+    if warmup:
+        run(warmup)
+    _sync()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t0.record()
-    model.rk4(y, dt, steps, out=out)
+    run(steps)
     t1.record()
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1)
-    kms, nl = None, 0
-    if prof_cls is not None:
-        import ctypes
-        tot = ctypes.c_double()
-        cnt = ctypes.c_int()
-        nat.check(nat.lib().aeo_model_profile_read(model._model, ctypes.byref(tot),
-                                                   ctypes.byref(cnt)), "profile_read")
-        nat.check(nat.lib().aeo_model_profile(model._model, -1, 0), "profile off")
-        kms, nl = tot.value, cnt.value
-    return ms / steps, out, kms, nl
+    _sync()
+    return t0.elapsed_time(t1) / steps
 
 
-def _class_bytes(cls, L, nlat, nlon, nsyn, nana, nstate):
-    """Algorithmic HBM bytes of one launch of a stage kernel class."""
+def _profiled(model, cls, run, steps):
+    """(ms per step, mean launch ms, launches) of kernel class `cls` over `steps`."""
+    from aeolus_amd import _native as nat
+    lib = nat.lib()
+    nat.check(lib.aeo_model_profile(model._model, cls, 4 * steps + 8), "profile")
+    try:
+        ms = _timed(run, steps, 0)
+        tot, cnt = ctypes.c_double(), ctypes.c_int()
+        nat.check(lib.aeo_model_profile_read(model._model, ctypes.byref(tot), ctypes.byref(cnt)),
+                  "profile_read")
+    finally:
+        nat.check(lib.aeo_model_profile(model._model, -1, 0), "profile off")
+    return ms, tot.value / max(cnt.value, 1), cnt.value
+
+
+def _stage_bytes(cls, L, nlat, nslices):
+    """Algorithmic HBM bytes of one launch of a BVE stage kernel class
+    (tables read once, intermediates written/read once, state passes)."""
     nh = (nlat + 1) // 2
     B = (L + 1) * (L + 2) // 2
     cut = (2 * L) // 3
     Bc = (cut + 1) * (cut + 2) // 2
     spec = (L + 1) * (L + 1) * 16
-    if cls == 0:   # Pbar table + state read + E/O write
-        return nh * B * 8 + nstate * spec + 2 * nsyn * nh * (L + 1) * 16
-    if cls == 1:   # E/O read + S/D write
-        return 2 * nsyn * nh * (L + 1) * 16 + 2 * nana * nh * (cut + 1) * 16
-    # analysis: truncated Pbar table + S/D read + stage state r/w (ys, y0, ksum r/w, ynext)
-    return nh * Bc * 8 + 2 * nana * nh * (cut + 1) * 16 + 5 * nstate * spec
+    tab = 8 if nslices == 0 else 2 * nslices      # bytes per table entry
+    eo = 2 * 4 * nh * (L + 1) * 16                # E/O, 4 fields
+    sd = 2 * 1 * nh * (cut + 1) * 16              # S/D, 1 field, m <= cut
+    if cls == 0:
+        return nh * B * tab + spec + eo
+    if cls == 1:
+        return eo + sd
+    # analysis table (truncated) + S/D + ys, y0, ksum r/w, ynext + synthesis table + E/O
+    return nh * Bc * tab + sd + 5 * spec + nh * B * tab + eo
 
 
-def run_gpu(args):
+def _stage_flops(cls, L, nlat):
+    """Algorithmic flops of one launch (2 nlat B per Legendre field transform)."""
+    B = (L + 1) * (L + 2) // 2
+    cut = (2 * L) // 3
+    Bc = (cut + 1) * (cut + 2) // 2
+    if cls == 0:
+        return 4 * 2 * nlat * B
+    if cls == 1:
+        n = 2 * (L + 1)
+        return 5 * 2.5 * n * math.log2(max(n, 2)) * nlat
+    return 2 * nlat * Bc + 4 * 2 * nlat * B
+
+
+def bve_modes(L, nlat, nlon, steps, warmup, seed, force=None):
+    """Per precision mode: ms/step over `steps` and the one-day error vs FP64."""
     import torch
-    from aeolus_amd.physics import BarotropicVorticity, ShallowWaterModel, Topography
-    from aeolus_amd.run.initial_conditions import spectrum_l15, swe_spectra
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    pl = _planet(L, nlat, nlon)
+    model = BarotropicVorticity(pl)
+    z0 = torch.as_tensor(spectrum_l15(L, np.random.default_rng(seed), rms=5e-5), device="cuda")
+    out = torch.empty_like(z0)
+    run = lambda k: model.rk4(z0, DT, k, out=out)
+    modes = {}
+    ref_day = model.rk4(z0, DT, DAY_STEPS)
+    for ns in (0,) + SPLITS:
+        if force is not None and ns not in (0, force):
+            continue
+        model.set_precision(ns)
+        day = model.rk4(z0, DT, DAY_STEPS)
+        err = 0.0 if ns == 0 else _rel(day, ref_day)
+        ms = _timed(run, min(steps, 192), warmup)
+        modes[ns] = {"ms_per_step": ms, "rel_l2_vs_fp64_1day": err,
+                     "finite": bool(torch.isfinite(day).all())}
+    ok = [ns for ns, v in modes.items() if v["rel_l2_vs_fp64_1day"] <= TOL and v["finite"]]
+    best = force if force is not None else min(ok, key=lambda ns: modes[ns]["ms_per_step"])
+    return model, run, modes, best
 
-    dev = torch.device("cuda", 0)
-    res = {}
-    # -- headline: BVE L = 170, Gauss 256 x 512 -------------------------
-    L, nlat, nlon = 170, 256, 512
-    pl = _bve_planet(L, nlat, nlon)
-    model = BarotropicVorticity(pl, viscosity=0.0)
-    z0 = torch.as_tensor(spectrum_l15(L, np.random.default_rng(2026), rms=5e-5), device=dev)
-    dt = 300.0
-    # per-class durations (calibration, untimed)
+
+def headline(args):
+    model, run, modes, best = bve_modes(170, 256, 512, args.steps, args.warmup, 2026,
+                                        args.precision)
+    model.set_precision(best)
     cls_ms = {}
-    for c in (0, 1, 2):
-        _, _, kms, nl = _timed_steps(model, z0, dt, 40, 2, prof_cls=c)
-        cls_ms[c] = kms / max(nl, 1)
+    for c in (1, 2):
+        _, kms, _ = _profiled(model, c, run, 24)
+        cls_ms[c] = kms
     dom = max(cls_ms, key=cls_ms.get)
-    ms_step, y_out, kms, nl = _timed_steps(model, z0, dt, args.steps, args.warmup, prof_cls=dom)
-    ms_plain, _, _, _ = _timed_steps(model, z0, dt, args.steps, 0, prof_cls=None)
-    res["bve170"] = dict(ms_per_step=ms_step, ms_per_step_uninstrumented=ms_plain,
-                         steps_per_s=1000.0 / ms_step, dom_cls=dom,
-                         dom_ms=kms / max(nl, 1), dom_launches=nl,
-                         cls_ms={str(k): v for k, v in cls_ms.items()},
-                         dom_bytes=_class_bytes(dom, L, nlat, nlon, 4, 1, 1),
-                         finite=bool(torch.isfinite(y_out).all()))
-    # -- secondary: SWE L = 85, Gauss 128 x 256, mountain ------------------
-    L2, nl2, nn2 = 85, 128, 256
-    pl2 = _bve_planet(L2, nl2, nn2)
+    run(args.warmup)
+    ms_step, kms, nl = _profiled(model, dom, run, args.steps)
+    ms_plain = _timed(run, args.steps, 0)
+    return dict(model=model, modes=modes, best=best, dom=dom, ms_step=ms_step,
+                ms_plain=ms_plain, dom_ms=kms, dom_launches=nl, cls_ms=cls_ms)
+
+
+def swe_modes(steps, warmup):
+    import torch
+    from aeolus_amd.physics import ShallowWaterModel, Topography
+    from aeolus_amd.run.initial_conditions import swe_spectra
+    L, nlat, nlon = 85, 128, 256
+    pl = _planet(L, nlat, nlon)
     rng = np.random.default_rng(85)
     lat0 = math.degrees(math.asin(rng.uniform(-1, 1)))
     lon0 = math.degrees(rng.uniform(0, 2 * math.pi))
-    topo = Topography.mountain(pl2, height_m=2000.0, lat_deg=lat0, lon_deg=lon0,
+    topo = Topography.mountain(pl, height_m=2000.0, lat_deg=lat0, lon_deg=lon0,
                                width_deg=math.degrees(1e6 / R_EARTH))
-    swe = ShallowWaterModel(pl2, gravity=9.80616, mean_depth=1e4, topography=topo)
-    y0 = torch.as_tensor(swe_spectra(L2, R_EARTH, 9.80616, seed=85), device=dev)
-    ms2, y2, _, _ = _timed_steps(swe, y0, 300.0, 288, 10, prof_cls=None)
-    res["swe85"] = dict(ms_per_step=ms2, steps_per_s=1000.0 / ms2,
-                        finite=bool(torch.isfinite(y2).all()))
-    return res, (pl, z0)
+    swe = ShallowWaterModel(pl, gravity=9.80616, mean_depth=1e4, topography=topo)
+    y0 = torch.as_tensor(swe_spectra(L, R_EARTH, 9.80616, seed=85), device="cuda")
+    out = torch.empty_like(y0)
+    run = lambda k: swe.rk4(y0, DT, k, out=out)
+    ref = swe.rk4(y0, DT, DAY_STEPS)
+    res = {}
+    for ns in (0,) + SPLITS:
+        swe.set_precision(ns)
+        day = swe.rk4(y0, DT, DAY_STEPS)
+        errs = [0.0 if ns == 0 else _rel(day[f], ref[f]) for f in range(3)]
+        res[ns] = {"ms_per_step": _timed(run, min(steps, 192), warmup),
+                   "rel_l2_vs_fp64_1day": dict(zip(("zeta", "delta", "phi"), errs)),
+                   "finite": bool(torch.isfinite(day).all())}
+    return res
+
+
+def ensemble_modes(steps, ns_split):
+    import torch
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    classes = [(21, 32, 64), (42, 64, 128), (85, 128, 256), (170, 256, 512)]
+    nper = 64
+    planets = [_planet(L, a, b) for L, a, b in classes]
+    states = [torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(1000 * L + k),
+                                                     rms=5e-5) for k in range(nper)]),
+                              device="cuda") for L, _, _ in classes]
+    res, ref = {}, None
+    for ns in (0, ns_split):
+        ens = Ensemble([(pl, nper) for pl in planets], precision=ns)
+        tuning = ens.autotune(states, DT, steps=2)
+        outs = [torch.empty_like(s) for s in states]
+        run = lambda k: ens.rk4(states, DT, k, out=outs, use_graph=True)
+        ms = _timed(run, steps, 4)
+        day = ens.rk4(states, DT, DAY_STEPS)
+        if ns == 0:
+            ref = day
+        errs = {f"L{L}": (0.0 if ns == 0 else _rel(d, r))
+                for (L, _, _), d, r in zip(classes, day, ref)}
+        res[ns] = {"ms_per_ensemble_step": ms, "member_steps_per_s": 4 * nper * 1000.0 / ms,
+                   "rel_l2_vs_fp64_1day": errs,
+                   "members_per_block": {f"L{L}": t["mb"] for (L, _, _), t in zip(classes, tuning)}}
+        del ens
+    return res
+
+
+def adjoint_cost(model, ns):
+    import torch
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    model.set_precision(ns)
+    n = 48
+    z0 = torch.as_tensor(spectrum_l15(170, np.random.default_rng(5), rms=5e-5), device="cuda")
+    yb = torch.as_tensor(spectrum_l15(170, np.random.default_rng(6), rms=5e-5), device="cuda")
+    out = torch.empty_like(z0)
+    reps = 5
+    fwd = _timed(lambda k: [model.rk4(z0, DT, n, out=out) for _ in range(k)], reps, 1)
+    _, ck = model.rk4_record(z0, DT, n)
+    rec = _timed(lambda k: [model.rk4_record(z0, DT, n, out=out) for _ in range(k)], reps, 1)
+    adj = _timed(lambda k: [model.rk4_adjoint(ck, DT, n, yb, out=out) for _ in range(k)], reps, 1)
+    return {"steps": n, "precision_slices": ns, "forward_ms": fwd, "record_ms": rec,
+            "adjoint_ms": adj, "adjoint_to_forward": adj / fwd,
+            "gradient_to_forward": (rec + adj) / fwd,
+            "checkpoint_MB": model.checkpoint_elems(n) * 16 / 1e6}
 
 
 def cpu_baseline(seconds: float):
@@ -146,58 +290,82 @@ def cpu_baseline(seconds: float):
                  nlon=512)
     y = spectrum_l15(170, np.random.default_rng(2026), rms=5e-5)
     tend = lambda s: bve_tendency(op, s)
-    y = rk4_step(tend, y, 300.0)  # warm-up step
+    y = rk4_step(tend, y, DT)  # warm-up step
     n = 0
     t0 = time.perf_counter()
     while True:
-        y = rk4_step(tend, y, 300.0)
+        y = rk4_step(tend, y, DT)
         n += 1
         if time.perf_counter() - t0 >= seconds:
             break
     el = time.perf_counter() - t0
-    cores = len(os.sched_getaffinity(0))
     return {"value": n / el, "unit": "RK4 steps/s", "cores": int(blas),
-            "host_cores_available": cores, "kind": "port",
+            "host_cores_available": len(os.sched_getaffinity(0)), "kind": "port",
             "sample": f"{n} RK4 steps of the L=170 Gauss 256x512 BVE step (oracle NumPy "
-                      f"FFT + stable-Legendre restatement), after 1 warm-up step"}
+                      f"FFT + stable-Legendre restatement, FP64), after 1 warm-up step"}
+
+
+def _mode_name(ns):
+    return "fp64" if ns == 0 else f"split{ns}"
 
 
 def main():
     args = _args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "measurements cover one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "every figure of this benchmark is one GPU's; "
+                                   "run with --gpus 1"}))
         sys.exit(2)
-    res, _ = run_gpu(args)
-    b = res["bve170"]
-    achieved = b["dom_bytes"] / (b["dom_ms"] * 1e-3) / 1e9
-    names = {0: "legendre_syn_kernel<JacLoader> (4 fields)", 1: "grid_jac_kernel (FFT + J)",
-             2: "legendre_ana_kernel<EpiBve> (analysis + RK4 epilogue)"}
+    h = headline(args)
+    best, dom = h["best"], h["dom"]
+    nbytes = _stage_bytes(dom, 170, 256, best)
+    achieved = nbytes / (h["dom_ms"] * 1e-3) / 1e9
+    flops = _stage_flops(dom, 170, 256)
+    secondary = {
+        "bve170_modes": {_mode_name(k): v for k, v in h["modes"].items()},
+        "bve170_ms_per_step_uninstrumented": h["ms_plain"],
+        "bve170_kernel_class_ms": {KCLASS[k]: v for k, v in h["cls_ms"].items()},
+        "dominant_kernel_gflops_per_s": flops / (h["dom_ms"] * 1e-3) / 1e9,
+    }
+    if not args.headline_only:
+        _, _, m85, b85 = bve_modes(85, 128, 256, args.steps, args.warmup, 85)
+        fp = h["modes"][0]["ms_per_step"]
+        secondary["bve170_split_speedup_vs_fp64"] = {
+            _mode_name(k): fp / v["ms_per_step"] for k, v in h["modes"].items() if k}
+        secondary["bve85_modes"] = {_mode_name(k): v for k, v in m85.items()}
+        secondary["bve85_best"] = _mode_name(b85)
+        secondary["swe85_mountain_modes"] = {_mode_name(k): v
+                                             for k, v in swe_modes(args.steps, 5).items()}
+        ns_split = best if best else 6
+        secondary["ensemble_256_members"] = {_mode_name(k): v for k, v in
+                                             ensemble_modes(48, ns_split).items()}
+        secondary["adjoint_L170"] = adjoint_cost(h["model"], ns_split)
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
-        "value": b["steps_per_s"],
+        "value": 1000.0 / h["ms_step"],
         "unit": "RK4 steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": b["ms_per_step"],
+        "ms_per_step": h["ms_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded l^-1.5 spectra)",
-        "config": {"workload": "BVE RK4, Gauss 256x512, L=170, single member, dt=300 s, "
-                               "plain FP64 (split precision not yet built)",
-                   "grid": "256x512", "l_max": 170, "dt_s": 300.0, "radius_m": R_EARTH},
-        "secondary": {"swe_L85_gauss128x256_mountain_steps_per_s": res["swe85"]["steps_per_s"],
-                      "bve170_ms_per_step_uninstrumented": b["ms_per_step_uninstrumented"],
-                      "kernel_class_ms": b["cls_ms"]},
-        "roofline": {"bound": "hbm", "kernel": names[b["dom_cls"]],
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": b["dom_bytes"], "launch_ms": b["dom_ms"],
-                     "launches": b["dom_launches"]},
+        "dtype": "f64" if best == 0 else f"bf16 digits x{best} (fp32 accum) + f64",
+        "data": "synthetic (seeded l^-1.5 spectra, random-phase)",
+        "config": {"workload": f"BVE RK4, Gauss 256x512, L=170, single member, dt=300 s, "
+                               f"{_mode_name(best)} Legendre contractions",
+                   "grid": "256x512", "l_max": 170, "dt_s": DT, "radius_m": R_EARTH,
+                   "precision": _mode_name(best), "accuracy_bar": "rel-L2 <= 1e-11 vs FP64 "
+                   "after 1 simulated day"},
+        "secondary": secondary,
+        "roofline": {"bound": "hbm", "kernel": KCLASS[dom], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": nbytes, "launch_ms": h["dom_ms"],
+                     "launches": h["dom_launches"]},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
-        "finite": b["finite"] and res["swe85"]["finite"],
     }
     print(json.dumps(line))
 
