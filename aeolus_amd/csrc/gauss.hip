@@ -454,6 +454,7 @@ template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
 static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& ep,
                            hipStream_t s) {
     const size_t lds = StageLds<KS, Epi, MB, NS>::bytes(p->L, p->nh, ANA, SYN);
+    if (lds > 163840) armed_events() = ArmedEvents{};
     AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
                 "Legendre stage kernel: LDS demand exceeds 160 KiB (reduce members per block)");
     auto kern = leg_stage_kernel<KS, Epi, MB, ANA, SYN, NS>;
@@ -462,7 +463,7 @@ static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& e
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     dim3 grid(p->L + 1, SYN ? (p->nh + 63) / 64 : 1, (a.nmem + MB - 1) / MB);
-    hipLaunchKernelGGL(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
+    launch_k(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
@@ -635,14 +636,12 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     sa.E = md->d_E;
     sa.O = md->d_O;
     sa.k = col_consts(p, md->R, md->d_lap, f10);
-    ProfRec& pr = md->prof;
-    pr.begin(kClsSynthesis, s);
+    md->prof.arm(kClsSynthesis);
     int rc = (md->kind == kBVE)
                  ? launch_stage<KindBve, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
                                                                  md->nslices)
                  : launch_stage<KindSwe, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
                                                                  md->nslices);
-    pr.end(kClsSynthesis, s);
     return rc;
 }
 
@@ -663,21 +662,22 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     const size_t fs = (size_t)(L + 1) * (L + 1);
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
-    pr.begin(kClsGrid, s);
     if (md->kind == kBVE) {
         int rc0 = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p));
         if (rc0) return rc0;
-        hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), jac_lds(p), s, p->gdev(), md->d_E, md->d_O,
-                           md->R, cut, md->nmem, md->d_S, md->d_D);
+        pr.arm(kClsGrid);
+        launch_k(grid_jac_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), jac_lds(p), s,
+                 p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
+                 md->nmem, md->d_S, md->d_D);
     } else {
         const size_t swe_lds = grid_lds(p, 12, 10);
         int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);
         if (rc0) return rc0;
-        hipLaunchKernelGGL(grid_swe_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds,
-                           s, p->gdev(), md->d_E, md->d_O, md->R, cut, md->nmem, md->d_S,
-                           md->d_D);
+        pr.arm(kClsGrid);
+        launch_k(grid_swe_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds, s,
+                 p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
+                 md->nmem, md->d_S, md->d_D);
     }
-    pr.end(kClsGrid, s);
     AEO_LAUNCH_CHECK();
     StageArgs sa{};
     sa.ys = ys;
@@ -694,7 +694,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     Rk4B rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
     int rc;
-    pr.begin(kClsAnalysis, s);
+    pr.arm(kClsAnalysis);
     if (md->kind == kBVE) {
         EpiBveS ep{L, fs, forcing, md->d_nulap, rk};
         rc = stage_ana<KindBve>(md, sa, ep, syn_next, s);
@@ -705,7 +705,6 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                    rk};
         rc = stage_ana<KindSwe>(md, sa, ep, syn_next, s);
     }
-    pr.end(kClsAnalysis, s);
     return rc;
 }
 
@@ -727,12 +726,9 @@ int gauss_adjoint_prime(aeo_model_s* md, const double2* yfwd, const double2* kb,
     sa.E = md->d_aE;
     sa.O = md->d_aO;
     sa.k = col_consts(p, md->R, md->d_lap, f10);
-    ProfRec& pr = md->prof;
-    pr.begin(kClsSynthesis, s);
-    int rc = launch_stage<KindBveAdj, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
-                                                             md->nslices);
-    pr.end(kClsSynthesis, s);
-    return rc;
+    md->prof.arm(kClsSynthesis);
+    return launch_stage<KindBveAdj, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
+                                                           md->nslices);
 }
 
 int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt,
@@ -746,10 +742,10 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     const size_t lds = grid_lds(p, 5, 5);
     int rc0 = allow_lds((const void*)grid_adj_kernel_for(p), lds);
     if (rc0) return rc0;
-    pr.begin(kClsGrid, s);
-    hipLaunchKernelGGL(grid_adj_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
-                       p->gdev(), md->d_aE, md->d_aO, md->R, md->nmem, md->d_aS, md->d_aD);
-    pr.end(kClsGrid, s);
+    pr.arm(kClsGrid);
+    launch_k(grid_adj_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
+             p->gdev(), (const double2*)md->d_aE, (const double2*)md->d_aO, md->R, md->nmem,
+             md->d_aS, md->d_aD);
     AEO_LAUNCH_CHECK();
     StageArgs sa{};
     sa.ys = nullptr;              // column 0 is filled by the epilogue (fwd_next)
@@ -768,14 +764,11 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     Rk4Adj rk{stage, dt, a, md->d_asum, anext, kout, knext};
     EpiBveAdjS ep{L, md->R, fs, md->d_nulap, md->d_lap, p->d_Cp, p->d_Cm, fwd_next, rk};
-    int rc;
-    pr.begin(kClsAnalysis, s);
+    pr.arm(kClsAnalysis);
     if (fwd_next)
-        rc = launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, md->mb, sa, ep, s, md->nslices);
-    else
-        rc = launch_stage<KindBveAdj, EpiBveAdjS, true, false>(p, md->mb, sa, ep, s, md->nslices);
-    pr.end(kClsAnalysis, s);
-    return rc;
+        return launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, md->mb, sa, ep, s,
+                                                                md->nslices);
+    return launch_stage<KindBveAdj, EpiBveAdjS, true, false>(p, md->mb, sa, ep, s, md->nslices);
 }
 
 }  // namespace aeo

@@ -103,18 +103,41 @@ enum ModelKind { kBVE = 0, kSWE = 1 };
 // Kernel classes of one model stage (for aeo_model_profile).
 enum KernelClass { kClsSynthesis = 0, kClsGrid = 1, kClsAnalysis = 2 };
 
-// Optional HIP-event bracketing of every launch of one kernel class.
+// Start/stop events for the next kernel launch of this host thread: armed
+// by ProfRec::arm, consumed by launch_k, which then launches with
+// hipExtLaunchKernelGGL so the events carry the dispatch's own begin and end
+// timestamps (no gap between an event record and the kernel).
+struct ArmedEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+inline ArmedEvents& armed_events() {
+    static thread_local ArmedEvents a;
+    return a;
+}
+
+template <class K, class... Args>
+inline void launch_k(K kern, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+    ArmedEvents& a = armed_events();
+    if (a.start) {
+        const hipEvent_t e0 = a.start, e1 = a.stop;
+        a = ArmedEvents{};
+        hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)lds, s, e0, e1, 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
+    }
+}
+
+// Optional timing of every launch of one kernel class (see launch_k).
 struct ProfRec {
     int cls = -1;
     int cap = 0;
     int n = 0;
     hipEvent_t* ev = nullptr;  // 2 * cap events
-    void begin(int c, hipStream_t s) {
-        if (c == cls && n < cap) (void)hipEventRecord(ev[2 * n], s);
-    }
-    void end(int c, hipStream_t s) {
+    // the next launch_k of this thread is timed if it belongs to class c
+    void arm(int c) {
+        armed_events() = ArmedEvents{};
         if (c == cls && n < cap) {
-            (void)hipEventRecord(ev[2 * n + 1], s);
+            armed_events() = ArmedEvents{ev[2 * n], ev[2 * n + 1]};
             ++n;
         }
     }
