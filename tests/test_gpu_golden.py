@@ -1,0 +1,121 @@
+"""Full-size trajectories against the oracle's golden fixtures (tests/golden,
+written by tests/golden/make_golden.py from the NumPy oracle).
+
+Bars: plain FP64 within 1e-12 relative L2 of the oracle; split-precision
+Legendre contractions (6 or 7 BF16 digit slices, FP64 state) within 1e-11
+relative L2 of the FP64 oracle after one simulated day, per field.
+"""
+import math
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+R = 6.371e6
+OMEGA = 2.0 * math.pi / 86400.0
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, f"{name}.npz"))
+
+
+def _tri(a):
+    n = a.shape[-1]
+    li, mi = np.tril_indices(n)
+    return a[..., li, mi]
+
+
+def _planet(kind, L, nlat=0, nlon=0, res=0):
+    from aeolus_amd.planet import Planet
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return Planet.with_radius(R, OMEGA, grid_type=kind, nlat=nlat, nlon=nlon,
+                                  grid_resolution=res, l_max=L, product_quadrature="fine")
+
+
+def test_c2_geodesic_trajectory():
+    """Geodesic res 4 / res 5 product, L = 21, 96 x 900 s."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print("case golden c2 geodesic L=21", flush=True)
+    g = _load("c2")
+    z0 = spectrum_l15(21, np.random.default_rng(int(g["seed"])), rms=5e-5)
+    assert float(np.sum(np.abs(z0))) == float(g["z0_checksum"])
+    model = BarotropicVorticity(_planet("geodesic", 21, res=4))
+    zT = model.rk4(torch.as_tensor(z0, device="cuda"), float(g["dt"]), int(g["nsteps"]))
+    assert rel_l2(_tri(zT.cpu().numpy()), g["zT"]) < 1e-12
+
+
+@pytest.mark.parametrize("name,L,nlat,nlon", [("bve85", 85, 128, 256),
+                                              ("bve170", 170, 256, 512)])
+def test_bve_one_day_fp64_and_split(name, L, nlat, nlon):
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    g = _load(name)
+    z0 = spectrum_l15(L, np.random.default_rng(int(g["seed"])), rms=5e-5)
+    assert float(np.sum(np.abs(z0))) == float(g["z0_checksum"])
+    model = BarotropicVorticity(_planet("latlon", L, nlat, nlon))
+    zt = torch.as_tensor(z0, device="cuda")
+    for ns, bar in ((0, 1e-12), (6, 1e-11), (7, 1e-11)):
+        print(f"case golden {name} nslices={ns}", flush=True)
+        model.set_precision(ns)
+        zT = model.rk4(zt, float(g["dt"]), int(g["nsteps"]))
+        err = rel_l2(_tri(zT.cpu().numpy()), g["zT"])
+        print(f"  rel-L2 vs oracle after one day: {err:.3e}", flush=True)
+        assert err < bar, (ns, err)
+
+
+def test_c3_swe_one_day_fp64_and_split():
+    """Shallow water L = 85 with the 2 km mountain, 288 x 300 s, per field."""
+    import torch
+    from aeolus_amd.physics import ShallowWaterModel, Topography
+    from aeolus_amd.run.initial_conditions import swe_spectra
+    from oracle.models import Planet as OPlanet, mountain
+    g = _load("c3")
+    h, lat0, lon0, width = (float(v) for v in g["mountain"])
+    op = OPlanet(l_max=85, radius=R, omega=OMEGA, grid_type="latlon", nlat=128, nlon=256)
+    elev = mountain(op, height_m=h, lat_deg=lat0, lon_deg=lon0, width_deg=width)
+    y0 = swe_spectra(85, R, 9.80616, seed=int(g["seed"]))
+    assert float(np.sum(np.abs(y0))) == float(g["y0_checksum"])
+    topo = Topography(torch.as_tensor(elev, device="cuda"), preset="mountain")
+    swe = ShallowWaterModel(_planet("latlon", 85, 128, 256), gravity=9.80616, mean_depth=1e4,
+                            topography=topo)
+    yt = torch.as_tensor(y0, device="cuda")
+    for ns, bar in ((0, 1e-12), (6, 1e-11)):
+        print(f"case golden c3 swe nslices={ns}", flush=True)
+        swe.set_precision(ns)
+        yT = _tri(swe.rk4(yt, float(g["dt"]), int(g["nsteps"])).cpu().numpy())
+        for f, name in enumerate(("zeta", "delta", "phi")):
+            err = rel_l2(yT[f], g["yT"][f])
+            print(f"  {name}: {err:.3e}", flush=True)
+            assert err < bar, (ns, name, err)
+
+
+def test_ensemble_members_match_golden_days():
+    """Uneven-L ensemble (graph replay, split 6 slices): the golden members
+    of the L = 85 and L = 170 classes stay within 1e-11 of the oracle."""
+    import torch
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print("case golden ensemble L85+L170 split6", flush=True)
+    g85, g170 = _load("bve85"), _load("bve170")
+    p85, p170 = _planet("latlon", 85, 128, 256), _planet("latlon", 170, 256, 512)
+    ens = Ensemble([(p85, 3), (p170, 3)], precision=6)
+    states = []
+    for L, g in ((85, g85), (170, g170)):
+        zs = [spectrum_l15(L, np.random.default_rng(int(g["seed"])), rms=5e-5)]
+        zs += [spectrum_l15(L, np.random.default_rng(7000 + k), rms=5e-5) for k in range(2)]
+        states.append(torch.as_tensor(np.stack(zs), device="cuda"))
+    out = ens.rk4(states, 300.0, 288, use_graph=True)
+    for y, g in zip(out, (g85, g170)):
+        err = rel_l2(_tri(y[0].cpu().numpy()), g["zT"])
+        print(f"  rel-L2 vs oracle: {err:.3e}", flush=True)
+        assert err < 1e-11
