@@ -61,7 +61,7 @@ static void free_plan(aeo_plan_s* p) {
                     (void*)p->d_pcoslat, (void*)p->d_gw, (void*)p->d_part, (void*)p->d_Cp,
                     (void*)p->d_Cm, (void*)p->d_spec, (void*)p->d_Psl, (void*)p->d_offs_ss,
                     (void*)p->d_esyn, (void*)p->d_Pasl, (void*)p->d_offs_sa, (void*)p->d_eana,
-                    (void*)p->d_offs_ea})
+                    (void*)p->d_offs_ea, (void*)p->d_xn, (void*)p->d_rec})
         dev_free(q);
     delete p;
 }
@@ -173,7 +173,15 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
         tw[t] = make_double2(std::cos(ang), -std::sin(ang));
     }
     std::vector<double> xn(x_nodes, x_nodes + nh);
-    double* d_x = nullptr;
+    // coefficients of Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}), m-major
+    std::vector<double2> rec((size_t)(L + 1) * (L + 1), make_double2(0.0, 0.0));
+    for (int m = 0; m <= L; ++m)
+        for (int l = m + 2; l <= L; ++l) {
+            const double ll = (double)l * l, mm = (double)m * m, l1 = l - 1.0;
+            rec[(size_t)m * (L + 1) + l] =
+                make_double2(std::sqrt((4.0 * ll - 1.0) / (ll - mm)),
+                             std::sqrt((l1 * l1 - mm) / (4.0 * l1 * l1 - 1.0)));
+        }
     int rc = AEO_OK;
     const size_t plane = (size_t)nh * (L + 1);
     const int nwork = ((std::max(max_fields, 12) + 3) / 4) * 4;
@@ -188,7 +196,8 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
         if ((rc = dev_alloc(&p->d_O, plane * nwork))) break;
         if ((rc = dev_alloc(&p->d_S, plane * nwork))) break;
         if ((rc = dev_alloc(&p->d_D, plane * nwork))) break;
-        if ((rc = upload(&d_x, xn.data(), xn.size()))) break;
+        if ((rc = upload(&p->d_xn, xn.data(), xn.size()))) break;
+        if ((rc = upload(&p->d_rec, rec.data(), rec.size()))) break;
         if ((rc = common_tables(p))) break;
         p->fft.n = nlon;
         p->fft.nstages = nst;
@@ -200,14 +209,13 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
             while ((1 << lg) < nlon) ++lg;
             p->fft.lgn = lg;
         }
-        if ((rc = gauss_build_tables(p, d_x, nullptr))) break;
+        if ((rc = gauss_build_tables(p, nullptr))) break;
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) {
             set_error(std::string("aeo_plan_create_gauss: ") + hipGetErrorString(e));
             rc = AEO_ERR_HIP;
         }
     } while (0);
-    dev_free(d_x);
     if (rc) {
         free_plan(p);
         return rc;

@@ -34,20 +34,23 @@ __host__ __device__ inline int row_cap(int n) {
 //   Pbar_mm = -sqrt((2m+1)/(2m)) s Pbar_{m-1,m-1}      (Condon-Shortley phase)
 //   Pbar_{m+1,m} = sqrt(2m+3) x Pbar_mm
 //   Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m})
+// with a_lm, b_lm from the plan's recurrence table; rec_step is the exact
+// expression the synthesis kernels use to continue the recurrence from two
+// table rows, so both produce the same values.
 // ---------------------------------------------------------------------------
-__global__ void legendre_table_kernel(int L, int nh, const double* __restrict__ x_north,
-                                      const long long* __restrict__ offs,
-                                      double* __restrict__ Ps, double* __restrict__ Pa) {
+__global__ void legendre_table_kernel(GaussDev g, double* __restrict__ Ps,
+                                      double* __restrict__ Pa) {
     const int m = blockIdx.x;
     const int jp = blockIdx.y * blockDim.x + threadIdx.x;
+    const int L = g.L, nh = g.nh;
     if (m > L || jp >= nh) return;
-    const double x = x_north[jp];
+    const double x = g.xn[jp];
     const double s = sqrt((1.0 - x) * (1.0 + x));
     double pmm = 0.28209479177387814347;  // 1/sqrt(4 pi)
     for (int k = 1; k <= m; ++k) pmm *= -sqrt((2.0 * k + 1.0) / (2.0 * k)) * s;
     const int K = L - m + 1;
-    double* ps = Ps + offs[m];
-    double* pa = Pa + offs[m];
+    double* ps = Ps + g.offs[m];
+    double* pa = Pa + g.offs[m];
     double p2 = 0.0, p1 = pmm;
     ps[jp] = pmm;
     pa[(size_t)jp * K] = pmm;
@@ -58,11 +61,9 @@ __global__ void legendre_table_kernel(int L, int nh, const double* __restrict__ 
         p2 = pmm;
         p1 = p;
     }
+    const double2* rc = g.rec + (size_t)m * (L + 1);
     for (int l = m + 2; l <= L; ++l) {
-        const double ll = (double)l * l, mm = (double)m * m;
-        const double a = sqrt((4.0 * ll - 1.0) / (ll - mm));
-        const double b = sqrt(((l - 1.0) * (l - 1.0) - mm) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-        const double p = a * (x * p1 - b * p2);
+        const double p = rec_step(rc[l], x, p1, p2);
         ps[(size_t)(l - m) * nh + jp] = p;
         pa[(size_t)jp * K + (l - m)] = p;
         p2 = p1;
@@ -462,7 +463,7 @@ static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& e
         AEO_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    dim3 grid(p->L + 1, SYN ? (p->nh + 63) / 64 : 1, (a.nmem + MB - 1) / MB);
+    dim3 grid(p->L + 1, 1, (a.nmem + MB - 1) / MB);
     launch_k(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
@@ -499,10 +500,9 @@ static int launch_stage(const aeo_plan_s* p, int mb, const StageArgs& a, const E
     }
 }
 
-int gauss_build_tables(aeo_plan_s* p, const double* d_x, hipStream_t s) {
+int gauss_build_tables(aeo_plan_s* p, hipStream_t s) {
     dim3 grid(p->L + 1, (p->nh + 63) / 64);
-    hipLaunchKernelGGL(legendre_table_kernel, grid, dim3(64), 0, s, p->L, p->nh, d_x, p->d_offs,
-                       p->d_Ps, p->d_Pa);
+    hipLaunchKernelGGL(legendre_table_kernel, grid, dim3(64), 0, s, p->gdev(), p->d_Ps, p->d_Pa);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }

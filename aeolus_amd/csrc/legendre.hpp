@@ -26,14 +26,22 @@ namespace aeo {
 
 constexpr int kStageThreads = 512;
 constexpr int kStageWaves = kStageThreads / 64;
-// FP64 table loads issued ahead per wave (analysis: per 64 degrees, for the
-// first PC chunks of 64; synthesis: rows), smaller for wide member/field tiles
+// FP64 analysis-table loads issued ahead per wave (per 64 degrees, for the
+// first PC chunks of 64), fewer for wide member/field tiles
 template <int MB, int NA, int NF>
 struct Prefetch {
     static constexpr int PA = (MB * NA <= 2) ? 16 : 8;
     static constexpr int PC = (MB * NA <= 2) ? 2 : 1;
-    static constexpr int PS = (MB * NF <= 5) ? 24 : 12;
 };
+
+// One step of the Pbar recurrence: a (x p1 - b p2) with (a, b) = ab.  The
+// table kernel and the synthesis kernels share this exact expression.
+__device__ inline double rec_step(double2 ab, double x, double p1, double p2) {
+    return ab.x * fma(x, p1, -(ab.y * p2));
+}
+
+// p * c as the first term of a multiply-add chain (fma(p, c, 0) rounds alike)
+__device__ inline double pe_mul(double p, double c) { return fma(p, c, 0.0); }
 
 struct ColK {
     int L;
@@ -400,8 +408,9 @@ struct StageArgs {
     ColK k;
 };
 
-// double2 slots holding the three FP64 coefficient columns (C+, C-, lap) of m
-__host__ __device__ inline int coef_slots(int n) { return (3 * n + 1) / 2; }
+// double2 slots of the per-column constants in LDS: C+, C-, lap (FP64, 3n
+// doubles, rounded up) and the recurrence pairs (a_lm, b_lm) (n double2)
+__host__ __device__ inline int coef_slots(int n) { return (3 * n + 1) / 2 + n; }
 
 template <class KS, class Epi, int MB, int NS>
 struct StageLds {
@@ -419,8 +428,7 @@ struct StageLds {
                             sizeof(int) * 2 * NTA * 16 + 2 * sizeof(__bf16) * NS * NTA * 16 * rup(nh, 32) + 64;
         }
         if (syn) {
-            c = sizeof(double2) * ((size_t)MB * KS::NF * n +
-                                   (size_t)(kStageWaves - 2) * MB * KS::NF * 64);
+            c = sizeof(double2) * ((size_t)MB * KS::NF * n + 2 * 2 * 2 * (size_t)MB * KS::NF * 64);
             if (NS > 0) c = sizeof(double2) * ((size_t)MB * KS::NF * n) + sizeof(int) * NTS * 16 +
                             2 * sizeof(__bf16) * NS * NTS * 16 * rup(split_kp(L, 0), 32) + 64;
         }
@@ -463,10 +471,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     constexpr int NF = KS::NF;
     constexpr int kPre = Prefetch<MB, NA, NF>::PA;
     constexpr int kPC = Prefetch<MB, NA, NF>::PC;
-    constexpr int kPreS = Prefetch<MB, NA, NF>::PS;
     extern __shared__ double2 lds[];
     const int m = blockIdx.x;
-    const int jt = blockIdx.y;
     const int b0 = blockIdx.z * MB;
     const int L = g.L, nh = g.nh, n = L + 1;
     const int K = L - m + 1;
@@ -477,6 +483,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     double* cpc = reinterpret_cast<double*>(lds + (size_t)MB * NST * n);   // C+ column m
     double* cmc = cpc + n;                                                 // C- column m
     double* lapc = cmc + n;                                                // lap
+    double2* rcc = lds + (size_t)MB * NST * n + (3 * n + 1) / 2;           // (a, b) of m
     double2* work = lds + (size_t)MB * NST * n + coef_slots(n);
     ColK kc = a.k;
     kc.cpc = cpc;
@@ -484,12 +491,11 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     if (a.k.lap) kc.lap = lapc;
 
     // -- prologue: every load that does not depend on another workgroup's
-    //    results of this launch is issued before the first wait.  FP64 table
-    //    rows go to registers (pa: first 64 kPC analysed degrees, ps: first
-    //    kPreS synthesis rows of this wave); state, S/D and coefficient
-    //    columns go through registers into LDS.
+    //    results of this launch is issued before the first wait.  FP64
+    //    analysis-table rows go to registers (pa: the first 64 kPC analysed
+    //    degrees); state, S/D and coefficient columns go through registers
+    //    into LDS.
     double pa[kPC][kPre];
-    double ps[kPreS];
     if constexpr (ANA && NS == 0) {
         const double* Pa = g.Pa + g.offs[m];
 #pragma unroll
@@ -500,18 +506,21 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 pa[c][u] = (active && l <= top && jp < nh) ? Pa[(size_t)jp * K + (l - m)] : 0.0;
             }
     }
-    // synthesis rows: issued here for synthesis-only launches, otherwise just
-    // before the epilogue (overlapping its loads, not held through the analysis)
-    auto load_ps = [&]() {
-        const int jps = jt * 64 + lane;
-        const double* Ps = g.Ps + g.offs[m] + jps;
-#pragma unroll
-        for (int u = 0; u < kPreS; ++u) {
-            const int i = wave + kStageWaves * u;
-            ps[u] = (jps < nh && i < K) ? Ps[(size_t)i * nh] : 0.0;
+
+    // FP64 synthesis seeds of this lane for the first latitude pass: node x
+    // and the table rows at the start of the lane's degree block
+    double sx = 0.0, sp0 = 0.0, sp1 = 0.0;
+    if constexpr (SYN && NS == 0) {
+        const int kb = wave & 3, jp = (wave >> 2) * 64 + lane;
+        const int Kb = ((K + 3) / 4 + 1) & ~1;
+        const int i0 = kb * Kb;
+        if (jp < nh && i0 < K) {
+            const double* P = g.Ps + g.offs[m] + jp;
+            sx = g.xn[jp];
+            sp0 = P[(size_t)i0 * nh];
+            if (i0 + 1 < K) sp1 = P[(size_t)(i0 + 1) * nh];
         }
-    };
-    if constexpr (SYN && NS == 0 && !ANA) load_ps();
+    }
     double2* X = work;                                  // [MB][NA][2][nh]  (S then D)
     double2* accl = X + (size_t)MB * NA * 2 * nh;       // [MB][NA][n]
     {
@@ -521,7 +530,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         // loads are unconditional where possible (indices clamped in range) so
         // the values stay in registers; only the LDS stores are predicated
         for (int t0 = 0; t0 < tot; t0 += 2 * kStageThreads) {
-            double2 vs[2], vx[2][2];
+            double2 vs[2], vx[2][2], vr[2];
             double vc[2][3];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -558,6 +567,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 vc[u][0] = a.k.Cp[(size_t)tc * n + m];
                 vc[u][1] = a.k.Cm[(size_t)tc * n + m];
                 vc[u][2] = a.k.lap ? a.k.lap[tc] : 0.0;
+                vr[u] = g.rec[(size_t)m * n + tc];
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -574,6 +584,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     cpc[t] = vc[u][0];
                     cmc[t] = vc[u][1];
                     lapc[t] = vc[u][2];
+                    rcc[t] = vr[u];
                 }
             }
         }
@@ -675,27 +686,36 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             __syncthreads();
         }
         if constexpr (NS == 0) if (active) {
+            // one chunk of 64 degrees per pass (lanes = degrees); the first kPC
+            // chunks use the rows prefetched in the prologue
+            constexpr int kCH = 1;
             const double* Pa = g.Pa + g.offs[m];
-            for (int l0 = m, c = 0; l0 <= top; l0 += 64, ++c) {
-                const int l = l0 + lane;
-                double2 acc[MB][NA];
+            for (int l0 = m, c = 0; l0 <= top; l0 += 64 * kCH, ++c) {
+                double2 acc[kCH][MB][NA];
 #pragma unroll
-                for (int bb = 0; bb < MB; ++bb)
+                for (int h = 0; h < kCH; ++h)
 #pragma unroll
-                    for (int f = 0; f < NA; ++f) acc[bb][f] = c2(0.0, 0.0);
-                if (l <= top) {
-                    const int par = (l - m) & 1;
-                    const double* P = Pa + (l - m);
+                    for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                        for (int f = 0; f < NA; ++f) acc[h][bb][f] = c2(0.0, 0.0);
+                if (l0 + lane <= top) {
+                    const int par = (l0 + lane - m) & 1;
                     for (int j0 = wave; j0 < nh; j0 += kStageWaves * kPre) {
-                        double pv[kPre];
+                        double pv[kCH][kPre];
                         if (j0 == wave && c < kPC) {
 #pragma unroll
-                            for (int u = 0; u < kPre; ++u) pv[u] = (c == 0) ? pa[0][u] : pa[kPC - 1][u];
+                            for (int u = 0; u < kPre; ++u)
+                                pv[0][u] = (c == 0) ? pa[0][u] : pa[kPC - 1][u];
                         } else {
 #pragma unroll
-                            for (int u = 0; u < kPre; ++u) {
-                                const int jp = j0 + kStageWaves * u;
-                                pv[u] = (jp < nh) ? P[(size_t)jp * K] : 0.0;
+                            for (int h = 0; h < kCH; ++h) {
+                                const int l = l0 + 64 * h + lane;
+#pragma unroll
+                                for (int u = 0; u < kPre; ++u) {
+                                    const int jp = j0 + kStageWaves * u;
+                                    pv[h][u] = (jp < nh && l <= top) ? Pa[(size_t)jp * K + (l - m)]
+                                                                     : 0.0;
+                                }
                             }
                         }
 #pragma unroll
@@ -707,36 +727,42 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 #pragma unroll
                                     for (int f = 0; f < NA; ++f) {
                                         const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
-                                        acc[bb][f].x = fma(pv[u], x.x, acc[bb][f].x);
-                                        acc[bb][f].y = fma(pv[u], x.y, acc[bb][f].y);
+#pragma unroll
+                                        for (int h = 0; h < kCH; ++h) {
+                                            acc[h][bb][f].x = fma(pv[h][u], x.x, acc[h][bb][f].x);
+                                            acc[h][bb][f].y = fma(pv[h][u], x.y, acc[h][bb][f].y);
+                                        }
                                     }
                             }
                         }
                     }
                 }
 #pragma unroll
-                for (int bb = 0; bb < MB; ++bb)
+                for (int h = 0; h < kCH; ++h)
 #pragma unroll
-                    for (int f = 0; f < NA; ++f)
-                        red[((wave * MB + bb) * NA + f) * 64 + lane] = acc[bb][f];
+                    for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                        for (int f = 0; f < NA; ++f)
+                            red[((wave * MB + bb) * NA + f) * (64 * kCH) + 64 * h + lane] =
+                                acc[h][bb][f];
                 __syncthreads();
-                for (int t = threadIdx.x; t < MB * NA * 64; t += kStageThreads) {
-                    const int ll = l0 + (t & 63);
+                for (int t = threadIdx.x; t < MB * NA * 64 * kCH; t += kStageThreads) {
+                    const int r = t % (64 * kCH);
+                    const int ll = l0 + r;
                     if (ll <= top) {
-                        const int q = t >> 6;  // (bb * NA + f)
-                        double2 s = red[q * 64 + (t & 63)];
+                        const int q = t / (64 * kCH);  // (bb * NA + f)
+                        double2 sv = red[q * (64 * kCH) + r];
 #pragma unroll
                         for (int w = 1; w < kStageWaves; ++w)
-                            s = cadd(s, red[(w * MB * NA + q) * 64 + (t & 63)]);
-                        accl[q * n + ll] = s;
+                            sv = cadd(sv, red[(w * MB * NA + q) * (64 * kCH) + r]);
+                        accl[q * n + ll] = sv;
                     }
                 }
                 __syncthreads();
             }
         }
-        if constexpr (SYN && NS == 0) load_ps();
         // epilogue over every (l, m) of the column and every member
-        const bool writer = (jt == 0);
+        const bool writer = true;   // one workgroup per (m, member group)
         for (int t = threadIdx.x; t < MB * n; t += kStageThreads) {
             const int bb = t / n, l = t - bb * n;
             const int b = b0 + bb;
@@ -781,14 +807,13 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             for (int q = 0; q < NS; ++q) qs[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = (__bf16)d[q];
         }
         __syncthreads();
-        // waves 0-3 / 4-7: even / odd degrees of the same four 16-row tiles
-        static_assert(kStageWaves == 8, "split synthesis assigns one parity per 4 waves");
-        const int j0 = jt * 64 + (wave & 3) * 16;
-        if (j0 < nh) {
-            const __bf16* Pt = g.Psl + g.offs_ss[m];
-            const int* er = g.esyn + (size_t)m * 2 * nhp;
+        // (16-latitude tile, parity) pairs spread over the waves
+        const __bf16* Pt = g.Psl + g.offs_ss[m];
+        const int* er = g.esyn + (size_t)m * 2 * nhp;
+        for (int tt = wave; tt < 2 * (nhp / 16); tt += kStageWaves) {
+            const int p = tt & 1;
+            const int j0 = (tt >> 1) * 16;
             {
-                const int p = wave >> 2;
                 floatx4 acc[NS][NTS];
 #pragma unroll
                 for (int q = 0; q < NS; ++q)
@@ -838,71 +863,126 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         }
     }
     if constexpr (SYN && NS == 0) {
+        // FP64 synthesis, every latitude of column m in this workgroup.  Wave
+        // w takes degree block w % 4 (contiguous, even length) for the 64
+        // latitudes of group w / 4; Pbar along the block comes from the
+        // recurrence (rec_step) seeded by two table rows, so the table is read
+        // twice per block instead of once per degree, and all lanes of a wave
+        // read the same column entry (LDS broadcast).  The four block sums
+        // of each latitude meet in a two-round LDS tree.
+        static_assert(kStageWaves == 8, "synthesis maps 8 waves to 4 blocks x 2 latitude groups");
         double2* cols = work;                               // [MB][NF][K]
-        double2* red = cols + (size_t)MB * NF * n;          // [2][MB][NF][64]
+        double2* red = cols + (size_t)MB * NF * K;          // [2 lg][2][2 parity][MB][NF][64]
         for (int bb = 0; bb < MB; ++bb)
             derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
-        const int jp = jt * 64 + lane;
-        double2 acc[MB][NF];
+        constexpr int kRecG = (MB * NF <= 4) ? 4 : 2;      // degrees per recurrence group
+        const int kb = wave & 3, lg = wave >> 2;
+        const int Kb = ((K + 3) / 4 + 1) & ~1;              // even block length
+        const int i0 = kb * Kb, i1 = (i0 + Kb < K) ? i0 + Kb : K;
+        auto slot = [&](int s2, int par, int bb, int f) -> double2& {
+            return red[((((lg * 2 + s2) * 2 + par) * MB + bb) * NF + f) * 64 + lane];
+        };
+        for (int jbase = 0; jbase < nh; jbase += 128) {
+            const int jp = jbase + lg * 64 + lane;
+            const bool ok = jp < nh;
+            double2 accE[MB][NF], accO[MB][NF];
 #pragma unroll
-        for (int bb = 0; bb < MB; ++bb)
+            for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-            for (int f = 0; f < NF; ++f) acc[bb][f] = c2(0.0, 0.0);
-        if (jp < nh) {
-            // kPre table rows in flight per wave before the multiply-adds
-            const double* P = g.Ps + g.offs[m] + jp;
-            for (int i0 = wave; i0 < K; i0 += kStageWaves * kPreS) {
-                double pv[kPreS];
-                if (i0 == wave) {
+                for (int f = 0; f < NF; ++f) accE[bb][f] = accO[bb][f] = c2(0.0, 0.0);
+            if (ok && i0 < i1) {
+                const double* P = g.Ps + g.offs[m] + jp;
+                const bool first = (jbase == 0);        // seeds issued in the prologue
+                const double x = first ? sx : g.xn[jp];
+                double p2 = first ? sp0 : P[(size_t)i0 * nh];
+                double p1 = first ? sp1 : ((i0 + 1 < i1) ? P[(size_t)(i0 + 1) * nh] : 0.0);
 #pragma unroll
-                    for (int u = 0; u < kPreS; ++u) pv[u] = ps[u];
-                } else {
+                for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-                    for (int u = 0; u < kPreS; ++u) {
-                        const int i = i0 + kStageWaves * u;
-                        pv[u] = (i < K) ? P[(size_t)i * nh] : 0.0;
+                    for (int f = 0; f < NF; ++f) {
+                        const double2* cf = cols + (bb * NF + f) * K;
+                        const double2 ce = cf[i0];
+                        const double2 co = cf[(i0 + 1 < i1) ? i0 + 1 : i0];
+                        accE[bb][f] = c2(pe_mul(p2, ce.x), pe_mul(p2, ce.y));
+                        accO[bb][f] = c2(pe_mul(p1, co.x), pe_mul(p1, co.y));
                     }
-                }
+                // groups of kRecG degrees (i0 + 2 is even, so degree parity =
+                // parity of u): every LDS read of a group is issued before its
+                // recurrence and multiply-adds; out-of-block degrees get p = 0
+                for (int i = i0 + 2; i < i1; i += kRecG) {
+                    double2 ab[kRecG];
+                    double2 cv[kRecG][MB][NF];
 #pragma unroll
-                for (int u = 0; u < kPreS; ++u) {
-                    const int i = i0 + kStageWaves * u;
-                    if (i < K) {
+                    for (int u = 0; u < kRecG; ++u) {
+                        const int ii = (i + u < i1) ? i + u : i1 - 1;
+                        ab[u] = rcc[m + ii];
+#pragma unroll
+                        for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                            for (int f = 0; f < NF; ++f) cv[u][bb][f] = cols[(bb * NF + f) * K + ii];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kRecG; ++u) {
+                        double p = rec_step(ab[u], x, p1, p2);
+                        p = (i + u < i1) ? p : 0.0;
+                        p2 = p1;
+                        p1 = p;
 #pragma unroll
                         for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
                             for (int f = 0; f < NF; ++f) {
-                                const double2 c = cols[(bb * NF + f) * K + i];
-                                acc[bb][f].x = fma(pv[u], c.x, acc[bb][f].x);
-                                acc[bb][f].y = fma(pv[u], c.y, acc[bb][f].y);
+                                double2& acc = (u & 1) ? accO[bb][f] : accE[bb][f];
+                                acc.x = fma(p, cv[u][bb][f].x, acc.x);
+                                acc.y = fma(p, cv[u][bb][f].y, acc.y);
                             }
                     }
                 }
             }
-        }
-        if (wave >= 2) {
+            // round 1: blocks 2, 3 -> blocks 0, 1
+            if (kb >= 2) {
 #pragma unroll
-            for (int bb = 0; bb < MB; ++bb)
+                for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    red[(((wave - 2) * MB + bb) * NF + f) * 64 + lane] = acc[bb][f];
-        }
-        __syncthreads();
-        if (wave < 2 && jp < nh) {
-            // waves of equal parity (wave mod 2) hold partial sums of E (even) / O (odd)
-            double2* dst = (wave == 0) ? a.E : a.O;
+                    for (int f = 0; f < NF; ++f) {
+                        slot(kb - 2, 0, bb, f) = accE[bb][f];
+                        slot(kb - 2, 1, bb, f) = accO[bb][f];
+                    }
+            }
+            __syncthreads();
+            if (kb < 2) {
 #pragma unroll
-            for (int bb = 0; bb < MB; ++bb) {
-                const int b = b0 + bb;
-                if (b >= a.nmem) continue;
+                for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    double2 v = acc[bb][f];
+                    for (int f = 0; f < NF; ++f) {
+                        accE[bb][f] = cadd(accE[bb][f], slot(kb, 0, bb, f));
+                        accO[bb][f] = cadd(accO[bb][f], slot(kb, 1, bb, f));
+                    }
+            }
+            // round 2: block 1 -> block 0 (block 1 reuses its own slot)
+            if (kb == 1) {
 #pragma unroll
-                    for (int w = wave + 2; w < kStageWaves; w += 2)
-                        v = cadd(v, red[(((w - 2) * MB + bb) * NF + f) * 64 + lane]);
-                    dst[(((size_t)jp * n + m) * a.nmem + b) * NF + f] = v;
+                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) {
+                        slot(1, 0, bb, f) = accE[bb][f];
+                        slot(1, 1, bb, f) = accO[bb][f];
+                    }
+            }
+            __syncthreads();
+            if (kb == 0 && ok) {
+#pragma unroll
+                for (int bb = 0; bb < MB; ++bb) {
+                    const int b = b0 + bb;
+                    if (b >= a.nmem) continue;
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) {
+                        const size_t o = (((size_t)jp * n + m) * a.nmem + b) * NF + f;
+                        a.E[o] = cadd(accE[bb][f], slot(1, 0, bb, f));
+                        a.O[o] = cadd(accO[bb][f], slot(1, 1, bb, f));
+                    }
                 }
             }
+            __syncthreads();
         }
     }
 }

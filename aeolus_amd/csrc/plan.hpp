@@ -16,6 +16,8 @@ struct GaussDev {
     const long long* offs;    // element offset of block m (L+2 entries)
     const double* w_rows;     // nlat solid-angle weights per point of a row
     const double* coslat;     // nlat cos(latitude), unclamped
+    const double* xn;         // nh northern nodes x_j = cos(colatitude)
+    const double2* rec;       // [m][l] (a_lm, b_lm) of the Pbar recurrence (l >= m + 2)
     FftDev fft;
     // Split-precision tables (nslices > 0): BF16 integer digits of Pbar with
     // one power-of-two exponent per table row (see split.hpp).
@@ -50,6 +52,8 @@ struct aeo_plan_s {
     long long* d_offs = nullptr;
     double* d_w_rows = nullptr;
     double* d_coslat = nullptr;
+    double* d_xn = nullptr;
+    double2* d_rec = nullptr;
     double2* d_tw = nullptr;
     aeo::FftDev fft{};
     int nslices = 0;
@@ -84,7 +88,7 @@ struct aeo_plan_s {
         aeo::GaussDev g;
         g.L = L; g.nlat = nlat; g.nlon = nlon; g.nh = nh;
         g.Ps = d_Ps; g.Pa = d_Pa; g.offs = d_offs;
-        g.w_rows = d_w_rows; g.coslat = d_coslat; g.fft = fft;
+        g.w_rows = d_w_rows; g.coslat = d_coslat; g.xn = d_xn; g.rec = d_rec; g.fft = fft;
         g.nslices = nslices;
         g.Psl = d_Psl; g.offs_ss = d_offs_ss; g.esyn = d_esyn;
         g.Pasl = d_Pasl; g.offs_sa = d_offs_sa; g.eana = d_eana; g.offs_ea = d_offs_ea;
@@ -186,7 +190,7 @@ struct aeo_model_s {
 namespace aeo {
 
 // Launchers implemented in gauss.hip / points.hip / spectral.hip.
-int gauss_build_tables(aeo_plan_s* p, const double* h_x, hipStream_t s);
+int gauss_build_tables(aeo_plan_s* p, hipStream_t s);
 int gauss_build_split_tables(aeo_plan_s* p, int nslices, hipStream_t s);
 int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
