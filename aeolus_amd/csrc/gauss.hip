@@ -135,16 +135,16 @@ enum SpecMode { kSpecPlain = 0, kSpecDlam = 1, kSpecScale = 2 };
 // optional (i m / R) or 1/R factor.  Its unnormalised inverse DFT is
 // f_N + i f_S.
 __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er,
-                                     const double2* Or, size_t ms, int mode, double R, int t0,
+                                     const double2* Or, size_t ms, int mode, double invR, int t0,
                                      int tstride) {
-    // Er[m * ms], Or[m * ms]: even/odd partial sums of one field at wavenumber m
-    const double invR = 1.0 / R;
+    // Er[m * ms], Or[m * ms]: even/odd partial sums of one field at wavenumber m;
+    // invR = 1/R (the (i m / R) factor is formed as i (m * invR))
     for (int b = t0; b < n; b += tstride) {
         double2 z = c2(0.0, 0.0);
         for (int m = b; m <= L; m += n) {
             double2 e = Er[m * ms], o = Or[m * ms];
             double2 gn = cadd(e, o), gs = csub(e, o);
-            if (mode == kSpecDlam) { gn = dlam(gn, m, R); gs = dlam(gs, m, R); }
+            if (mode == kSpecDlam) { gn = dlam_q(gn, m * invR); gs = dlam_q(gs, m * invR); }
             else if (mode == kSpecScale) { gn = cscale(gn, invR); gs = cscale(gs, invR); }
             if (m == 0) { gn.y = 0.0; gs.y = 0.0; }
             z = cadd(z, c2(gn.x - gs.y, gn.y + gs.x));
@@ -152,7 +152,7 @@ __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er
         for (int m = (b == 0 ? n : n - b); m <= L; m += n) {
             double2 e = Er[m * ms], o = Or[m * ms];
             double2 gn = cadd(e, o), gs = csub(e, o);
-            if (mode == kSpecDlam) { gn = dlam(gn, m, R); gs = dlam(gs, m, R); }
+            if (mode == kSpecDlam) { gn = dlam_q(gn, m * invR); gs = dlam_q(gs, m * invR); }
             else if (mode == kSpecScale) { gn = cscale(gn, invR); gs = cscale(gs, invR); }
             z = cadd(z, c2(gn.x + gs.y, -gn.y + gs.x));
         }
@@ -292,6 +292,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     const int jp = blockIdx.x, b = blockIdx.y;
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
+    const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
     const size_t ms = (size_t)nmem * 4;
     // LDS: rows [4][rs] | twiddles [n] | E, O columns [L+1][4] each
@@ -303,17 +304,18 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
         pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 4,
-                      (r & 1) ? kSpecScale : kSpecDlam, R, threadIdx.x, kGridThreads);
+                      (r & 1) ? kSpecScale : kSpecDlam, invR, threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 4, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
+    const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
         const double2 al = lds[e], as = lds[rs + e], bl = lds[2 * rs + e], bs = lds[3 * rs + e];
-        const double jn = jac_point(al.x, as.x, bl.x, bs.x, cN);
-        const double js = jac_point(al.y, as.y, bl.y, bs.y, cS);
+        const double jn = jac_point_q(al.x, as.x, bl.x, bs.x, qN);
+        const double js = jac_point_q(al.y, as.y, bl.y, bs.y, qS);
         lds[e] = c2(jn, js);
     }
     __syncthreads();
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     const int jp = blockIdx.x, b = blockIdx.y;
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
+    const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 10;
     const size_t ms = (size_t)nmem * 10;
     // grid row r <- (Legendre field, spectral factor)
@@ -348,13 +351,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     grid_prologue<10>(g, E, O, eo, ms, lds + 12 * rs, sE, sO);
     __syncthreads();
     for (int r = 0; r < 12; ++r) {
-        pack_spectrum(lds + r * rs, n, L, sE + src[r], sO + src[r], 10, mode[r], R,
+        pack_spectrum(lds + r * rs, n, L, sE + src[r], sO + src[r], 10, mode[r], invR,
                       threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 12, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
+    const double icN = 1.0 / cN, icS = 1.0 / cS;
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
         double gn[12], gs[12], on[4], os[4];
@@ -364,8 +368,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
             gn[r] = z.x;
             gs[r] = z.y;
         }
-        swe_point(gn, cN, on);
-        swe_point(gs, cS, os);
+        swe_point_ic(gn, icN, on);
+        swe_point_ic(gs, icS, os);
 #pragma unroll
         for (int r = 0; r < 4; ++r) lds[r * rs + e] = c2(on[r], os[r]);
     }
@@ -396,6 +400,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     const int jp = blockIdx.x, b = blockIdx.y;
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
+    const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
     const size_t ms = (size_t)nmem * 5;
     // LDS: rows [5][rs] | twiddles [n] | E, O columns [L+1][5] each
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     __syncthreads();
     for (int r = 0; r < 5; ++r) {
         const int mode = (r == 4) ? kSpecPlain : ((r & 1) ? kSpecScale : kSpecDlam);
-        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 5, mode, R, threadIdx.x, kGridThreads);
+        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 5, mode, invR, threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 5, g.fft, twl);

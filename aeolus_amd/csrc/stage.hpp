@@ -27,6 +27,9 @@ struct SpecK {
     const double* Cm;   // (L+1)^2
 };
 
+// (i q) * a for q = m / R given as a product m * (1/R)
+__device__ inline double2 dlam_q(double2 a, double q) { return c2(-q * a.y, q * a.x); }
+
 __device__ inline double2 dlam(double2 a, int m, double R) {
     // (i m / R) * a, as a complex product with (0, m/R)
     const double q = (double)m / R;
@@ -179,6 +182,11 @@ __device__ inline double jac_point(double al, double as, double bl, double bs, d
     return (as * bl - al * bs) / (c * c);
 }
 
+// The same with the reciprocal q = 1/c^2 of the row (one division per row).
+__device__ inline double jac_point_q(double al, double as, double bl, double bs, double q) {
+    return (as * bl - al * bs) * q;
+}
+
 // phy/shallow_water.py:377-386.  In: psi_l psi_s chi_l chi_s eta_l eta_s eta
 // phi_l phi_s phi zeta delta.  Out: zeta_dot, curl, kinetic, phi_dot.
 __device__ inline void swe_point(const double* g, double c, double* o) {
@@ -193,6 +201,22 @@ __device__ inline void swe_point(const double* g, double c, double* o) {
     o[1] = eta * zeta + (v * eta_l + u * eta_s) / c;
     o[2] = 0.5 * (u * u + v * v);
     const double adv_phi = (u * phi_l - v * phi_s) / c;
+    o[3] = -(adv_phi + phi * delta);
+}
+
+// swe_point with the row reciprocal ic = 1/c (multiplications only).
+__device__ inline void swe_point_ic(const double* g, double ic, double* o) {
+    const double psi_l = g[0], psi_s = g[1], chi_l = g[2], chi_s = g[3];
+    const double eta_l = g[4], eta_s = g[5], eta = g[6];
+    const double phi_l = g[7], phi_s = g[8], phi = g[9];
+    const double zeta = g[10], delta = g[11];
+    const double u = (psi_s + chi_l) * ic;
+    const double v = (psi_l - chi_s) * ic;
+    const double adv_eta = (u * eta_l - v * eta_s) * ic;
+    o[0] = -(adv_eta + eta * delta);
+    o[1] = eta * zeta + (v * eta_l + u * eta_s) * ic;
+    o[2] = 0.5 * (u * u + v * v);
+    const double adv_phi = (u * phi_l - v * phi_s) * ic;
     o[3] = -(adv_phi + phi * delta);
 }
 

@@ -16,7 +16,7 @@ Measured in the same run (``secondary``):
   * the uneven-L ensemble: 64 members each at L = 21/42/85/170 on
     32x64/64x128/128x256/256x512, one hipGraph replay per two steps,
     autotuned tiles, FP64 and split, with the one-day error;
-  * the adjoint: L = 170, 48 steps, split transforms; cost of the
+  * the adjoint: L = 170, 48 steps, split transforms (7 slices); cost of the
     checkpointed forward + reverse sweep relative to the plain forward.
 
 Inputs are resident in HBM before every timed region; a "step" is one full
@@ -360,7 +360,7 @@ def main():
         ns_split = best if best else 6
         secondary["ensemble_256_members"] = {_mode_name(k): v for k, v in
                                              ensemble_modes(48, ns_split).items()}
-        secondary["adjoint_L170"] = adjoint_cost(h["model"], ns_split)
+        secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
         "value": 1000.0 / h["ms_step"],
