@@ -30,8 +30,8 @@ constexpr int kStageWaves = kStageThreads / 64;
 // first PC chunks of 64), fewer for wide member/field tiles
 template <int MB, int NA, int NF>
 struct Prefetch {
-    static constexpr int PA = (MB * NA <= 2) ? 16 : 8;
-    static constexpr int PC = (MB * NA <= 2) ? 2 : 1;
+    static constexpr int PA = (MB == 1 && NA <= 2) ? 16 : 8;
+    static constexpr int PC = (MB == 1 && NA <= 2) ? 2 : 1;
 };
 
 // One step of the Pbar recurrence: a (x p1 - b p2) with (a, b) = ab.  The
@@ -428,7 +428,8 @@ struct StageLds {
                             sizeof(int) * 2 * NTA * 16 + 2 * sizeof(__bf16) * NS * NTA * 16 * rup(nh, 32) + 64;
         }
         if (syn) {
-            c = sizeof(double2) * ((size_t)MB * KS::NF * n + 2 * 2 * 2 * (size_t)MB * KS::NF * 64);
+            c = sizeof(double2) * ((size_t)MB * KS::NF * n +
+                                   (MB > 1 ? 0 : 2 * 2 * 2 * (size_t)MB * KS::NF * 64));
             if (NS > 0) c = sizeof(double2) * ((size_t)MB * KS::NF * n) + sizeof(int) * NTS * 16 +
                             2 * sizeof(__bf16) * NS * NTS * 16 * rup(split_kp(L, 0), 32) + 64;
         }
@@ -511,7 +512,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     // and the table rows at the start of the lane's degree block
     double sx = 0.0, sp0 = 0.0, sp1 = 0.0;
     if constexpr (SYN && NS == 0) {
-        const int kb = wave & 3, jp = (wave >> 2) * 64 + lane;
+        const int kb = (MB > 1) ? (lane >> 4) : (wave & 3);
+        const int jp = (MB > 1) ? (wave * 16 + (lane & 15)) : ((wave >> 2) * 64 + lane);
         const int Kb = ((K + 3) / 4 + 1) & ~1;
         const int i0 = kb * Kb;
         if (jp < nh && i0 < K) {
@@ -863,27 +865,34 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         }
     }
     if constexpr (SYN && NS == 0) {
-        // FP64 synthesis, every latitude of column m in this workgroup.  Wave
-        // w takes degree block w % 4 (contiguous, even length) for the 64
-        // latitudes of group w / 4; Pbar along the block comes from the
+        // FP64 synthesis, every latitude of column m in this workgroup, 128
+        // latitudes per pass.  Each latitude's degrees are split into 4
+        // contiguous blocks of even length; Pbar along a block comes from the
         // recurrence (rec_step) seeded by two table rows, so the table is read
-        // twice per block instead of once per degree, and all lanes of a wave
-        // read the same column entry (LDS broadcast).  The four block sums
-        // of each latitude meet in a two-round LDS tree.
-        static_assert(kStageWaves == 8, "synthesis maps 8 waves to 4 blocks x 2 latitude groups");
+        // twice per block instead of once per degree.
+        //   MB == 1: wave w takes block w % 4 of latitude group w / 4 (64
+        //            lanes = 64 latitudes; all lanes read the same column entry)
+        //            and the four block sums meet in a two-round LDS tree.
+        //   MB > 1:  lane = (16 latitudes, 4 blocks), waves = 16-latitude
+        //            groups; the block sums meet by lane shuffles, so no LDS
+        //            beyond the columns (more workgroups per CU for batches).
+        static_assert(kStageWaves == 8, "synthesis maps 8 waves to 128 latitudes");
+        constexpr bool kLaneBlocks = (MB > 1);
         double2* cols = work;                               // [MB][NF][K]
         double2* red = cols + (size_t)MB * NF * K;          // [2 lg][2][2 parity][MB][NF][64]
         for (int bb = 0; bb < MB; ++bb)
             derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
         constexpr int kRecG = (MB * NF <= 4) ? 4 : 2;      // degrees per recurrence group
-        const int kb = wave & 3, lg = wave >> 2;
+        const int kb = kLaneBlocks ? (lane >> 4) : (wave & 3);
+        const int lg = wave >> 2;
+        const int lat0 = kLaneBlocks ? (wave * 16 + (lane & 15)) : (lg * 64 + lane);
         const int Kb = ((K + 3) / 4 + 1) & ~1;              // even block length
         const int i0 = kb * Kb, i1 = (i0 + Kb < K) ? i0 + Kb : K;
         auto slot = [&](int s2, int par, int bb, int f) -> double2& {
             return red[((((lg * 2 + s2) * 2 + par) * MB + bb) * NF + f) * 64 + lane];
         };
         for (int jbase = 0; jbase < nh; jbase += 128) {
-            const int jp = jbase + lg * 64 + lane;
+            const int jp = jbase + lat0;
             const bool ok = jp < nh;
             double2 accE[MB][NF], accO[MB][NF];
 #pragma unroll
@@ -937,6 +946,36 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                             }
                     }
                 }
+            }
+            if constexpr (kLaneBlocks) {
+#pragma unroll
+                for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) {
+                        // (b0 + b2) + (b1 + b3): the association of the LDS tree,
+                        // so batched and single-member results agree bitwise
+                        double2 e = accE[bb][f], o = accO[bb][f];
+                        e.x += __shfl_xor(e.x, 32); e.y += __shfl_xor(e.y, 32);
+                        o.x += __shfl_xor(o.x, 32); o.y += __shfl_xor(o.y, 32);
+                        e.x += __shfl_xor(e.x, 16); e.y += __shfl_xor(e.y, 16);
+                        o.x += __shfl_xor(o.x, 16); o.y += __shfl_xor(o.y, 16);
+                        accE[bb][f] = e;
+                        accO[bb][f] = o;
+                    }
+                if (kb == 0 && ok) {
+#pragma unroll
+                    for (int bb = 0; bb < MB; ++bb) {
+                        const int b = b0 + bb;
+                        if (b >= a.nmem) continue;
+#pragma unroll
+                        for (int f = 0; f < NF; ++f) {
+                            const size_t o = (((size_t)jp * n + m) * a.nmem + b) * NF + f;
+                            a.E[o] = accE[bb][f];
+                            a.O[o] = accO[bb][f];
+                        }
+                    }
+                }
+                continue;
             }
             // round 1: blocks 2, 3 -> blocks 0, 1
             if (kb >= 2) {

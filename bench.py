@@ -129,22 +129,28 @@ def _profiled(model, cls, run, steps):
 
 
 def _stage_bytes(cls, L, nlat, nslices):
-    """Algorithmic HBM bytes of one launch of a BVE stage kernel class
-    (tables read once, intermediates written/read once, state passes)."""
+    """Algorithmic HBM bytes of one launch of a BVE stage kernel class:
+    tables and intermediates read or written once, state passes.  FP64
+    synthesis reads no table rows beyond the recurrence seeds (2 rows per
+    degree block, 4 blocks) and the (a, b) recurrence and C+/C- columns;
+    split-precision synthesis reads its digit table (2 nslices bytes/entry)."""
     nh = (nlat + 1) // 2
+    n = L + 1
     B = (L + 1) * (L + 2) // 2
     cut = (2 * L) // 3
     Bc = (cut + 1) * (cut + 2) // 2
-    spec = (L + 1) * (L + 1) * 16
+    spec = n * n * 16
     tab = 8 if nslices == 0 else 2 * nslices      # bytes per table entry
-    eo = 2 * 4 * nh * (L + 1) * 16                # E/O, 4 fields
+    eo = 2 * 4 * nh * n * 16                      # E/O, 4 fields
     sd = 2 * 1 * nh * (cut + 1) * 16              # S/D, 1 field, m <= cut
+    cols = n * n * (16 + 8 + 8)                   # (a, b), C+, C- columns
+    syn = nh * B * tab if nslices else 4 * 2 * nh * n * 8
     if cls == 0:
-        return nh * B * tab + spec + eo
+        return syn + cols + spec + eo
     if cls == 1:
         return eo + sd
-    # analysis table (truncated) + S/D + ys, y0, ksum r/w, ynext + synthesis table + E/O
-    return nh * Bc * tab + sd + 5 * spec + nh * B * tab + eo
+    # analysis table (truncated) + S/D + ys, y0, ksum r/w, ynext + synthesis + E/O
+    return nh * Bc * tab + sd + 5 * spec + syn + cols + eo
 
 
 def _stage_flops(cls, L, nlat):
