@@ -94,13 +94,13 @@ __global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
             }
         const int e = exp_above(mx);
         esyn[((size_t)m * 2 + p) * nhp + r] = e;
-        __bf16* dst = Psl + g.offs_ss[m];
+        unsigned short* dst = reinterpret_cast<unsigned short*>(Psl + g.offs_ss[m]);
         for (int ip = 0; ip < kpad; ++ip) {
             const int i = 2 * ip + p;
             const double v = (r < nh && ip < kp && i < K) ? P[(size_t)i * nh] : 0.0;
             digits_rt(v, e, ns, d);
             for (int s = 0; s < ns; ++s)
-                dst[((size_t)(p * ns + s) * nhp + r) * kpad + ip] = (__bf16)d[s];
+                dst[((size_t)(p * ns + s) * nhp + r) * kpad + ip] = bf16_bits(d[s]);
         }
     }
     if (r < mpad) {
@@ -112,12 +112,12 @@ __global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
             for (int j = 0; j < nh; ++j) mx = fmax(mx, fabs(P[(size_t)j * K]));
         const int e = exp_above(mx);
         eana[g.offs_ea[m] + (size_t)p * mpad + r] = e;
-        __bf16* dst = Pasl + g.offs_sa[m];
+        unsigned short* dst = reinterpret_cast<unsigned short*>(Pasl + g.offs_sa[m]);
         for (int j = 0; j < nhk; ++j) {
             const double v = (valid && j < nh) ? P[(size_t)j * K] : 0.0;
             digits_rt(v, e, ns, d);
             for (int s = 0; s < ns; ++s)
-                dst[((size_t)(p * ns + s) * mpad + r) * nhk + j] = (__bf16)d[s];
+                dst[((size_t)(p * ns + s) * mpad + r) * nhk + j] = bf16_bits(d[s]);
         }
     }
 }

@@ -458,11 +458,6 @@ __device__ inline void col_max_exp(int ncol, int len, int* ex, const F& val) {
     }
 }
 
-// 16-byte aligned pointer at or after p
-template <class T>
-__device__ inline T* align16(void* p) {
-    return reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
-}
 
 template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
 __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, StageArgs a,
@@ -508,6 +503,77 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             }
     }
 
+    // Split precision: per-m table offsets, and each wave's first analysis and
+    // synthesis tile: its digit rows (first K-chunks) and row exponents.  The
+    // loads are unconditional (waves without a tile read tile 0) so the
+    // compiler counts them exactly and no earlier wait has to drain them.
+    constexpr int NSX = (NS > 0) ? NS : 1;
+    constexpr int kNTA = StageLds<KS, Epi, MB, NS>::NTA, kNTS = StageLds<KS, Epi, MB, NS>::NTS;
+    constexpr int kKCA = (NS * kNTA <= 6) ? 4 : ((NS * kNTA <= 14) ? 2 : 1);
+    constexpr int kKCS = (NS * kNTS <= 6) ? 3 : ((NS * kNTS <= 14) ? 2 : 1);
+    constexpr int kPFA = (NS * kNTA <= 6) ? ((2 < kKCA) ? 2 : kKCA) : 1;   // chunks prefetched
+    constexpr int kPFS = (NS * kNTS <= 6) ? kKCS : 1;
+    bf16x8 avA[kPFA][NSX];
+    int erA[4] = {0, 0, 0, 0};
+    bf16x8 avS[kPFS][NSX];
+    int erS[4] = {0, 0, 0, 0};
+    long long oSA = 0, oEA = 0, oSS = 0;
+    if constexpr (NS > 0) {
+        if constexpr (ANA) {
+            oSA = g.offs_sa[m];
+            oEA = g.offs_ea[m];
+        }
+        if constexpr (SYN) oSS = g.offs_ss[m];
+    }
+    // analysis: issued after the prologue's own loads, before their LDS stores
+    auto prefetch_ana = [&]() {
+        if constexpr (ANA && NS > 0) {
+            const int nhk = rup(nh, 32);
+            const int kp = split_kp(L, m), mpad = rup(kp, 16);
+            const int Mv0 = (top >= m) ? (top - m) / 2 + 1 : 0;
+            const int Mv1 = (top >= m + 1) ? (top - m - 1) / 2 + 1 : 0;
+            const int MT0 = (Mv0 + 15) / 16, MT1 = (Mv1 + 15) / 16;
+            const bool has = wave < MT0 + MT1;
+            const int p = (has && wave >= MT0) ? 1 : 0;
+            const int mt = has ? wave - p * MT0 : 0;
+            const __bf16* Pt = g.Pasl + oSA;
+#pragma unroll
+            for (int c4 = 0; c4 < kPFA; ++c4) {
+                const int k0 = (32 * c4 < nhk) ? 32 * c4 : 0;
+#pragma unroll
+                for (int q = 0; q < NSX; ++q)
+                    avA[c4][q] = *reinterpret_cast<const bf16x8*>(
+                        Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) * nhk + k0 +
+                        (lane >> 4) * 8);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                erA[r] = g.eana[oEA + p * mpad + mt * 16 + (lane >> 4) * 4 + r];
+        }
+    };
+    // synthesis: issued with the prologue loads for synthesis-only launches,
+    // else after the epilogue (whose global loads it would otherwise delay)
+    auto prefetch_syn = [&]() {
+        if constexpr (SYN && NS > 0) {
+            const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
+            const int tt0 = (wave < 2 * (nhp / 16)) ? wave : 0;
+            const int p = tt0 & 1, j0 = (tt0 >> 1) * 16;
+            const __bf16* Pt = g.Psl + oSS;
+#pragma unroll
+            for (int c4 = 0; c4 < kPFS; ++c4) {
+                const int k0 = (32 * c4 < kpad) ? 32 * c4 : 0;
+#pragma unroll
+                for (int q = 0; q < NSX; ++q)
+                    avS[c4][q] = *reinterpret_cast<const bf16x8*>(
+                        Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
+                        (lane >> 4) * 8);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                erS[r] = g.esyn[(size_t)m * 2 * nhp + p * nhp + j0 + (lane >> 4) * 4 + r];
+        }
+    };
+
     // FP64 synthesis seeds of this lane for the first latitude pass: node x
     // and the table rows at the start of the lane's degree block
     double sx = 0.0, sp0 = 0.0, sp1 = 0.0;
@@ -530,7 +596,9 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         const int tot_x = active ? MB * NA * nh : 0;
         const int tot = (tot_s > tot_x ? tot_s : tot_x) > n ? (tot_s > tot_x ? tot_s : tot_x) : n;
         // loads are unconditional where possible (indices clamped in range) so
-        // the values stay in registers; only the LDS stores are predicated
+        // the values stay in registers; only the LDS stores are predicated.
+        // The first pass also issues the split tile prefetch between its
+        // loads and its stores (peeled, so the prefetch stays behind them).
         for (int t0 = 0; t0 < tot; t0 += 2 * kStageThreads) {
             double2 vs[2], vx[2][2], vr[2];
             double vc[2][3];
@@ -569,7 +637,12 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 vc[u][0] = a.k.Cp[(size_t)tc * n + m];
                 vc[u][1] = a.k.Cm[(size_t)tc * n + m];
                 vc[u][2] = a.k.lap ? a.k.lap[tc] : 0.0;
-                vr[u] = g.rec[(size_t)m * n + tc];
+                vr[u] = c2(0.0, 0.0);
+                if constexpr (SYN && NS == 0) vr[u] = g.rec[(size_t)m * n + tc];   // FP64 recurrence
+            }
+            if (t0 == 0) {
+                if constexpr (ANA && NS > 0) prefetch_ana();
+                if constexpr (SYN && NS > 0 && !ANA) prefetch_syn();
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -582,11 +655,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     X[((bb * NA + f) * 2 + 0) * nh + jp] = vx[u][0];
                     X[((bb * NA + f) * 2 + 1) * nh + jp] = vx[u][1];
                 }
-                if (t < n) {
-                    cpc[t] = vc[u][0];
-                    cmc[t] = vc[u][1];
-                    lapc[t] = vc[u][2];
-                    rcc[t] = vr[u];
+                // u = 0: unconditional at the clamped index (threads past n
+                // rewrite entry n - 1 with its own value), so the loads are
+                // not sunk behind later ones
+                if (u == 0 || t < n) {
+                    const int tc = (t < n) ? t : n - 1;
+                    cpc[tc] = vc[u][0];
+                    cmc[tc] = vc[u][1];
+                    lapc[tc] = vc[u][2];
+                    if constexpr (SYN && NS == 0) rcc[tc] = vr[u];
                 }
             }
         }
@@ -600,12 +677,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         if constexpr (NS > 0) if (active) {
             // ---- split-precision analysis: BF16 digit MFMA, exact FP32, FP64 recombination
             constexpr int NCA = StageLds<KS, Epi, MB, NS>::NCA;
-            constexpr int NTA = StageLds<KS, Epi, MB, NS>::NTA;
+            constexpr int NTA = kNTA;
             constexpr int NCP = NTA * 16;
             const int nhk = rup(nh, 32);
             const int kp = split_kp(L, m), mpad = rup(kp, 16);
             int* gexp = reinterpret_cast<int*>(accl + (size_t)MB * NA * n);   // [2][NCP]
-            __bf16* xs = align16<__bf16>(gexp + 2 * NCP);                      // [2][NS][NCP][nhk]
+            // 2 NCP ints after a 16-byte boundary keep xs 16-byte aligned; a plain
+            // cast (no integer round trip) keeps the LDS address space
+            __bf16* xs = reinterpret_cast<__bf16*>(gexp + 2 * NCP);            // [2][NS][NCP][nhk]
+            unsigned short* xsb = reinterpret_cast<unsigned short*>(xs);
             col_max_exp(2 * NCP, nh, gexp, [&](int t, int j) {
                 const int p = t / NCP, c = t - p * NCP;
                 if (c >= NCA) return 0.0;
@@ -613,10 +693,10 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 return cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
             });
             __syncthreads();
-            for (int t = threadIdx.x; t < 2 * NCP * nhk; t += kStageThreads) {
-                const int p = t / (NCP * nhk);
-                const int r2 = t - p * NCP * nhk;
-                const int c = r2 / nhk, j = r2 - c * nhk;
+            // rows (p, c) over waves, latitudes over lanes: no runtime divisions
+            for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
+            for (int j = lane; j < nhk; j += 64) {
+                const int p = rr / NCP, c = rr - p * NCP;
                 double v = 0.0;
                 if (c < NCA && j < nh) {
                     const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
@@ -626,11 +706,11 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 digits<NS>(v, gexp[p * NCP + c], d);
 #pragma unroll
                 for (int q = 0; q < NS; ++q)
-                    xs[((size_t)(p * NS + q) * NCP + c) * nhk + j] = (__bf16)d[q];
+                    xsb[((size_t)(p * NS + q) * NCP + c) * nhk + j] = bf16_bits(d[q]);
             }
             __syncthreads();
-            const __bf16* Pt = g.Pasl + g.offs_sa[m];
-            const int* er = g.eana + g.offs_ea[m];
+            const __bf16* Pt = g.Pasl + oSA;
+            const int* er = g.eana + oEA;
             double* accd = reinterpret_cast<double*>(accl);
             // 16-degree tiles of both parities spread over all waves
             const int Mv0 = (top >= m) ? (top - m) / 2 + 1 : 0;
@@ -646,25 +726,51 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     for (int q = 0; q < NS; ++q)
 #pragma unroll
                         for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-                    for (int k0 = 0; k0 < nhk; k0 += 32) {
-                        bf16x8 av[NS];
+                    // the table digits of kKC K-chunks are loaded before their MFMAs
+                    // (the wave's first tile: in the prologue)
+                    constexpr int kKC = kKCA;
+                    for (int kg = 0; kg < nhk; kg += 32 * kKC) {
+                        bf16x8 av[kKC][NS];
+                        if (tt == wave && kg == 0) {
 #pragma unroll
-                        for (int q = 0; q < NS; ++q)
-                            av[q] = *reinterpret_cast<const bf16x8*>(
-                                Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) * nhk + k0 +
-                                (lane >> 4) * 8);
+                            for (int c4 = 0; c4 < kKC; ++c4) {
+                                const int k0 = (32 * c4 < nhk) ? 32 * c4 : 0;
 #pragma unroll
-                        for (int nt = 0; nt < NTA; ++nt)
-#pragma unroll
-                            for (int t2 = 0; t2 < NS; ++t2) {
-                                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
-                                    xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * nhk +
-                                    k0 + (lane >> 4) * 8);
-#pragma unroll
-                                for (int q = 0; q + t2 < NS; ++q)
-                                    acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                        av[q], bv, acc[q + t2][nt], 0, 0, 0);
+                                for (int q = 0; q < NS; ++q)
+                                    av[c4][q] = (c4 < kPFA) ? avA[c4 < kPFA ? c4 : 0][q]
+                                        : *reinterpret_cast<const bf16x8*>(
+                                              Pt + ((size_t)(p * NS + q) * mpad + mt * 16 +
+                                                    (lane & 15)) * nhk + k0 + (lane >> 4) * 8);
                             }
+                        } else {
+#pragma unroll
+                            for (int c4 = 0; c4 < kKC; ++c4) {
+                                const int k0 = (kg + 32 * c4 < nhk) ? kg + 32 * c4 : kg;
+#pragma unroll
+                                for (int q = 0; q < NS; ++q)
+                                    av[c4][q] = *reinterpret_cast<const bf16x8*>(
+                                        Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) *
+                                                 nhk + k0 + (lane >> 4) * 8);
+                            }
+                        }
+#pragma unroll
+                        for (int c4 = 0; c4 < kKC; ++c4) {
+                            const int k0 = kg + 32 * c4;
+                            if (k0 < nhk) {
+#pragma unroll
+                                for (int nt = 0; nt < NTA; ++nt)
+#pragma unroll
+                                    for (int t2 = 0; t2 < NS; ++t2) {
+                                        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                            xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
+                                                     nhk + k0 + (lane >> 4) * 8);
+#pragma unroll
+                                        for (int q = 0; q + t2 < NS; ++q)
+                                            acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
+                                    }
+                            }
+                        }
                     }
 #pragma unroll
                     for (int nt = 0; nt < NTA; ++nt) {
@@ -678,8 +784,9 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                             if (c < NCA && ir < Mv) {
                                 const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
                                 const int l = m + 2 * ir + p;
+                                const int e = (tt == wave) ? erA[r] : er[p * mpad + ir];
                                 accd[2 * ((size_t)(bb * NA + f) * n + l) + ri] =
-                                    ldexp(v, er[p * mpad + ir] + gexp[p * NCP + c] - 14);
+                                    ldexp(v, e + gexp[p * NCP + c] - 14);
                             }
                         }
                     }
@@ -784,33 +891,34 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 
     if constexpr (SYN && NS > 0) {
         // ---- split-precision synthesis: BF16 digit MFMA, exact FP32, FP64 recombination
+        if constexpr (ANA) prefetch_syn();
         constexpr int NCS = StageLds<KS, Epi, MB, NS>::NCS;
-        constexpr int NTS = StageLds<KS, Epi, MB, NS>::NTS;
+        constexpr int NTS = kNTS;
         constexpr int NCP = NTS * 16;
         double2* cols = work;                               // [MB][NF][K]
         for (int bb = 0; bb < MB; ++bb)
             derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
         const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
         int* fexp = reinterpret_cast<int*>(cols + (size_t)MB * NF * n);       // [NCP]
-        __bf16* qs = align16<__bf16>(fexp + NCP);                             // [2][NS][NCP][kpad]
+        __bf16* qs = reinterpret_cast<__bf16*>(fexp + NCP);                  // [2][NS][NCP][kpad]
+        unsigned short* qsb = reinterpret_cast<unsigned short*>(qs);
         col_max_exp(NCP, K, fexp, [&](int c, int i) {
             return (c < NCS) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
         });
         __syncthreads();
-        for (int t = threadIdx.x; t < 2 * NCP * kpad; t += kStageThreads) {
-            const int p = t / (NCP * kpad);
-            const int r2 = t - p * NCP * kpad;
-            const int c = r2 / kpad, ip = r2 - c * kpad;
+        for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
+        for (int ip = lane; ip < kpad; ip += 64) {
+            const int p = rr / NCP, c = rr - p * NCP;
             const int i = 2 * ip + p;
             const double v = (c < NCS && i < K) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
             float d[NS];
             digits<NS>(v, fexp[c], d);
 #pragma unroll
-            for (int q = 0; q < NS; ++q) qs[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = (__bf16)d[q];
+            for (int q = 0; q < NS; ++q) qsb[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = bf16_bits(d[q]);
         }
         __syncthreads();
         // (16-latitude tile, parity) pairs spread over the waves
-        const __bf16* Pt = g.Psl + g.offs_ss[m];
+        const __bf16* Pt = g.Psl + oSS;
         const int* er = g.esyn + (size_t)m * 2 * nhp;
         for (int tt = wave; tt < 2 * (nhp / 16); tt += kStageWaves) {
             const int p = tt & 1;
@@ -821,25 +929,51 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 for (int q = 0; q < NS; ++q)
 #pragma unroll
                     for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-                for (int k0 = 0; k0 < kpad; k0 += 32) {
-                    bf16x8 av[NS];
+                // the table digits of kKC K-chunks are loaded before their MFMAs
+                // (the wave's first tile: before the epilogue)
+                constexpr int kKC = kKCS;
+                for (int kg = 0; kg < kpad; kg += 32 * kKC) {
+                    bf16x8 av[kKC][NS];
+                    if (tt == wave && kg == 0) {
 #pragma unroll
-                    for (int q = 0; q < NS; ++q)
-                        av[q] = *reinterpret_cast<const bf16x8*>(
-                            Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
-                            (lane >> 4) * 8);
+                        for (int c4 = 0; c4 < kKC; ++c4) {
+                            const int k0 = (32 * c4 < kpad) ? 32 * c4 : 0;
 #pragma unroll
-                    for (int nt = 0; nt < NTS; ++nt)
-#pragma unroll
-                        for (int t2 = 0; t2 < NS; ++t2) {
-                            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
-                                qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * kpad + k0 +
-                                (lane >> 4) * 8);
-#pragma unroll
-                            for (int q = 0; q + t2 < NS; ++q)
-                                acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    av[q], bv, acc[q + t2][nt], 0, 0, 0);
+                            for (int q = 0; q < NS; ++q)
+                                av[c4][q] = (c4 < kPFS) ? avS[c4 < kPFS ? c4 : 0][q]
+                                    : *reinterpret_cast<const bf16x8*>(
+                                          Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) *
+                                                   kpad + k0 + (lane >> 4) * 8);
                         }
+                    } else {
+#pragma unroll
+                        for (int c4 = 0; c4 < kKC; ++c4) {
+                            const int k0 = (kg + 32 * c4 < kpad) ? kg + 32 * c4 : kg;
+#pragma unroll
+                            for (int q = 0; q < NS; ++q)
+                                av[c4][q] = *reinterpret_cast<const bf16x8*>(
+                                    Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad +
+                                    k0 + (lane >> 4) * 8);
+                        }
+                    }
+#pragma unroll
+                    for (int c4 = 0; c4 < kKC; ++c4) {
+                        const int k0 = kg + 32 * c4;
+                        if (k0 < kpad) {
+#pragma unroll
+                            for (int nt = 0; nt < NTS; ++nt)
+#pragma unroll
+                                for (int t2 = 0; t2 < NS; ++t2) {
+                                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                        qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
+                                                 kpad + k0 + (lane >> 4) * 8);
+#pragma unroll
+                                    for (int q = 0; q + t2 < NS; ++q)
+                                        acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                            av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
+                                }
+                        }
+                    }
                 }
                 double2* dst = (p == 0) ? a.E : a.O;
 #pragma unroll
@@ -851,7 +985,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                         double v = 0.0;
 #pragma unroll
                         for (int q = NS - 1; q >= 0; --q) v = v * (1.0 / 256.0) + (double)acc[q][nt][r];
-                        v = ldexp(v, er[p * nhp + j] + fexp[c] - 14);
+                        v = ldexp(v, ((tt == wave) ? erS[r] : er[p * nhp + j]) + fexp[c] - 14);
                         const double w = __shfl_xor(v, 1);
                         if (!(c & 1) && c < NCS && j < nh) {
                             const int bb = c / (NF * 2), f = (c >> 1) % NF;

@@ -56,6 +56,12 @@ __device__ inline void digits_rt(double v, int e, int ns, float* d) {
     }
 }
 
+// BF16 bit pattern of a digit: an integer of magnitude <= 256 is exact in
+// FP32 and in BF16, whose bits are the upper half of the FP32 bits.
+__device__ inline unsigned short bf16_bits(float d) {
+    return (unsigned short)(__float_as_uint(d) >> 16);
+}
+
 // Largest number of FP32-exact accumulations: (nslices) * K * 2^14 < 2^24.
 inline bool split_exact(int nslices, int kmax) {
     return (long long)nslices * kmax * 16384LL < (1LL << 24);
