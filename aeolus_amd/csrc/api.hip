@@ -282,7 +282,7 @@ int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
     const int L = plan->L, nh = plan->nh;
     AEO_REQUIRE(split_exact(nslices, split_kp(L, 0)) && split_exact(nslices, nh),
                 AEO_ERR_UNSUPPORTED,
-                "aeo_plan_enable_split: FP32 accumulation would not be exact at this size");
+                "aeo_plan_enable_split: INT32 accumulation would not be exact at this size");
     if (plan->nslices == nslices) return AEO_OK;
     for (void* q : {(void*)plan->d_Psl, (void*)plan->d_offs_ss, (void*)plan->d_esyn,
                     (void*)plan->d_Pasl, (void*)plan->d_offs_sa, (void*)plan->d_eana,
@@ -292,11 +292,11 @@ int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
     plan->d_Pasl = nullptr; plan->d_offs_sa = nullptr; plan->d_eana = nullptr;
     plan->d_offs_ea = nullptr;
     plan->nslices = 0;
-    const int nhp = rup(nh, 16), nhk = rup(nh, 32);
+    const int nhp = rup(nh, 16), nhk = split_nhk(nh);
     std::vector<long long> oss(L + 2, 0), osa(L + 2, 0), oea(L + 2, 0);
     for (int m = 0; m <= L; ++m) {
         const int kp = split_kp(L, m);
-        oss[m + 1] = oss[m] + 2LL * nslices * nhp * rup(kp, 32);
+        oss[m + 1] = oss[m] + 2LL * nslices * nhp * split_kpad(L, m);
         osa[m + 1] = osa[m] + 2LL * nslices * rup(kp, 16) * nhk;
         oea[m + 1] = oea[m] + 2LL * rup(kp, 16);
     }

@@ -75,15 +75,15 @@ __global__ void legendre_table_kernel(GaussDev g, double* __restrict__ Ps,
 // Split-precision digit tables (see split.hpp).  grid (L+1, 2 parities,
 // row tiles of 64); one thread per table row.
 // ---------------------------------------------------------------------------
-__global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
-                                   int* __restrict__ esyn, __bf16* __restrict__ Pasl,
+__global__ void split_table_kernel(GaussDev g, int ns, signed char* __restrict__ Psl,
+                                   int* __restrict__ esyn, signed char* __restrict__ Pasl,
                                    int* __restrict__ eana, int nhp, int nhk) {
     const int m = blockIdx.x, p = blockIdx.y;
     const int r = blockIdx.z * blockDim.x + threadIdx.x;
     const int L = g.L, nh = g.nh;
     const int K = L - m + 1;
-    const int kp = split_kp(L, m), kpad = rup(kp, 32), mpad = rup(kp, 16);
-    float d[8];
+    const int kp = split_kp(L, m), kpad = split_kpad(L, m), mpad = rup(kp, 16);
+    int d[8];
     if (r < nhp) {
         const double* P = g.Ps + g.offs[m] + r;
         double mx = 0.0;
@@ -94,13 +94,13 @@ __global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
             }
         const int e = exp_above(mx);
         esyn[((size_t)m * 2 + p) * nhp + r] = e;
-        unsigned short* dst = reinterpret_cast<unsigned short*>(Psl + g.offs_ss[m]);
+        signed char* dst = Psl + g.offs_ss[m];
         for (int ip = 0; ip < kpad; ++ip) {
             const int i = 2 * ip + p;
             const double v = (r < nh && ip < kp && i < K) ? P[(size_t)i * nh] : 0.0;
             digits_rt(v, e, ns, d);
             for (int s = 0; s < ns; ++s)
-                dst[((size_t)(p * ns + s) * nhp + r) * kpad + ip] = bf16_bits(d[s]);
+                dst[((size_t)(p * ns + s) * nhp + r) * kpad + ip] = (signed char)d[s];
         }
     }
     if (r < mpad) {
@@ -112,12 +112,12 @@ __global__ void split_table_kernel(GaussDev g, int ns, __bf16* __restrict__ Psl,
             for (int j = 0; j < nh; ++j) mx = fmax(mx, fabs(P[(size_t)j * K]));
         const int e = exp_above(mx);
         eana[g.offs_ea[m] + (size_t)p * mpad + r] = e;
-        unsigned short* dst = reinterpret_cast<unsigned short*>(Pasl + g.offs_sa[m]);
+        signed char* dst = Pasl + g.offs_sa[m];
         for (int j = 0; j < nhk; ++j) {
             const double v = (valid && j < nh) ? P[(size_t)j * K] : 0.0;
             digits_rt(v, e, ns, d);
             for (int s = 0; s < ns; ++s)
-                dst[((size_t)(p * ns + s) * mpad + r) * nhk + j] = bf16_bits(d[s]);
+                dst[((size_t)(p * ns + s) * mpad + r) * nhk + j] = (signed char)d[s];
         }
     }
 }
@@ -513,7 +513,7 @@ int gauss_build_tables(aeo_plan_s* p, hipStream_t s) {
 }
 
 int gauss_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s) {
-    const int nhp = rup(p->nh, 16), nhk = rup(p->nh, 32);
+    const int nhp = rup(p->nh, 16), nhk = split_nhk(p->nh);
     const int rows = (nhp > rup(split_kp(p->L, 0), 16)) ? nhp : rup(split_kp(p->L, 0), 16);
     aeo::GaussDev g = p->gdev();
     dim3 grid(p->L + 1, 2, (rows + 63) / 64);

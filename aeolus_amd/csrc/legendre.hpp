@@ -425,13 +425,13 @@ struct StageLds {
             a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) +
                                    (size_t)kStageWaves * MB * Epi::NA * 64);
             if (NS > 0) a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n)) +
-                            sizeof(int) * 2 * NTA * 16 + 2 * sizeof(__bf16) * NS * NTA * 16 * rup(nh, 32) + 64;
+                            sizeof(int) * 2 * NTA * 16 + 2 * NS * NTA * 16 * split_nhk(nh) + 64;
         }
         if (syn) {
             c = sizeof(double2) * ((size_t)MB * KS::NF * n +
                                    (MB > 1 ? 0 : 2 * 2 * 2 * (size_t)MB * KS::NF * 64));
             if (NS > 0) c = sizeof(double2) * ((size_t)MB * KS::NF * n) + sizeof(int) * NTS * 16 +
-                            2 * sizeof(__bf16) * NS * NTS * 16 * rup(split_kp(L, 0), 32) + 64;
+                            2 * NS * NTS * 16 * split_kpad(L, 0) + 64;
         }
         return s + (a > c ? a : c);
     }
@@ -509,13 +509,13 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     // compiler counts them exactly and no earlier wait has to drain them.
     constexpr int NSX = (NS > 0) ? NS : 1;
     constexpr int kNTA = StageLds<KS, Epi, MB, NS>::NTA, kNTS = StageLds<KS, Epi, MB, NS>::NTS;
-    constexpr int kKCA = (NS * kNTA <= 6) ? 4 : ((NS * kNTA <= 14) ? 2 : 1);
-    constexpr int kKCS = (NS * kNTS <= 6) ? 3 : ((NS * kNTS <= 14) ? 2 : 1);
-    constexpr int kPFA = (NS * kNTA <= 6) ? ((2 < kKCA) ? 2 : kKCA) : 1;   // chunks prefetched
-    constexpr int kPFS = (NS * kNTS <= 6) ? kKCS : 1;
-    bf16x8 avA[kPFA][NSX];
+    // K-chunks (kSplitK deep) loaded per group, and prefetched
+    constexpr int kKCA = (NS * kNTA <= 7) ? 2 : 1;
+    constexpr int kKCS = (NS * kNTS <= 7) ? 2 : 1;
+    constexpr int kPFA = kKCA, kPFS = kKCS;
+    intx4 avA[kPFA][NSX];
     int erA[4] = {0, 0, 0, 0};
-    bf16x8 avS[kPFS][NSX];
+    intx4 avS[kPFS][NSX];
     int erS[4] = {0, 0, 0, 0};
     long long oSA = 0, oEA = 0, oSS = 0;
     if constexpr (NS > 0) {
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     // analysis: issued after the prologue's own loads, before their LDS stores
     auto prefetch_ana = [&]() {
         if constexpr (ANA && NS > 0) {
-            const int nhk = rup(nh, 32);
+            const int nhk = split_nhk(nh);
             const int kp = split_kp(L, m), mpad = rup(kp, 16);
             const int Mv0 = (top >= m) ? (top - m) / 2 + 1 : 0;
             const int Mv1 = (top >= m + 1) ? (top - m - 1) / 2 + 1 : 0;
@@ -536,15 +536,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             const bool has = wave < MT0 + MT1;
             const int p = (has && wave >= MT0) ? 1 : 0;
             const int mt = has ? wave - p * MT0 : 0;
-            const __bf16* Pt = g.Pasl + oSA;
+            const signed char* Pt = g.Pasl + oSA;
 #pragma unroll
             for (int c4 = 0; c4 < kPFA; ++c4) {
-                const int k0 = (32 * c4 < nhk) ? 32 * c4 : 0;
+                const int k0 = (kSplitK * c4 < nhk) ? kSplitK * c4 : 0;
 #pragma unroll
                 for (int q = 0; q < NSX; ++q)
-                    avA[c4][q] = *reinterpret_cast<const bf16x8*>(
+                    avA[c4][q] = *reinterpret_cast<const intx4*>(
                         Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) * nhk + k0 +
-                        (lane >> 4) * 8);
+                        (lane >> 4) * 16);
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -555,18 +555,18 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     // else after the epilogue (whose global loads it would otherwise delay)
     auto prefetch_syn = [&]() {
         if constexpr (SYN && NS > 0) {
-            const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
+            const int kpad = split_kpad(L, m), nhp = rup(nh, 16);
             const int tt0 = (wave < 2 * (nhp / 16)) ? wave : 0;
             const int p = tt0 & 1, j0 = (tt0 >> 1) * 16;
-            const __bf16* Pt = g.Psl + oSS;
+            const signed char* Pt = g.Psl + oSS;
 #pragma unroll
             for (int c4 = 0; c4 < kPFS; ++c4) {
-                const int k0 = (32 * c4 < kpad) ? 32 * c4 : 0;
+                const int k0 = (kSplitK * c4 < kpad) ? kSplitK * c4 : 0;
 #pragma unroll
                 for (int q = 0; q < NSX; ++q)
-                    avS[c4][q] = *reinterpret_cast<const bf16x8*>(
+                    avS[c4][q] = *reinterpret_cast<const intx4*>(
                         Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
-                        (lane >> 4) * 8);
+                        (lane >> 4) * 16);
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -675,17 +675,17 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     if constexpr (ANA) {
         double2* red = accl + (size_t)MB * NA * n;          // [waves][MB][NA][64]
         if constexpr (NS > 0) if (active) {
-            // ---- split-precision analysis: BF16 digit MFMA, exact FP32, FP64 recombination
+            // ---- split-precision analysis: INT8 digit MFMA, exact INT32, FP64 recombination
             constexpr int NCA = StageLds<KS, Epi, MB, NS>::NCA;
             constexpr int NTA = kNTA;
             constexpr int NCP = NTA * 16;
-            const int nhk = rup(nh, 32);
+            const int nhk = split_nhk(nh);
             const int kp = split_kp(L, m), mpad = rup(kp, 16);
             int* gexp = reinterpret_cast<int*>(accl + (size_t)MB * NA * n);   // [2][NCP]
             // 2 NCP ints after a 16-byte boundary keep xs 16-byte aligned; a plain
             // cast (no integer round trip) keeps the LDS address space
-            __bf16* xs = reinterpret_cast<__bf16*>(gexp + 2 * NCP);            // [2][NS][NCP][nhk]
-            unsigned short* xsb = reinterpret_cast<unsigned short*>(xs);
+            signed char* xs = reinterpret_cast<signed char*>(gexp + 2 * NCP);  // [2][NS][NCP][nhk]
+            unsigned short* xsh = reinterpret_cast<unsigned short*>(xs);
             col_max_exp(2 * NCP, nh, gexp, [&](int t, int j) {
                 const int p = t / NCP, c = t - p * NCP;
                 if (c >= NCA) return 0.0;
@@ -693,23 +693,27 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 return cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
             });
             __syncthreads();
-            // rows (p, c) over waves, latitudes over lanes: no runtime divisions
+            // rows (p, c) over waves, latitude pairs over lanes (one 16-bit
+            // store per slice): no runtime divisions
             for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
-            for (int j = lane; j < nhk; j += 64) {
+            for (int j = 2 * lane; j < nhk; j += 128) {
                 const int p = rr / NCP, c = rr - p * NCP;
-                double v = 0.0;
-                if (c < NCA && j < nh) {
+                double v0 = 0.0, v1 = 0.0;
+                if (c < NCA) {
                     const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                    v = cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
+                    const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
+                    if (j < nh) v0 = cpart(xr[j], ri);
+                    if (j + 1 < nh) v1 = cpart(xr[j + 1], ri);
                 }
-                float d[NS];
-                digits<NS>(v, gexp[p * NCP + c], d);
+                int d0[NS], d1[NS];
+                digits<NS>(v0, gexp[p * NCP + c], d0);
+                digits<NS>(v1, gexp[p * NCP + c], d1);
 #pragma unroll
                 for (int q = 0; q < NS; ++q)
-                    xsb[((size_t)(p * NS + q) * NCP + c) * nhk + j] = bf16_bits(d[q]);
+                    xsh[(((size_t)(p * NS + q) * NCP + c) * nhk + j) >> 1] = digit_pair(d0[q], d1[q]);
             }
             __syncthreads();
-            const __bf16* Pt = g.Pasl + oSA;
+            const signed char* Pt = g.Pasl + oSA;
             const int* er = g.eana + oEA;
             double* accd = reinterpret_cast<double*>(accl);
             // 16-degree tiles of both parities spread over all waves
@@ -721,52 +725,46 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 const int mt = (tt < MT0) ? tt : tt - MT0;
                 const int Mv = p ? Mv1 : Mv0;
                 {
-                    floatx4 acc[NS][NTA];
+                    intx4 acc[NS][NTA];
 #pragma unroll
                     for (int q = 0; q < NS; ++q)
 #pragma unroll
-                        for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                        for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
                     // the table digits of kKC K-chunks are loaded before their MFMAs
                     // (the wave's first tile: in the prologue)
                     constexpr int kKC = kKCA;
-                    for (int kg = 0; kg < nhk; kg += 32 * kKC) {
-                        bf16x8 av[kKC][NS];
+                    for (int kg = 0; kg < nhk; kg += kSplitK * kKC) {
+                        intx4 av[kKC][NS];
                         if (tt == wave && kg == 0) {
 #pragma unroll
-                            for (int c4 = 0; c4 < kKC; ++c4) {
-                                const int k0 = (32 * c4 < nhk) ? 32 * c4 : 0;
+                            for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
-                                for (int q = 0; q < NS; ++q)
-                                    av[c4][q] = (c4 < kPFA) ? avA[c4 < kPFA ? c4 : 0][q]
-                                        : *reinterpret_cast<const bf16x8*>(
-                                              Pt + ((size_t)(p * NS + q) * mpad + mt * 16 +
-                                                    (lane & 15)) * nhk + k0 + (lane >> 4) * 8);
-                            }
+                                for (int q = 0; q < NS; ++q) av[c4][q] = avA[c4][q];
                         } else {
 #pragma unroll
                             for (int c4 = 0; c4 < kKC; ++c4) {
-                                const int k0 = (kg + 32 * c4 < nhk) ? kg + 32 * c4 : kg;
+                                const int k0 = (kg + kSplitK * c4 < nhk) ? kg + kSplitK * c4 : kg;
 #pragma unroll
                                 for (int q = 0; q < NS; ++q)
-                                    av[c4][q] = *reinterpret_cast<const bf16x8*>(
+                                    av[c4][q] = *reinterpret_cast<const intx4*>(
                                         Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) *
-                                                 nhk + k0 + (lane >> 4) * 8);
+                                                 nhk + k0 + (lane >> 4) * 16);
                             }
                         }
 #pragma unroll
                         for (int c4 = 0; c4 < kKC; ++c4) {
-                            const int k0 = kg + 32 * c4;
+                            const int k0 = kg + kSplitK * c4;
                             if (k0 < nhk) {
 #pragma unroll
                                 for (int nt = 0; nt < NTA; ++nt)
 #pragma unroll
                                     for (int t2 = 0; t2 < NS; ++t2) {
-                                        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                        const intx4 bv = *reinterpret_cast<const intx4*>(
                                             xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
-                                                     nhk + k0 + (lane >> 4) * 8);
+                                                     nhk + k0 + (lane >> 4) * 16);
 #pragma unroll
                                         for (int q = 0; q + t2 < NS; ++q)
-                                            acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                            acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                                                 av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
                                     }
                             }
@@ -780,13 +778,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                             const int ir = mt * 16 + (lane >> 4) * 4 + r;
                             double v = 0.0;
 #pragma unroll
-                            for (int q = NS - 1; q >= 0; --q) v = v * (1.0 / 256.0) + (double)acc[q][nt][r];
+                            // explicit fma: the same rounding in every member tiling
+                            // explicit fma: the same rounding in every member tiling
+                        for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
                             if (c < NCA && ir < Mv) {
                                 const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
                                 const int l = m + 2 * ir + p;
                                 const int e = (tt == wave) ? erA[r] : er[p * mpad + ir];
                                 accd[2 * ((size_t)(bb * NA + f) * n + l) + ri] =
-                                    ldexp(v, e + gexp[p * NCP + c] - 14);
+                                    ldexp(v, e + gexp[p * NCP + c]) * kInvTop2;
                             }
                         }
                     }
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     }
 
     if constexpr (SYN && NS > 0) {
-        // ---- split-precision synthesis: BF16 digit MFMA, exact FP32, FP64 recombination
+        // ---- split-precision synthesis: INT8 digit MFMA, exact INT32, FP64 recombination
         if constexpr (ANA) prefetch_syn();
         constexpr int NCS = StageLds<KS, Epi, MB, NS>::NCS;
         constexpr int NTS = kNTS;
@@ -898,78 +898,80 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         double2* cols = work;                               // [MB][NF][K]
         for (int bb = 0; bb < MB; ++bb)
             derive_columns<KS>(st + (size_t)bb * NST * n, m, kc, cols + (size_t)bb * NF * K);
-        const int kp = split_kp(L, m), kpad = rup(kp, 32), nhp = rup(nh, 16);
+        const int kpad = split_kpad(L, m), nhp = rup(nh, 16);
         int* fexp = reinterpret_cast<int*>(cols + (size_t)MB * NF * n);       // [NCP]
-        __bf16* qs = reinterpret_cast<__bf16*>(fexp + NCP);                  // [2][NS][NCP][kpad]
-        unsigned short* qsb = reinterpret_cast<unsigned short*>(qs);
+        signed char* qs = reinterpret_cast<signed char*>(fexp + NCP);        // [2][NS][NCP][kpad]
+        unsigned short* qsh = reinterpret_cast<unsigned short*>(qs);
         col_max_exp(NCP, K, fexp, [&](int c, int i) {
             return (c < NCS) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
         });
         __syncthreads();
+        // rows (p, c) over waves, degree pairs (ip, ip + 1) over lanes
         for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
-        for (int ip = lane; ip < kpad; ip += 64) {
+        for (int ip = 2 * lane; ip < kpad; ip += 128) {
             const int p = rr / NCP, c = rr - p * NCP;
             const int i = 2 * ip + p;
-            const double v = (c < NCS && i < K) ? cpart(cols[(size_t)(c >> 1) * K + i], c & 1) : 0.0;
-            float d[NS];
-            digits<NS>(v, fexp[c], d);
+            double v0 = 0.0, v1 = 0.0;
+            if (c < NCS) {
+                const double2* cr = cols + (size_t)(c >> 1) * K;
+                if (i < K) v0 = cpart(cr[i], c & 1);
+                if (i + 2 < K) v1 = cpart(cr[i + 2], c & 1);
+            }
+            int d0[NS], d1[NS];
+            digits<NS>(v0, fexp[c], d0);
+            digits<NS>(v1, fexp[c], d1);
 #pragma unroll
-            for (int q = 0; q < NS; ++q) qsb[((size_t)(p * NS + q) * NCP + c) * kpad + ip] = bf16_bits(d[q]);
+            for (int q = 0; q < NS; ++q)
+                qsh[(((size_t)(p * NS + q) * NCP + c) * kpad + ip) >> 1] = digit_pair(d0[q], d1[q]);
         }
         __syncthreads();
         // (16-latitude tile, parity) pairs spread over the waves
-        const __bf16* Pt = g.Psl + oSS;
+        const signed char* Pt = g.Psl + oSS;
         const int* er = g.esyn + (size_t)m * 2 * nhp;
         for (int tt = wave; tt < 2 * (nhp / 16); tt += kStageWaves) {
             const int p = tt & 1;
             const int j0 = (tt >> 1) * 16;
             {
-                floatx4 acc[NS][NTS];
+                intx4 acc[NS][NTS];
 #pragma unroll
                 for (int q = 0; q < NS; ++q)
 #pragma unroll
-                    for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
                 // the table digits of kKC K-chunks are loaded before their MFMAs
                 // (the wave's first tile: before the epilogue)
                 constexpr int kKC = kKCS;
-                for (int kg = 0; kg < kpad; kg += 32 * kKC) {
-                    bf16x8 av[kKC][NS];
+                for (int kg = 0; kg < kpad; kg += kSplitK * kKC) {
+                    intx4 av[kKC][NS];
                     if (tt == wave && kg == 0) {
 #pragma unroll
-                        for (int c4 = 0; c4 < kKC; ++c4) {
-                            const int k0 = (32 * c4 < kpad) ? 32 * c4 : 0;
+                        for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
-                            for (int q = 0; q < NS; ++q)
-                                av[c4][q] = (c4 < kPFS) ? avS[c4 < kPFS ? c4 : 0][q]
-                                    : *reinterpret_cast<const bf16x8*>(
-                                          Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) *
-                                                   kpad + k0 + (lane >> 4) * 8);
-                        }
+                            for (int q = 0; q < NS; ++q) av[c4][q] = avS[c4][q];
                     } else {
 #pragma unroll
                         for (int c4 = 0; c4 < kKC; ++c4) {
-                            const int k0 = (kg + 32 * c4 < kpad) ? kg + 32 * c4 : kg;
+                            const int k0 = (kg + kSplitK * c4 < kpad) ? kg + kSplitK * c4 : kg;
 #pragma unroll
                             for (int q = 0; q < NS; ++q)
-                                av[c4][q] = *reinterpret_cast<const bf16x8*>(
+                                av[c4][q] = *reinterpret_cast<const intx4*>(
                                     Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad +
-                                    k0 + (lane >> 4) * 8);
+                                    k0 + (lane >> 4) * 16);
                         }
                     }
 #pragma unroll
                     for (int c4 = 0; c4 < kKC; ++c4) {
-                        const int k0 = kg + 32 * c4;
+                        const int k0 = kg + kSplitK * c4;
                         if (k0 < kpad) {
 #pragma unroll
                             for (int nt = 0; nt < NTS; ++nt)
 #pragma unroll
                                 for (int t2 = 0; t2 < NS; ++t2) {
-                                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                                    const intx4 bv = *reinterpret_cast<const intx4*>(
                                         qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
-                                                 kpad + k0 + (lane >> 4) * 8);
+                                                 kpad + k0 + (lane >> 4) * 16);
 #pragma unroll
                                     for (int q = 0; q + t2 < NS; ++q)
-                                        acc[q + t2][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                                             av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
                                 }
                         }
@@ -984,8 +986,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                         const int j = j0 + (lane >> 4) * 4 + r;
                         double v = 0.0;
 #pragma unroll
-                        for (int q = NS - 1; q >= 0; --q) v = v * (1.0 / 256.0) + (double)acc[q][nt][r];
-                        v = ldexp(v, ((tt == wave) ? erS[r] : er[p * nhp + j]) + fexp[c] - 14);
+                        for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
+                        v = ldexp(v, ((tt == wave) ? erS[r] : er[p * nhp + j]) + fexp[c]) * kInvTop2;
                         const double w = __shfl_xor(v, 1);
                         if (!(c & 1) && c < NCS && j < nh) {
                             const int bb = c / (NF * 2), f = (c >> 1) % NF;

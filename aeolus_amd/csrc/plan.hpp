@@ -19,13 +19,13 @@ struct GaussDev {
     const double* xn;         // nh northern nodes x_j = cos(colatitude)
     const double2* rec;       // [m][l] (a_lm, b_lm) of the Pbar recurrence (l >= m + 2)
     FftDev fft;
-    // Split-precision tables (nslices > 0): BF16 integer digits of Pbar with
-    // one power-of-two exponent per table row (see split.hpp).
+    // Split-precision tables (nslices > 0): INT8 radix-254 digits of Pbar
+    // with one power-of-two exponent per table row (see split.hpp).
     int nslices;
-    const __bf16* Psl;        // synthesis digits, block m: [parity][slice][nhp][kpad_m]
+    const signed char* Psl;   // synthesis digits, block m: [parity][slice][nhp][kpad_m]
     const long long* offs_ss; // element offset of block m
     const int* esyn;          // row exponents, block m: [parity][nhp] at m * 2 * nhp
-    const __bf16* Pasl;       // analysis digits, block m: [parity][slice][mpad_m][nhk]
+    const signed char* Pasl;  // analysis digits, block m: [parity][slice][mpad_m][nhk]
     const long long* offs_sa;
     const int* eana;          // row exponents, block m: [parity][mpad_m] at offs_ea[m]
     const long long* offs_ea;
@@ -57,10 +57,10 @@ struct aeo_plan_s {
     double2* d_tw = nullptr;
     aeo::FftDev fft{};
     int nslices = 0;
-    __bf16* d_Psl = nullptr;
+    signed char* d_Psl = nullptr;
     long long* d_offs_ss = nullptr;
     int* d_esyn = nullptr;
-    __bf16* d_Pasl = nullptr;
+    signed char* d_Pasl = nullptr;
     long long* d_offs_sa = nullptr;
     int* d_eana = nullptr;
     long long* d_offs_ea = nullptr;
@@ -183,7 +183,7 @@ struct aeo_model_s {
     double2* d_kbuf = nullptr;     // 2 slots: stage tendency cotangents
     int nmem = 1;                  // members (independent states) in the group
     int mb = 1;                    // members per workgroup in the Legendre stage kernel
-    int nslices = 0;               // 0: FP64 contractions; > 0: split BF16 MFMA digits
+    int nslices = 0;               // 0: FP64 contractions; > 0: split INT8 MFMA digits
     ProfRec prof;
 };
 

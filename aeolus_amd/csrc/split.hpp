@@ -1,20 +1,23 @@
-// Split-precision digits for the Legendre contractions on BF16 MFMA.
+// Split-precision digits for the Legendre contractions on INT8 MFMA.
 //
 // A value v of a row (or column) whose largest magnitude is below 2^e is
-// written as
-//     v = 2^(e-7) * sum_s d_s 2^(-8 s),   d_s integer in [-128, 128],
-// with round-to-nearest digits (r = v 2^(7-e); d_s = rint(r); r = (r - d_s) 256).
-// Every digit is an exact BF16 value.  A product of two digits is below 2^14
-// and a K-term sum of products of equal shift s + t = q stays below 2^24 when
-// (q+1) K 2^14 < 2^24, so the FP32 MFMA accumulator of each shift q is exact;
-// the result is recombined in FP64:
-//     sum_i P A = 2^(eP + eA - 14) sum_q 2^(-8 q) acc_q.
-// Shifts q >= nslices are dropped (relative truncation ~ nslices 2^(-8 nslices)).
+// written with balanced radix-254 digits, one signed byte each:
+//     v = (2^e / 127) * sum_s d_s 254^(-s),   d_s integer in [-127, 127],
+// r_0 = 127 v 2^-e (|r_0| < 127), d_s = rint(r_s), r_(s+1) = 254 (r_s - d_s).
+// |r_s - d_s| <= 1/2 keeps every later digit in [-127, 127]; r_s - d_s is
+// exact, the products by 127 and 254 carry one FP64 rounding each (relative
+// 2^-53, far below the truncation 254^-NS).  A digit product is at most
+// 127^2 and the INT32 MFMA accumulator of each shift q = s + t collects
+// (q+1) K of them, exact while NS K 127^2 < 2^31; the result is recombined
+// in FP64:
+//     sum_i P A = 2^(eP + eA) / 127^2 * sum_q 254^(-q) acc_q.
+// Shifts q >= nslices are dropped (relative truncation ~ 254^-nslices).
 //
-// Table layouts (m-blocks; kp(m) = ceil((L-m+1)/2) degrees per parity):
-//   synthesis  [parity][slice][nhp = roundup(nh,16)][kpad = roundup(kp,32)]
+// Table layouts (m-blocks; kp(m) = ceil((L-m+1)/2) degrees per parity; K
+// padded to the MFMA depth kSplitK = 64 with zero digits):
+//   synthesis  [parity][slice][nhp = roundup(nh,16)][kpad = roundup(kp,64)]
 //              rows = latitude, K = degree index within the parity class
-//   analysis   [parity][slice][mpad = roundup(kp,16)][nhk = roundup(nh,32)]
+//   analysis   [parity][slice][mpad = roundup(kp,16)][nhk = roundup(nh,64)]
 //              rows = degree within the parity class, K = latitude
 #pragma once
 
@@ -22,11 +25,18 @@
 
 namespace aeo {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef int intx4 __attribute__((ext_vector_type(4)));   // 16 INT8 digits or 4 INT32 sums
+
+constexpr int kSplitK = 64;              // K depth of one v_mfma_i32_16x16x64_i8
+constexpr double kDigitTop = 127.0;      // first digit scale
+constexpr double kDigitRadix = 254.0;
+constexpr double kInvRadix = 1.0 / 254.0;
+constexpr double kInvTop2 = 1.0 / (127.0 * 127.0);
 
 __host__ __device__ inline int rup(int x, int a) { return (x + a - 1) / a * a; }
 __host__ __device__ inline int split_kp(int L, int m) { return (L - m + 2) / 2; }
+__host__ __device__ inline int split_kpad(int L, int m) { return rup(split_kp(L, m), kSplitK); }
+__host__ __device__ inline int split_nhk(int nh) { return rup(nh, kSplitK); }
 
 // Exponent e with max < 2^e (frexp convention; 0 for an all-zero row).
 __device__ inline int exp_above(double mx) {
@@ -36,35 +46,37 @@ __device__ inline int exp_above(double mx) {
     return e;
 }
 
+// The scalings are written as fma(x, c, 0): a rounded product that the
+// compiler cannot fuse with the following subtraction, so every build and
+// every member tiling yields the same digits.
 template <int NS>
-__device__ inline void digits(double v, int e, float* d) {
-    double r = ldexp(v, 7 - e);
+__device__ inline void digits(double v, int e, int* d) {
+    double r = fma(ldexp(v, -e), kDigitTop, 0.0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const double q = rint(r);
-        d[s] = (float)q;
-        r = (r - q) * 256.0;
+        d[s] = (int)q;
+        r = fma(r - q, kDigitRadix, 0.0);
     }
 }
 
-__device__ inline void digits_rt(double v, int e, int ns, float* d) {
-    double r = ldexp(v, 7 - e);
+__device__ inline void digits_rt(double v, int e, int ns, int* d) {
+    double r = fma(ldexp(v, -e), kDigitTop, 0.0);
     for (int s = 0; s < ns; ++s) {
         const double q = rint(r);
-        d[s] = (float)q;
-        r = (r - q) * 256.0;
+        d[s] = (int)q;
+        r = fma(r - q, kDigitRadix, 0.0);
     }
 }
 
-// BF16 bit pattern of a digit: an integer of magnitude <= 256 is exact in
-// FP32 and in BF16, whose bits are the upper half of the FP32 bits.
-__device__ inline unsigned short bf16_bits(float d) {
-    return (unsigned short)(__float_as_uint(d) >> 16);
+// Two digits as one 16-bit word (lower address = first digit).
+__device__ inline unsigned short digit_pair(int d0, int d1) {
+    return (unsigned short)((d0 & 0xff) | ((d1 & 0xff) << 8));
 }
 
-// Largest number of FP32-exact accumulations: (nslices) * K * 2^14 < 2^24.
+// INT32 accumulation stays exact: nslices * K * 127^2 < 2^31.
 inline bool split_exact(int nslices, int kmax) {
-    return (long long)nslices * kmax * 16384LL < (1LL << 24);
+    return (long long)nslices * kmax * 16129LL < (1LL << 31);
 }
 
 }  // namespace aeo

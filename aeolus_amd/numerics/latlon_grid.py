@@ -125,6 +125,6 @@ class GaussLatLonSphericalHarmonics(_TransformBase):
         self.coslat_rows = torch.as_tensor(c, device=grid.device)
 
     def enable_split(self, nslices: int) -> None:
-        """Build the BF16 digit tables for split-precision contractions."""
+        """Build the INT8 digit tables for split-precision contractions."""
         nat.check(nat.lib().aeo_plan_enable_split(self._plan, int(nslices)), "enable_split")
         self.nslices = int(nslices)

@@ -5,7 +5,7 @@ Gauss 256 x 512 grid (single member, dt = 300 s, seeded l^-1.5 vorticity
 with rms 5e-5 s^-1, R = 6.371e6 m, 24 h rotation) for the fastest precision
 mode whose vorticity after one simulated day (288 steps) stays within 1e-11
 relative L2 of the plain FP64 step: plain FP64 Legendre contractions, or
-split BF16-digit MFMA contractions with k in {4, 5, 6, 7} slices (FP64 state,
+split INT8-digit MFMA contractions with k in {4, 5, 6, 7} slices (FP64 state,
 recombination, tables and RK4 update).  The plain FP64 step itself matches
 the NumPy oracle to <= 1e-12 (tests).
 
@@ -378,7 +378,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best == 0 else f"bf16 digits x{best} (fp32 accum) + f64",
+        "dtype": "f64" if best == 0 else f"int8 digits x{best} (int32 accum) + f64",
         "data": "synthetic (seeded l^-1.5 spectra, random-phase)",
         "config": {"workload": f"BVE RK4, Gauss 256x512, L=170, single member, dt=300 s, "
                                f"{_mode_name(best)} Legendre contractions",

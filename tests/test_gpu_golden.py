@@ -2,7 +2,7 @@
 written by tests/golden/make_golden.py from the NumPy oracle).
 
 Bars: plain FP64 within 1e-12 relative L2 of the oracle; split-precision
-Legendre contractions (6 or 7 BF16 digit slices, FP64 state) within 1e-11
+Legendre contractions (6 or 7 INT8 digit slices, FP64 state) within 1e-11
 relative L2 of the FP64 oracle after one simulated day, per field.
 """
 import math

@@ -1,11 +1,12 @@
-"""Split-precision (BF16 digit MFMA, exact FP32 accumulation, FP64
-recombination) Legendre contractions against the FP64 path and the oracle.
+"""Split-precision (INT8 radix-254 digit MFMA, exact INT32 accumulation,
+FP64 recombination) Legendre contractions against the FP64 path and the
+oracle.
 
-Bars (measured on MI355X: 6 slices 5e-14 / 9e-13 / 1.7e-12 and 7 slices
-6e-16 / 2e-15 / 7e-15 relative L2 at L = 21 / 85 / 170): 6 slices within
-5e-12 of the FP64 tendency, 7 slices within 1e-13 (FP64 level), 4 slices
-coarser than 1e-12 but within 1e-7 — the digits carry what the slice count
-says.
+Bars (measured on MI355X at L = 170: 4 slices 1.1e-7, 6 slices 2.9e-12,
+7 slices 2.2e-15 relative L2): 6 slices within 5e-12 of the FP64 tendency,
+7 slices within 1e-13 (FP64 level), 4 slices coarser than 1e-12 but within
+5e-7 — the digits carry what the slice count says (254^-4 = 2.4e-10 per
+entry before the sums).
 """
 import math
 import warnings
@@ -47,7 +48,7 @@ def test_split_bve_tendency_matches_fp64(L, nlat, nlon):
     m.set_precision(0)
     print("split tendency rel errors", errs)
     assert errs[6] < 5e-12 and errs[7] < 1e-13
-    assert 1e-12 < errs[4] < 1e-7
+    assert 1e-12 < errs[4] < 5e-7
 
 
 def test_split_batched_members_match_single():
