@@ -695,22 +695,30 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             __syncthreads();
             // rows (p, c) over waves, latitude pairs over lanes (one 16-bit
             // store per slice): no runtime divisions
-            for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
-            for (int j = 2 * lane; j < nhk; j += 128) {
+            // (padding columns c >= NCA: zero rows, no digit work; the branch is
+            // uniform over the wave)
+            for (int rr = wave; rr < 2 * NCP; rr += kStageWaves) {
                 const int p = rr / NCP, c = rr - p * NCP;
-                double v0 = 0.0, v1 = 0.0;
-                if (c < NCA) {
-                    const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                    const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
-                    if (j < nh) v0 = cpart(xr[j], ri);
-                    if (j + 1 < nh) v1 = cpart(xr[j + 1], ri);
-                }
-                int d0[NS], d1[NS];
-                digits<NS>(v0, gexp[p * NCP + c], d0);
-                digits<NS>(v1, gexp[p * NCP + c], d1);
+                unsigned short* row = xsh + (((size_t)p * NS * NCP + c) * nhk >> 1);
+                const size_t qstride = (size_t)NCP * nhk >> 1;
+                if (c >= NCA) {
+                    for (int j = 2 * lane; j < nhk; j += 128)
 #pragma unroll
-                for (int q = 0; q < NS; ++q)
-                    xsh[(((size_t)(p * NS + q) * NCP + c) * nhk + j) >> 1] = digit_pair(d0[q], d1[q]);
+                        for (int q = 0; q < NS; ++q) row[q * qstride + (j >> 1)] = 0;
+                    continue;
+                }
+                const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
+                const int e = gexp[p * NCP + c];
+                for (int j = 2 * lane; j < nhk; j += 128) {
+                    const double v0 = (j < nh) ? cpart(xr[j], ri) : 0.0;
+                    const double v1 = (j + 1 < nh) ? cpart(xr[j + 1], ri) : 0.0;
+                    int d0[NS], d1[NS];
+                    digits<NS>(v0, e, d0);
+                    digits<NS>(v1, e, d1);
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) row[q * qstride + (j >> 1)] = digit_pair(d0[q], d1[q]);
+                }
             }
             __syncthreads();
             const signed char* Pt = g.Pasl + oSA;
@@ -907,22 +915,29 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         });
         __syncthreads();
         // rows (p, c) over waves, degree pairs (ip, ip + 1) over lanes
-        for (int rr = wave; rr < 2 * NCP; rr += kStageWaves)
-        for (int ip = 2 * lane; ip < kpad; ip += 128) {
+        // (padding columns c >= NCS: zero rows, no digit work)
+        for (int rr = wave; rr < 2 * NCP; rr += kStageWaves) {
             const int p = rr / NCP, c = rr - p * NCP;
-            const int i = 2 * ip + p;
-            double v0 = 0.0, v1 = 0.0;
-            if (c < NCS) {
-                const double2* cr = cols + (size_t)(c >> 1) * K;
-                if (i < K) v0 = cpart(cr[i], c & 1);
-                if (i + 2 < K) v1 = cpart(cr[i + 2], c & 1);
-            }
-            int d0[NS], d1[NS];
-            digits<NS>(v0, fexp[c], d0);
-            digits<NS>(v1, fexp[c], d1);
+            unsigned short* row = qsh + (((size_t)p * NS * NCP + c) * kpad >> 1);
+            const size_t qstride = (size_t)NCP * kpad >> 1;
+            if (c >= NCS) {
+                for (int ip = 2 * lane; ip < kpad; ip += 128)
 #pragma unroll
-            for (int q = 0; q < NS; ++q)
-                qsh[(((size_t)(p * NS + q) * NCP + c) * kpad + ip) >> 1] = digit_pair(d0[q], d1[q]);
+                    for (int q = 0; q < NS; ++q) row[q * qstride + (ip >> 1)] = 0;
+                continue;
+            }
+            const double2* cr = cols + (size_t)(c >> 1) * K;
+            const int e = fexp[c];
+            for (int ip = 2 * lane; ip < kpad; ip += 128) {
+                const int i = 2 * ip + p;
+                const double v0 = (i < K) ? cpart(cr[i], c & 1) : 0.0;
+                const double v1 = (i + 2 < K) ? cpart(cr[i + 2], c & 1) : 0.0;
+                int d0[NS], d1[NS];
+                digits<NS>(v0, e, d0);
+                digits<NS>(v1, e, d1);
+#pragma unroll
+                for (int q = 0; q < NS; ++q) row[q * qstride + (ip >> 1)] = digit_pair(d0[q], d1[q]);
+            }
         }
         __syncthreads();
         // (16-latitude tile, parity) pairs spread over the waves
