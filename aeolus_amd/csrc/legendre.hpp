@@ -728,77 +728,106 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             const int Mv0 = (top >= m) ? (top - m) / 2 + 1 : 0;
             const int Mv1 = (top >= m + 1) ? (top - m - 1) / 2 + 1 : 0;
             const int MT0 = (Mv0 + 15) / 16, MT1 = (Mv1 + 15) / 16;
+            // Tiles are software-pipelined: the next tile's first K-group and
+            // row exponents are loaded while this tile's MFMAs drain and it is
+            // recombined; the wave's first tile came from the prologue.
+            constexpr int kKC = kKCA;
+            intx4 cur[kKC][NS], nxt[kKC][NS];
+            int ecur[4], enxt[4];
+#pragma unroll
+            for (int c4 = 0; c4 < kKC; ++c4)
+#pragma unroll
+                for (int q = 0; q < NS; ++q) cur[c4][q] = avA[c4][q];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ecur[r] = erA[r];
             for (int tt = wave; tt < MT0 + MT1; tt += kStageWaves) {
                 const int p = (tt < MT0) ? 0 : 1;
                 const int mt = (tt < MT0) ? tt : tt - MT0;
                 const int Mv = p ? Mv1 : Mv0;
-                {
-                    intx4 acc[NS][NTA];
+                intx4 acc[NS][NTA];
 #pragma unroll
-                    for (int q = 0; q < NS; ++q)
+                for (int q = 0; q < NS; ++q)
 #pragma unroll
-                        for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
-                    // the table digits of kKC K-chunks are loaded before their MFMAs
-                    // (the wave's first tile: in the prologue)
-                    constexpr int kKC = kKCA;
-                    for (int kg = 0; kg < nhk; kg += kSplitK * kKC) {
-                        intx4 av[kKC][NS];
-                        if (tt == wave && kg == 0) {
+                    for (int nt = 0; nt < NTA; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
+                for (int kg = 0; kg < nhk; kg += kSplitK * kKC) {
+                    intx4 av[kKC][NS];
+                    if (kg == 0) {
 #pragma unroll
-                            for (int c4 = 0; c4 < kKC; ++c4)
+                        for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
-                                for (int q = 0; q < NS; ++q) av[c4][q] = avA[c4][q];
-                        } else {
-#pragma unroll
-                            for (int c4 = 0; c4 < kKC; ++c4) {
-                                const int k0 = (kg + kSplitK * c4 < nhk) ? kg + kSplitK * c4 : kg;
-#pragma unroll
-                                for (int q = 0; q < NS; ++q)
-                                    av[c4][q] = *reinterpret_cast<const intx4*>(
-                                        Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) *
-                                                 nhk + k0 + (lane >> 4) * 16);
-                            }
-                        }
+                            for (int q = 0; q < NS; ++q) av[c4][q] = cur[c4][q];
+                    } else {
 #pragma unroll
                         for (int c4 = 0; c4 < kKC; ++c4) {
-                            const int k0 = kg + kSplitK * c4;
-                            if (k0 < nhk) {
+                            const int k0 = (kg + kSplitK * c4 < nhk) ? kg + kSplitK * c4 : kg;
 #pragma unroll
-                                for (int nt = 0; nt < NTA; ++nt)
-#pragma unroll
-                                    for (int t2 = 0; t2 < NS; ++t2) {
-                                        const intx4 bv = *reinterpret_cast<const intx4*>(
-                                            xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
-                                                     nhk + k0 + (lane >> 4) * 16);
-#pragma unroll
-                                        for (int q = 0; q + t2 < NS; ++q)
-                                            acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                                av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
-                                    }
-                            }
+                            for (int q = 0; q < NS; ++q)
+                                av[c4][q] = *reinterpret_cast<const intx4*>(
+                                    Pt + ((size_t)(p * NS + q) * mpad + mt * 16 + (lane & 15)) * nhk +
+                                    k0 + (lane >> 4) * 16);
                         }
                     }
 #pragma unroll
-                    for (int nt = 0; nt < NTA; ++nt) {
-                        const int c = nt * 16 + (lane & 15);
+                    for (int c4 = 0; c4 < kKC; ++c4) {
+                        const int k0 = kg + kSplitK * c4;
+                        if (k0 < nhk) {
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int ir = mt * 16 + (lane >> 4) * 4 + r;
-                            double v = 0.0;
+                            for (int nt = 0; nt < NTA; ++nt)
 #pragma unroll
-                            // explicit fma: the same rounding in every member tiling
-                            // explicit fma: the same rounding in every member tiling
-                        for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
-                            if (c < NCA && ir < Mv) {
-                                const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                                const int l = m + 2 * ir + p;
-                                const int e = (tt == wave) ? erA[r] : er[p * mpad + ir];
-                                accd[2 * ((size_t)(bb * NA + f) * n + l) + ri] =
-                                    ldexp(v, e + gexp[p * NCP + c]) * kInvTop2;
-                            }
+                                for (int t2 = 0; t2 < NS; ++t2) {
+                                    const intx4 bv = *reinterpret_cast<const intx4*>(
+                                        xs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * nhk +
+                                        k0 + (lane >> 4) * 16);
+#pragma unroll
+                                    for (int q = 0; q + t2 < NS; ++q)
+                                        acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                            av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
+                                }
                         }
                     }
                 }
+                // next tile's first K-group and exponents: issued after this
+                // tile's MFMAs, read at the next iteration (no reload on the last)
+                const int tn = tt + kStageWaves;
+                if (tn < MT0 + MT1) {
+                    const int pn = (tn < MT0) ? 0 : 1;
+                    const int mtn = (tn < MT0) ? tn : tn - MT0;
+#pragma unroll
+                    for (int c4 = 0; c4 < kKC; ++c4) {
+                        const int k0 = (kSplitK * c4 < nhk) ? kSplitK * c4 : 0;
+#pragma unroll
+                        for (int q = 0; q < NS; ++q)
+                            nxt[c4][q] = *reinterpret_cast<const intx4*>(
+                                Pt + ((size_t)(pn * NS + q) * mpad + mtn * 16 + (lane & 15)) * nhk +
+                                k0 + (lane >> 4) * 16);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) enxt[r] = er[pn * mpad + mtn * 16 + (lane >> 4) * 4 + r];
+                }
+#pragma unroll
+                for (int nt = 0; nt < NTA; ++nt) {
+                    const int c = nt * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int ir = mt * 16 + (lane >> 4) * 4 + r;
+                        // explicit fma: the same rounding in every member tiling
+                        double v = 0.0;
+#pragma unroll
+                        for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
+                        if (c < NCA && ir < Mv) {
+                            const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
+                            const int l = m + 2 * ir + p;
+                            accd[2 * ((size_t)(bb * NA + f) * n + l) + ri] =
+                                ldexp(v, ecur[r] + gexp[p * NCP + c]) * kInvTop2;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c4 = 0; c4 < kKC; ++c4)
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) cur[c4][q] = nxt[c4][q];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ecur[r] = enxt[r];
             }
             __syncthreads();
         }
@@ -943,76 +972,105 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         // (16-latitude tile, parity) pairs spread over the waves
         const signed char* Pt = g.Psl + oSS;
         const int* er = g.esyn + (size_t)m * 2 * nhp;
-        for (int tt = wave; tt < 2 * (nhp / 16); tt += kStageWaves) {
+        // software-pipelined like the analysis tiles; the wave's first tile
+        // was issued before derive_columns
+        constexpr int kKC = kKCS;
+        const int T = 2 * (nhp / 16);
+        intx4 cur[kKC][NS], nxt[kKC][NS];
+        int ecur[4], enxt[4];
+#pragma unroll
+        for (int c4 = 0; c4 < kKC; ++c4)
+#pragma unroll
+            for (int q = 0; q < NS; ++q) cur[c4][q] = avS[c4][q];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ecur[r] = erS[r];
+        for (int tt = wave; tt < T; tt += kStageWaves) {
             const int p = tt & 1;
             const int j0 = (tt >> 1) * 16;
-            {
-                intx4 acc[NS][NTS];
+            intx4 acc[NS][NTS];
 #pragma unroll
-                for (int q = 0; q < NS; ++q)
+            for (int q = 0; q < NS; ++q)
 #pragma unroll
-                    for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
-                // the table digits of kKC K-chunks are loaded before their MFMAs
-                // (the wave's first tile: before the epilogue)
-                constexpr int kKC = kKCS;
-                for (int kg = 0; kg < kpad; kg += kSplitK * kKC) {
-                    intx4 av[kKC][NS];
-                    if (tt == wave && kg == 0) {
+                for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
+            for (int kg = 0; kg < kpad; kg += kSplitK * kKC) {
+                intx4 av[kKC][NS];
+                if (kg == 0) {
 #pragma unroll
-                        for (int c4 = 0; c4 < kKC; ++c4)
+                    for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
-                            for (int q = 0; q < NS; ++q) av[c4][q] = avS[c4][q];
-                    } else {
-#pragma unroll
-                        for (int c4 = 0; c4 < kKC; ++c4) {
-                            const int k0 = (kg + kSplitK * c4 < kpad) ? kg + kSplitK * c4 : kg;
-#pragma unroll
-                            for (int q = 0; q < NS; ++q)
-                                av[c4][q] = *reinterpret_cast<const intx4*>(
-                                    Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad +
-                                    k0 + (lane >> 4) * 16);
-                        }
-                    }
+                        for (int q = 0; q < NS; ++q) av[c4][q] = cur[c4][q];
+                } else {
 #pragma unroll
                     for (int c4 = 0; c4 < kKC; ++c4) {
-                        const int k0 = kg + kSplitK * c4;
-                        if (k0 < kpad) {
+                        const int k0 = (kg + kSplitK * c4 < kpad) ? kg + kSplitK * c4 : kg;
 #pragma unroll
-                            for (int nt = 0; nt < NTS; ++nt)
-#pragma unroll
-                                for (int t2 = 0; t2 < NS; ++t2) {
-                                    const intx4 bv = *reinterpret_cast<const intx4*>(
-                                        qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) *
-                                                 kpad + k0 + (lane >> 4) * 16);
-#pragma unroll
-                                    for (int q = 0; q + t2 < NS; ++q)
-                                        acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                            av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
-                                }
-                        }
+                        for (int q = 0; q < NS; ++q)
+                            av[c4][q] = *reinterpret_cast<const intx4*>(
+                                Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
+                                (lane >> 4) * 16);
                     }
                 }
-                double2* dst = (p == 0) ? a.E : a.O;
 #pragma unroll
-                for (int nt = 0; nt < NTS; ++nt) {
-                    const int c = nt * 16 + (lane & 15);
+                for (int c4 = 0; c4 < kKC; ++c4) {
+                    const int k0 = kg + kSplitK * c4;
+                    if (k0 < kpad) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int j = j0 + (lane >> 4) * 4 + r;
-                        double v = 0.0;
+                        for (int nt = 0; nt < NTS; ++nt)
 #pragma unroll
-                        for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
-                        v = ldexp(v, ((tt == wave) ? erS[r] : er[p * nhp + j]) + fexp[c]) * kInvTop2;
-                        const double w = __shfl_xor(v, 1);
-                        if (!(c & 1) && c < NCS && j < nh) {
-                            const int bb = c / (NF * 2), f = (c >> 1) % NF;
-                            const int b = b0 + bb;
-                            if (b < a.nmem)
-                                dst[(((size_t)j * n + m) * a.nmem + b) * NF + f] = c2(v, w);
-                        }
+                            for (int t2 = 0; t2 < NS; ++t2) {
+                                const intx4 bv = *reinterpret_cast<const intx4*>(
+                                    qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * kpad +
+                                    k0 + (lane >> 4) * 16);
+#pragma unroll
+                                for (int q = 0; q + t2 < NS; ++q)
+                                    acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                        av[c4][q], bv, acc[q + t2][nt], 0, 0, 0);
+                            }
                     }
                 }
             }
+            // next tile's first K-group and exponents (after this tile's MFMAs)
+            const int tn = tt + kStageWaves;
+            if (tn < T) {
+                const int pn = tn & 1, jn = (tn >> 1) * 16;
+#pragma unroll
+                for (int c4 = 0; c4 < kKC; ++c4) {
+                    const int k0 = (kSplitK * c4 < kpad) ? kSplitK * c4 : 0;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q)
+                        nxt[c4][q] = *reinterpret_cast<const intx4*>(
+                            Pt + ((size_t)(pn * NS + q) * nhp + jn + (lane & 15)) * kpad + k0 +
+                            (lane >> 4) * 16);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) enxt[r] = er[pn * nhp + jn + (lane >> 4) * 4 + r];
+            }
+            double2* dst = (p == 0) ? a.E : a.O;
+#pragma unroll
+            for (int nt = 0; nt < NTS; ++nt) {
+                const int c = nt * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = j0 + (lane >> 4) * 4 + r;
+                    // explicit fma: the same rounding in every member tiling
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][nt][r]);
+                    v = ldexp(v, ecur[r] + fexp[c]) * kInvTop2;
+                    const double w = __shfl_xor(v, 1);
+                    if (!(c & 1) && c < NCS && j < nh) {
+                        const int bb = c / (NF * 2), f = (c >> 1) % NF;
+                        const int b = b0 + bb;
+                        if (b < a.nmem) dst[(((size_t)j * n + m) * a.nmem + b) * NF + f] = c2(v, w);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c4 = 0; c4 < kKC; ++c4)
+#pragma unroll
+                for (int q = 0; q < NS; ++q) cur[c4][q] = nxt[c4][q];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ecur[r] = enxt[r];
         }
     }
     if constexpr (SYN && NS == 0) {
