@@ -516,6 +516,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     intx4 avA[kPFA][NSX];
     int erA[4] = {0, 0, 0, 0};
     intx4 avS[kPFS][NSX];
+    long avSt[NSX];                      // synthesis K tail (32 deep)
     int erS[4] = {0, 0, 0, 0};
     long long oSA = 0, oEA = 0, oSS = 0;
     if constexpr (NS > 0) {
@@ -556,18 +557,25 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     auto prefetch_syn = [&]() {
         if constexpr (SYN && NS > 0) {
             const int kpad = split_kpad(L, m), nhp = rup(nh, 16);
+            const int kfull = kpad & ~(kSplitK - 1);
+            // rows of 32 bytes (no full chunk): the lane offsets stay in the row
+            const int lo = (kfull > 0) ? (lane >> 4) * 16 : ((lane >> 4) & 1) * 16;
             const int tt0 = (wave < 2 * (nhp / 16)) ? wave : 0;
             const int p = tt0 & 1, j0 = (tt0 >> 1) * 16;
             const signed char* Pt = g.Psl + oSS;
 #pragma unroll
             for (int c4 = 0; c4 < kPFS; ++c4) {
-                const int k0 = (kSplitK * c4 < kpad) ? kSplitK * c4 : 0;
+                const int k0 = (kSplitK * c4 + kSplitK <= kfull) ? kSplitK * c4 : 0;
 #pragma unroll
                 for (int q = 0; q < NSX; ++q)
                     avS[c4][q] = *reinterpret_cast<const intx4*>(
-                        Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
-                        (lane >> 4) * 16);
+                        Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 + lo);
             }
+#pragma unroll
+            for (int q = 0; q < NSX; ++q)
+                avSt[q] = *reinterpret_cast<const long*>(
+                    Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + kpad - 32 +
+                    (lane >> 4) * 8);
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 erS[r] = g.esyn[(size_t)m * 2 * nhp + p * nhp + j0 + (lane >> 4) * 4 + r];
@@ -976,12 +984,17 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         // was issued before derive_columns
         constexpr int kKC = kKCS;
         const int T = 2 * (nhp / 16);
+        const int kfull = kpad & ~(kSplitK - 1);     // 64-deep chunks; a 32-deep tail follows
+        const int lo = (kfull > 0) ? (lane >> 4) * 16 : ((lane >> 4) & 1) * 16;
         intx4 cur[kKC][NS], nxt[kKC][NS];
+        long curt[NS], nxtt[NS];
         int ecur[4], enxt[4];
 #pragma unroll
         for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
             for (int q = 0; q < NS; ++q) cur[c4][q] = avS[c4][q];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) curt[q] = avSt[q];
 #pragma unroll
         for (int r = 0; r < 4; ++r) ecur[r] = erS[r];
         for (int tt = wave; tt < T; tt += kStageWaves) {
@@ -992,7 +1005,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             for (int q = 0; q < NS; ++q)
 #pragma unroll
                 for (int nt = 0; nt < NTS; ++nt) acc[q][nt] = intx4{0, 0, 0, 0};
-            for (int kg = 0; kg < kpad; kg += kSplitK * kKC) {
+            for (int kg = 0; kg < kfull; kg += kSplitK * kKC) {
                 intx4 av[kKC][NS];
                 if (kg == 0) {
 #pragma unroll
@@ -1002,7 +1015,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 } else {
 #pragma unroll
                     for (int c4 = 0; c4 < kKC; ++c4) {
-                        const int k0 = (kg + kSplitK * c4 < kpad) ? kg + kSplitK * c4 : kg;
+                        const int k0 = (kg + kSplitK * c4 < kfull) ? kg + kSplitK * c4 : kg;
 #pragma unroll
                         for (int q = 0; q < NS; ++q)
                             av[c4][q] = *reinterpret_cast<const intx4*>(
@@ -1013,7 +1026,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 #pragma unroll
                 for (int c4 = 0; c4 < kKC; ++c4) {
                     const int k0 = kg + kSplitK * c4;
-                    if (k0 < kpad) {
+                    if (k0 < kfull) {
 #pragma unroll
                         for (int nt = 0; nt < NTS; ++nt)
 #pragma unroll
@@ -1029,19 +1042,37 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     }
                 }
             }
+            if (kpad != kfull) {   // the 32-deep tail at kfull = kpad - 32
+#pragma unroll
+                for (int nt = 0; nt < NTS; ++nt)
+#pragma unroll
+                    for (int t2 = 0; t2 < NS; ++t2) {
+                        const long bv = *reinterpret_cast<const long*>(
+                            qs + ((size_t)(p * NS + t2) * NCP + nt * 16 + (lane & 15)) * kpad + kfull +
+                            (lane >> 4) * 8);
+#pragma unroll
+                        for (int q = 0; q + t2 < NS; ++q)
+                            acc[q + t2][nt] = __builtin_amdgcn_mfma_i32_16x16x32_i8(
+                                curt[q], bv, acc[q + t2][nt], 0, 0, 0);
+                    }
+            }
             // next tile's first K-group and exponents (after this tile's MFMAs)
             const int tn = tt + kStageWaves;
             if (tn < T) {
                 const int pn = tn & 1, jn = (tn >> 1) * 16;
 #pragma unroll
                 for (int c4 = 0; c4 < kKC; ++c4) {
-                    const int k0 = (kSplitK * c4 < kpad) ? kSplitK * c4 : 0;
+                    const int k0 = (kSplitK * c4 + kSplitK <= kfull) ? kSplitK * c4 : 0;
 #pragma unroll
                     for (int q = 0; q < NS; ++q)
                         nxt[c4][q] = *reinterpret_cast<const intx4*>(
-                            Pt + ((size_t)(pn * NS + q) * nhp + jn + (lane & 15)) * kpad + k0 +
-                            (lane >> 4) * 16);
+                            Pt + ((size_t)(pn * NS + q) * nhp + jn + (lane & 15)) * kpad + k0 + lo);
                 }
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    nxtt[q] = *reinterpret_cast<const long*>(
+                        Pt + ((size_t)(pn * NS + q) * nhp + jn + (lane & 15)) * kpad + kpad - 32 +
+                        (lane >> 4) * 8);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) enxt[r] = er[pn * nhp + jn + (lane >> 4) * 4 + r];
             }
@@ -1069,6 +1100,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             for (int c4 = 0; c4 < kKC; ++c4)
 #pragma unroll
                 for (int q = 0; q < NS; ++q) cur[c4][q] = nxt[c4][q];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) curt[q] = nxtt[q];
 #pragma unroll
             for (int r = 0; r < 4; ++r) ecur[r] = enxt[r];
         }

@@ -14,8 +14,9 @@
 // Shifts q >= nslices are dropped (relative truncation ~ 254^-nslices).
 //
 // Table layouts (m-blocks; kp(m) = ceil((L-m+1)/2) degrees per parity; K
-// padded to the MFMA depth kSplitK = 64 with zero digits):
-//   synthesis  [parity][slice][nhp = roundup(nh,16)][kpad = roundup(kp,64)]
+// padded with zero digits to the MFMA depths: 64, or 32 for the synthesis
+// tail, which takes one v_mfma_i32_16x16x32_i8):
+//   synthesis  [parity][slice][nhp = roundup(nh,16)][kpad = roundup(kp,32)]
 //              rows = latitude, K = degree index within the parity class
 //   analysis   [parity][slice][mpad = roundup(kp,16)][nhk = roundup(nh,64)]
 //              rows = degree within the parity class, K = latitude
@@ -35,7 +36,7 @@ constexpr double kInvTop2 = 1.0 / (127.0 * 127.0);
 
 __host__ __device__ inline int rup(int x, int a) { return (x + a - 1) / a * a; }
 __host__ __device__ inline int split_kp(int L, int m) { return (L - m + 2) / 2; }
-__host__ __device__ inline int split_kpad(int L, int m) { return rup(split_kp(L, m), kSplitK); }
+__host__ __device__ inline int split_kpad(int L, int m) { return rup(split_kp(L, m), 32); }
 __host__ __device__ inline int split_nhk(int nh) { return rup(nh, kSplitK); }
 
 // Exponent e with max < 2^e (frexp convention; 0 for an all-zero row).
