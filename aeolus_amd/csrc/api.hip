@@ -61,7 +61,8 @@ static void free_plan(aeo_plan_s* p) {
                     (void*)p->d_pcoslat, (void*)p->d_gw, (void*)p->d_part, (void*)p->d_Cp,
                     (void*)p->d_Cm, (void*)p->d_spec, (void*)p->d_Psl, (void*)p->d_offs_ss,
                     (void*)p->d_esyn, (void*)p->d_Pasl, (void*)p->d_offs_sa, (void*)p->d_eana,
-                    (void*)p->d_offs_ea, (void*)p->d_xn, (void*)p->d_rec})
+                    (void*)p->d_offs_ea, (void*)p->d_xn, (void*)p->d_rec, (void*)p->d_CpT,
+                    (void*)p->d_CmT})
         dev_free(q);
     delete p;
 }
@@ -113,6 +114,16 @@ static int common_tables(aeo_plan_s* p) {
     if (rc) return rc;
     rc = upload(&p->d_Cm, Cm.data(), Cm.size());
     if (rc) return rc;
+    // column-major copies: the stage kernels read one column m per workgroup
+    const int n = p->L + 1;
+    std::vector<double> CpT((size_t)n * n), CmT((size_t)n * n);
+    for (int l = 0; l < n; ++l)
+        for (int m = 0; m < n; ++m) {
+            CpT[(size_t)m * n + l] = Cp[(size_t)l * n + m];
+            CmT[(size_t)m * n + l] = Cm[(size_t)l * n + m];
+        }
+    if ((rc = upload(&p->d_CpT, CpT.data(), CpT.size()))) return rc;
+    if ((rc = upload(&p->d_CmT, CmT.data(), CmT.size()))) return rc;
     const int nw = std::max(p->max_fields, 4);
     return dev_alloc(&p->d_spec, (size_t)nw * (p->L + 1) * (p->L + 1));
 }

@@ -453,7 +453,7 @@ AEO_GRID_PICK(grid_adj_kernel)
 // Launchers
 // ---------------------------------------------------------------------------
 static ColK col_consts(const aeo_plan_s* p, double R, const double* lap, double f10) {
-    return ColK{p->L, R, p->d_Cp, p->d_Cm, lap, f10};
+    return ColK{p->L, R, p->d_CpT, p->d_CmT, lap, f10};
 }
 
 template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
@@ -768,7 +768,7 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     sa.O = md->d_aO;
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     Rk4Adj rk{stage, dt, a, md->d_asum, anext, kout, knext};
-    EpiBveAdjS ep{L, md->R, fs, md->d_nulap, md->d_lap, p->d_Cp, p->d_Cm, fwd_next, rk};
+    EpiBveAdjS ep{L, md->R, fs, md->d_nulap, md->d_lap, p->d_CpT, p->d_CmT, fwd_next, rk};
     pr.arm(kClsAnalysis);
     if (fwd_next)
         return launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, md->mb, sa, ep, s,

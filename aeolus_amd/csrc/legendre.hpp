@@ -46,8 +46,8 @@ __device__ inline double pe_mul(double p, double c) { return fma(p, c, 0.0); }
 struct ColK {
     int L;
     double R;
-    const double* Cp;   // (L+1)^2, row-major [l][m]
-    const double* Cm;   // (L+1)^2
+    const double* Cp;   // (L+1)^2, column-major [m][l] (a column is contiguous)
+    const double* Cm;   // (L+1)^2, column-major [m][l]
     const double* lap;  // -l(l+1)/R^2 (l = 0 entry 0), or nullptr
     double f10;         // f at (1, 0)
     // set inside the stage kernel: LDS copies of column m of Cp / Cm ([l])
@@ -352,16 +352,16 @@ struct EpiBveAdjS {
     size_t member_stride;
     const double* nulap;
     const double* lap;
-    const double* Cp;
-    const double* Cm;
+    const double* Cp;          // column-major [m][l]
+    const double* Cm;          // column-major [m][l]
     const double2* fwd_next;   // forward stage state of the next adjoint stage, or nullptr
     Rk4Adj rk;
     __device__ double2 st_term(const double2* x, int l, int m) const {
         const int n = L + 1;
         double2 v = c2(0.0, 0.0);
         if (l >= m) {
-            if (l + 1 <= L) v = cadd(v, cscale(x[l + 1], Cp[l * n + m]));
-            if (l - 1 >= m) v = cadd(v, cscale(x[l - 1], Cm[l * n + m]));
+            if (l + 1 <= L) v = cadd(v, cscale(x[l + 1], Cp[m * n + l]));
+            if (l - 1 >= m) v = cadd(v, cscale(x[l - 1], Cm[m * n + l]));
         }
         return v;
     }
@@ -642,8 +642,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     }
                 }
                 const int tc = (t < n) ? t : n - 1;
-                vc[u][0] = a.k.Cp[(size_t)tc * n + m];
-                vc[u][1] = a.k.Cm[(size_t)tc * n + m];
+                vc[u][0] = a.k.Cp[(size_t)m * n + tc];
+                vc[u][1] = a.k.Cm[(size_t)m * n + tc];
                 vc[u][2] = a.k.lap ? a.k.lap[tc] : 0.0;
                 vr[u] = c2(0.0, 0.0);
                 if constexpr (SYN && NS == 0) vr[u] = g.rec[(size_t)m * n + tc];   // FP64 recurrence

@@ -81,6 +81,8 @@ struct aeo_plan_s {
     // Spectral constants ((L+1)^2 real each; lap has L+1 entries)
     double* d_Cp = nullptr;
     double* d_Cm = nullptr;
+    double* d_CpT = nullptr;       // the same, column-major [m][l] (stage kernels)
+    double* d_CmT = nullptr;
     // generic scratch for composed operations: [fields][(L+1)^2] complex
     double2* d_spec = nullptr;
 

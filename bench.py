@@ -133,14 +133,15 @@ def _stage_bytes(cls, L, nlat, nslices):
     tables and intermediates read or written once, state passes.  FP64
     synthesis reads no table rows beyond the recurrence seeds (2 rows per
     degree block, 4 blocks) and the (a, b) recurrence and C+/C- columns;
-    split-precision synthesis reads its digit table (2 nslices bytes/entry)."""
+    split-precision synthesis reads its digit table (nslices INT8 digits per
+    entry)."""
     nh = (nlat + 1) // 2
     n = L + 1
     B = (L + 1) * (L + 2) // 2
     cut = (2 * L) // 3
     Bc = (cut + 1) * (cut + 2) // 2
     spec = n * n * 16
-    tab = 8 if nslices == 0 else 2 * nslices      # bytes per table entry
+    tab = 8 if nslices == 0 else nslices          # bytes per table entry
     eo = 2 * 4 * nh * n * 16                      # E/O, 4 fields
     sd = 2 * 1 * nh * (cut + 1) * 16              # S/D, 1 field, m <= cut
     cols = n * n * (16 + 8 + 8)                   # (a, b), C+, C- columns
