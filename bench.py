@@ -64,6 +64,12 @@ def _args():
                    help="force the headline precision mode (0 = FP64, k = split slices)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 4 device copies of 256 MiB (scales profiler byte counters)")
+    p.add_argument("--ensemble-only", action="store_true",
+                   help="only the 256-member uneven-L ensemble, graph replay (for profiler "
+                        "runs; precision from --precision, default 6 slices)")
+    p.add_argument("--tiles", type=str, default=None,
+                   help="members per workgroup for the L = 21,42,85,170 classes, e.g. 8,8,8,4 "
+                        "(default: autotune)")
     return p.parse_args()
 
 
@@ -268,6 +274,35 @@ def ensemble_modes(steps, ns_split):
     return res
 
 
+def ensemble_only(args):
+    """The ensemble workload alone: `--steps` graph-replayed RK4 steps of the
+    256 members after `--warmup`, one JSON line (profiler runs)."""
+    import torch
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    classes = [(21, 32, 64), (42, 64, 128), (85, 128, 256), (170, 256, 512)]
+    nper = 64
+    ns = 6 if args.precision is None else args.precision
+    planets = [_planet(L, a, b) for L, a, b in classes]
+    states = [torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(1000 * L + k),
+                                                     rms=5e-5) for k in range(nper)]),
+                              device="cuda") for L, _, _ in classes]
+    ens = Ensemble([(pl, nper) for pl in planets], precision=ns)
+    if args.tiles:
+        tiles = [int(t) for t in args.tiles.split(",")]
+        for m, t in zip(ens.models, tiles):
+            m.set_tiles(t)
+    else:
+        tiles = [t["mb"] for t in ens.autotune(states, DT, steps=2)]
+    outs = [torch.empty_like(s) for s in states]
+    ms = _timed(lambda k: ens.rk4(states, DT, k, out=outs, use_graph=True), args.steps,
+                args.warmup)
+    print(json.dumps({"workload": "BVE ensemble, 64 members each at L = 21/42/85/170",
+                      "precision": _mode_name(ns), "members_per_block": tiles,
+                      "steps": args.steps, "ms_per_ensemble_step": ms,
+                      "member_steps_per_s": 4 * nper * 1000.0 / ms}))
+
+
 def adjoint_cost(model, ns):
     import torch
     from aeolus_amd.run.initial_conditions import spectrum_l15
@@ -339,6 +374,9 @@ def main():
     args = _args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.ensemble_only:
+        ensemble_only(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "every figure of this benchmark is one GPU's; "
