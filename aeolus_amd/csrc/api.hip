@@ -62,7 +62,8 @@ static void free_plan(aeo_plan_s* p) {
                     (void*)p->d_Cm, (void*)p->d_spec, (void*)p->d_Psl, (void*)p->d_offs_ss,
                     (void*)p->d_esyn, (void*)p->d_Pasl, (void*)p->d_offs_sa, (void*)p->d_eana,
                     (void*)p->d_offs_ea, (void*)p->d_xn, (void*)p->d_rec, (void*)p->d_CpT,
-                    (void*)p->d_CmT})
+                    (void*)p->d_CmT, (void*)p->d_Ysl, (void*)p->d_eYs, (void*)p->d_Yasl,
+                    (void*)p->d_eYa})
         dev_free(q);
     delete p;
 }
@@ -284,12 +285,37 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
     return AEO_OK;
 }
 
+static int enable_split_points(aeo_plan_s* plan, int nslices) {
+    const int B = plan->B;
+    const int K2 = rup(2 * B, 256);
+    AEO_REQUIRE(split_exact(nslices, K2) && split_exact(nslices, 1024), AEO_ERR_UNSUPPORTED,
+                "aeo_plan_enable_split: INT32 accumulation would not be exact at this size");
+    if (plan->nslices == nslices) return AEO_OK;
+    for (void* q : {(void*)plan->d_Ysl, (void*)plan->d_eYs, (void*)plan->d_Yasl,
+                    (void*)plan->d_eYa})
+        dev_free(q);
+    plan->d_Ysl = nullptr; plan->d_eYs = nullptr; plan->d_Yasl = nullptr; plan->d_eYa = nullptr;
+    plan->nslices = 0;
+    plan->nprow = rup(plan->npts, 16);
+    plan->K2 = K2;
+    plan->R2p = rup(2 * B, 16);
+    plan->Npad = plan->nchunks * 1024;
+    int rc;
+    if ((rc = dev_alloc(&plan->d_Ysl, (size_t)nslices * plan->nprow * plan->K2))) return rc;
+    if ((rc = dev_alloc(&plan->d_eYs, (size_t)plan->nprow))) return rc;
+    if ((rc = dev_alloc(&plan->d_Yasl, (size_t)nslices * plan->R2p * plan->Npad))) return rc;
+    if ((rc = dev_alloc(&plan->d_eYa, (size_t)plan->R2p))) return rc;
+    if ((rc = points_build_split_tables(plan, nslices, nullptr))) return rc;
+    AEO_CHECK_HIP(hipDeviceSynchronize());
+    plan->nslices = nslices;
+    return AEO_OK;
+}
+
 int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
     AEO_REQUIRE(plan != nullptr, AEO_ERR_INVALID, "aeo_plan_enable_split: plan is NULL");
-    AEO_REQUIRE(plan->kind == kGauss, AEO_ERR_UNSUPPORTED,
-                "aeo_plan_enable_split: split-precision contractions need a Gauss plan");
     AEO_REQUIRE(nslices >= 2 && nslices <= 8, AEO_ERR_INVALID,
                 "aeo_plan_enable_split: nslices must be in [2, 8]");
+    if (plan->kind == kPoints) return enable_split_points(plan, nslices);
     const int L = plan->L, nh = plan->nh;
     AEO_REQUIRE(split_exact(nslices, split_kp(L, 0)) && split_exact(nslices, nh),
                 AEO_ERR_UNSUPPORTED,
@@ -327,7 +353,7 @@ int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
 
 int aeo_model_set_precision(aeo_model_t model, int nslices) {
     AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_set_precision: model is NULL");
-    AEO_REQUIRE(nslices == 0 || (model->plan->kind == kGauss && model->plan->nslices == nslices),
+    AEO_REQUIRE(nslices == 0 || model->plan->nslices == nslices,
                 AEO_ERR_STATE,
                 "aeo_model_set_precision: enable the split tables on the plan first "
                 "(aeo_plan_enable_split with the same nslices)");

@@ -38,6 +38,15 @@ struct PointsDev {
     const double2* Yp;        // [npts][B]
     const double* w;          // npts quadrature weights
     const double* coslat;     // npts cos(latitude), unclamped
+    // Split-precision tables (nslices > 0): INT8 digits of the real GEMM
+    // operands with one exponent per row (see split.hpp).
+    //   synthesis  A[i][k'] = (Re Y_ik', -Im Y_i(k'-B)),  Ysl [slice][nprow][K2]
+    //   analysis   A[r][i]  = (Re Y_ir, -Im Y_i(r-B)),   Yasl [slice][R2p][Npad]
+    int nslices, nprow, K2, R2p, Npad;
+    const signed char* Ysl;
+    const int* eYs;           // [nprow]
+    const signed char* Yasl;
+    const int* eYa;           // [R2p]
 };
 
 }  // namespace aeo
@@ -85,6 +94,12 @@ struct aeo_plan_s {
     double* d_CmT = nullptr;
     // generic scratch for composed operations: [fields][(L+1)^2] complex
     double2* d_spec = nullptr;
+    // Points split-precision tables (nslices > 0; layouts in PointsDev)
+    int nprow = 0, K2 = 0, R2p = 0, Npad = 0;
+    signed char* d_Ysl = nullptr;
+    int* d_eYs = nullptr;
+    signed char* d_Yasl = nullptr;
+    int* d_eYa = nullptr;
 
     aeo::GaussDev gdev() const {
         aeo::GaussDev g;
@@ -100,6 +115,8 @@ struct aeo_plan_s {
         aeo::PointsDev p;
         p.L = L; p.B = B; p.npts = npts;
         p.Yk = d_Yk; p.Yp = d_Yp; p.w = d_w; p.coslat = d_pcoslat;
+        p.nslices = nslices; p.nprow = nprow; p.K2 = K2; p.R2p = R2p; p.Npad = Npad;
+        p.Ysl = d_Ysl; p.eYs = d_eYs; p.Yasl = d_Yasl; p.eYa = d_eYa;
         return p;
     }
 };
@@ -218,6 +235,7 @@ int gauss_adjoint_stage(aeo_model_s* m, const double2* kb, int stage, double dt,
                         const double2* fwd_next, hipStream_t s);
 
 int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s);
+int points_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s);
 int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s);
 int points_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s);
 int points_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, double2* out,

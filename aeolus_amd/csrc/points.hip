@@ -7,6 +7,7 @@
 //   synthesis  f_i = 2 Re(sum_k Y_ik a_k) - Re(sum_{m=0} Y_ik a_k)
 //                  = sum_k c_k (Re Y_ik Re a_k - Im Y_ik Im a_k),  c = 1 (m = 0), 2 (m > 0)
 #include "plan.hpp"
+#include "split.hpp"
 #include "stage.hpp"
 
 namespace aeo {
@@ -195,6 +196,211 @@ __global__ void points_finish_kernel(int L, int B, int nchunks, int nf_part, int
 }
 
 // ---------------------------------------------------------------------------
+// Split precision (INT8 digit MFMA, exact INT32 sums, FP64 recombination; the
+// digit scheme is the one of the Gauss path, split.hpp).  The dense complex
+// contractions are written as real GEMMs:
+//   synthesis  f[i][f] = sum_k' A[i][k'] C[k'][f],  A = (Re Y, -Im Y) over
+//              k' < 2B, C = (Re, Im) of the c_k-scaled coefficients;
+//   analysis   part[r][f] = sum_i A[r][i] g[i][f],  A rows (Re Y_k, -Im Y_k),
+//              r < 2B = (Re, Im) of conj(Y) g summed over one point chunk.
+// Tables (one-time build): one thread per row, exponent over the whole row.
+// ---------------------------------------------------------------------------
+__global__ void points_split_table_kernel(PointsDev p, int ns, signed char* __restrict__ Ysl,
+                                          int* __restrict__ eYs, signed char* __restrict__ Yasl,
+                                          int* __restrict__ eYa) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int B = p.B, n = p.npts;
+    int d[8];
+    if (r < p.nprow) {   // synthesis row = point r; Yk[k' * n + r] covers both planes
+        double mx = 0.0;
+        if (r < n)
+            for (int k = 0; k < 2 * B; ++k) mx = fmax(mx, fabs(p.Yk[(size_t)k * n + r]));
+        const int e = exp_above(mx);
+        eYs[r] = e;
+        for (int k = 0; k < p.K2; ++k) {
+            const double v = (r < n && k < 2 * B)
+                                 ? ((k < B) ? 1.0 : -1.0) * p.Yk[(size_t)k * n + r] : 0.0;
+            digits_rt(v, e, ns, d);
+            for (int q = 0; q < ns; ++q) Ysl[((size_t)q * p.nprow + r) * p.K2 + k] = (signed char)d[q];
+        }
+    }
+    if (r < p.R2p) {     // analysis row r: Re Y_r (r < B) or -Im Y_(r-B) over the points
+        const double sg = (r < B) ? 1.0 : -1.0;
+        double mx = 0.0;
+        if (r < 2 * B)
+            for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(p.Yk[(size_t)r * n + i]));
+        const int e = exp_above(mx);
+        eYa[r] = e;
+        for (int i = 0; i < p.Npad; ++i) {
+            const double v = (r < 2 * B && i < n) ? sg * p.Yk[(size_t)r * n + i] : 0.0;
+            digits_rt(v, e, ns, d);
+            for (int q = 0; q < ns; ++q) Yasl[((size_t)q * p.R2p + r) * p.Npad + i] = (signed char)d[q];
+        }
+    }
+}
+
+constexpr int kPtsSplitKC = 256;   // K values per LDS digit pass
+
+// Max exponent of 16 columns (one wave per column, lanes over K).
+template <class F>
+__device__ inline void pts_col_exp(int len, int* ex, const F& val) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = wave; c < 16; c += kPtsThreads / 64) {
+        double mx = 0.0;
+        for (int k = lane; k < len; k += 64) mx = fmax(mx, fabs(val(c, k)));
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if (lane == 0) ex[c] = exp_above(mx);
+    }
+}
+
+// Digits of 16 columns x kPtsSplitKC K values (pairs over threads) into
+// xs[slice][16][kPtsSplitKC].
+template <int NS, class F>
+__device__ inline void pts_digits(int k0, const int* ex, unsigned short* xsh, const F& val) {
+    constexpr int KP = kPtsSplitKC / 2;
+    for (int t = threadIdx.x; t < 16 * KP; t += kPtsThreads) {
+        const int c = t / KP, kk = 2 * (t - c * KP);
+        int d0[NS], d1[NS];
+        digits<NS>(val(c, k0 + kk), ex[c], d0);
+        digits<NS>(val(c, k0 + kk + 1), ex[c], d1);
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            xsh[(((size_t)q * 16 + c) * kPtsSplitKC + kk) >> 1] = digit_pair(d0[q], d1[q]);
+    }
+}
+
+// One 16-row tile x one kPtsSplitKC pass of MFMAs: rows of `tab` (row stride
+// ld, K offset k0) against xs.
+template <int NS>
+__device__ inline void pts_mfma_pass(const signed char* tab, size_t slice_stride, size_t ld,
+                                     int row0, int k0, const signed char* xs, intx4* acc) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c4 = 0; c4 < kPtsSplitKC / kSplitK; ++c4) {
+        intx4 av[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            av[q] = *reinterpret_cast<const intx4*>(tab + q * slice_stride +
+                                                    (size_t)(row0 + (lane & 15)) * ld + k0 +
+                                                    c4 * kSplitK + (lane >> 4) * 16);
+#pragma unroll
+        for (int t2 = 0; t2 < NS; ++t2) {
+            const intx4 bv = *reinterpret_cast<const intx4*>(
+                xs + ((size_t)t2 * 16 + (lane & 15)) * kPtsSplitKC + c4 * kSplitK + (lane >> 4) * 16);
+#pragma unroll
+            for (int q = 0; q + t2 < NS; ++q)
+                acc[q + t2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[q], bv, acc[q + t2], 0, 0, 0);
+        }
+    }
+}
+
+template <int NS>
+__device__ inline double pts_recombine(const intx4* acc, int r, int e) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = NS - 1; q >= 0; --q) v = fma(v, kInvRadix, (double)acc[q][r]);
+    return ldexp(v, e) * kInvTop2;
+}
+
+// Synthesis + pointwise stage: workgroup = 4 waves x 16 points; the loader's
+// NF <= 16 fields are the 16 MFMA columns.
+template <class Loader, class Pw, int NS>
+__global__ __launch_bounds__(kPtsThreads) void points_syn_split_kernel(PointsDev p, Loader ld,
+                                                                      Pw pw) {
+    constexpr int NF = Loader::NF;
+    static_assert(NF <= 16, "one 16-column MFMA tile");
+    extern __shared__ double2 lds[];
+    const int B = p.B, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double2* cv = lds;                                              // [NF][B]
+    double* outv = reinterpret_cast<double*>(cv + (size_t)NF * B);  // [4][16][NF]
+    int* fexp = reinterpret_cast<int*>(outv + 4 * 16 * NF);          // [16]
+    signed char* xs = reinterpret_cast<signed char*>(fexp + 16);     // [NS][16][KC]
+    for (int t = threadIdx.x; t < NF * B; t += kPtsThreads) {
+        const int f = t / B, k = t - f * B;
+        int l, m;
+        k_to_lm(k, l, m);
+        const double2 v = ld.value(f, l, m);
+        cv[t] = (m == 0) ? v : cscale(v, 2.0);
+    }
+    __syncthreads();
+    auto col = [&](int c, int k) -> double {
+        if (c >= NF || k >= 2 * B) return 0.0;
+        return (k < B) ? cv[c * B + k].x : cv[c * B + k - B].y;
+    };
+    pts_col_exp(2 * B, fexp, col);
+    __syncthreads();
+    const int row0 = (blockIdx.x * 4 + wave) * 16;
+    intx4 acc[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc[q] = intx4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < p.K2; k0 += kPtsSplitKC) {
+        pts_digits<NS>(k0, fexp, reinterpret_cast<unsigned short*>(xs), col);
+        __syncthreads();
+        if (row0 < p.nprow)
+            pts_mfma_pass<NS>(p.Ysl, (size_t)p.nprow * p.K2, p.K2, row0, k0, xs, acc);
+        __syncthreads();
+    }
+    // recombine: lane = (column f, 4 rows); gather each point's fields in LDS
+    double* ow = outv + (size_t)wave * 16 * NF;
+    const int f = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ir = (lane >> 4) * 4 + r;
+        const int i = row0 + ir;
+        if (f < NF && i < p.nprow) ow[ir * NF + f] = pts_recombine<NS>(acc, r, p.eYs[i] + fexp[f]);
+    }
+    __syncthreads();
+    if (lane < 16) {
+        const int i = row0 + lane;
+        if (i < p.npts) pw.template apply<NF>(i, ow + lane * NF, p.w[i], p.coslat[i]);
+    }
+}
+
+// Partial analysis sums of one point chunk: grid (R2p / 64, nchunks); wave =
+// 16 rows of the 2B real rows, columns = the nf <= 16 fields; per-chunk
+// column exponents (the partials are recombined per chunk in FP64).
+template <int NS>
+__global__ __launch_bounds__(kPtsThreads) void points_partial_split_kernel(
+    PointsDev p, const double* __restrict__ vals, const double* __restrict__ weights, int nf,
+    double2* __restrict__ part) {
+    __shared__ int gexp[16];
+    __shared__ __align__(16) signed char xs[NS * 16 * kPtsSplitKC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int B = p.B, chunk = blockIdx.y;
+    const int i0 = chunk * kPtsChunk;
+    auto col = [&](int c, int i) -> double {
+        const int ii = i0 + i;
+        if (c >= nf || i >= kPtsChunk || ii >= p.npts) return 0.0;
+        const double v = vals[(size_t)c * p.npts + ii];
+        return weights ? v * weights[ii] : v;
+    };
+    pts_col_exp(kPtsChunk, gexp, col);
+    __syncthreads();
+    const int row0 = (blockIdx.x * 4 + wave) * 16;
+    intx4 acc[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc[q] = intx4{0, 0, 0, 0};
+    for (int s0 = 0; s0 < kPtsChunk; s0 += kPtsSplitKC) {
+        pts_digits<NS>(s0, gexp, reinterpret_cast<unsigned short*>(xs), col);
+        __syncthreads();
+        if (row0 < p.R2p)
+            pts_mfma_pass<NS>(p.Yasl, (size_t)p.R2p * p.Npad, p.Npad, row0, i0 + s0, xs, acc);
+        __syncthreads();
+    }
+    double* pd = reinterpret_cast<double*>(part);
+    const int f = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int rr = row0 + (lane >> 4) * 4 + r;
+        if (f < nf && rr < 2 * B) {
+            const int k = (rr < B) ? rr : rr - B;
+            pd[2 * (((size_t)chunk * nf + f) * B + k) + (rr >= B)] =
+                pts_recombine<NS>(acc, r, p.eYa[rr] + gexp[f]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 template <class Loader, class Pw>
@@ -220,6 +426,70 @@ static int launch_partial(const aeo_plan_s* p, const double* vals, const double*
     return AEO_OK;
 }
 
+template <class Loader, class Pw, int NS>
+static int launch_psyn_split_ns(const aeo_plan_s* p, const Loader& ld, const Pw& pw,
+                                hipStream_t s) {
+    constexpr int NF = Loader::NF;
+    const size_t lds = sizeof(double2) * NF * p->B + sizeof(double) * 4 * 16 * NF +
+                       sizeof(int) * 16 + (size_t)NS * 16 * kPtsSplitKC;
+    AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
+                "split point synthesis: LDS demand exceeds 160 KiB at this l_max");
+    auto kern = points_syn_split_kernel<Loader, Pw, NS>;
+    if (lds > 65536)
+        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((p->nprow + 63) / 64), dim3(kPtsThreads), lds, s, p->pdev(),
+                       ld, pw);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+template <int NS>
+static int launch_partial_split_ns(const aeo_plan_s* p, const double* vals, const double* w,
+                                   int nf, double2* part, hipStream_t s) {
+    AEO_REQUIRE(nf <= 16, AEO_ERR_UNSUPPORTED, "split point analysis: at most 16 fields");
+    hipLaunchKernelGGL(points_partial_split_kernel<NS>, dim3((p->R2p + 63) / 64, p->nchunks),
+                       dim3(kPtsThreads), 0, s, p->pdev(), vals, w, nf, part);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+// ns = 0: FP64 contractions; ns in {4, 5, 6, 7}: split digits (plan tables must match).
+template <class Loader, class Pw>
+static int launch_psyn_ns(const aeo_plan_s* p, int ns, const Loader& ld, const Pw& pw,
+                          hipStream_t s) {
+    if (ns != 0)
+        AEO_REQUIRE(p->nslices == ns, AEO_ERR_STATE,
+                    "split point synthesis: the plan's digit tables have a different slice count");
+    switch (ns) {
+        case 0: return launch_psyn(p, ld, pw, s);
+        case 4: return launch_psyn_split_ns<Loader, Pw, 4>(p, ld, pw, s);
+        case 5: return launch_psyn_split_ns<Loader, Pw, 5>(p, ld, pw, s);
+        case 6: return launch_psyn_split_ns<Loader, Pw, 6>(p, ld, pw, s);
+        case 7: return launch_psyn_split_ns<Loader, Pw, 7>(p, ld, pw, s);
+        default:
+            set_error("split point synthesis: nslices must be 4, 5, 6 or 7");
+            return AEO_ERR_UNSUPPORTED;
+    }
+}
+
+static int launch_partial_ns(const aeo_plan_s* p, int ns, const double* vals, const double* w,
+                             int nf, double2* part, hipStream_t s) {
+    if (ns != 0)
+        AEO_REQUIRE(p->nslices == ns, AEO_ERR_STATE,
+                    "split point analysis: the plan's digit tables have a different slice count");
+    switch (ns) {
+        case 0: return launch_partial(p, vals, w, nf, part, s);
+        case 4: return launch_partial_split_ns<4>(p, vals, w, nf, part, s);
+        case 5: return launch_partial_split_ns<5>(p, vals, w, nf, part, s);
+        case 6: return launch_partial_split_ns<6>(p, vals, w, nf, part, s);
+        case 7: return launch_partial_split_ns<7>(p, vals, w, nf, part, s);
+        default:
+            set_error("split point analysis: nslices must be 4, 5, 6 or 7");
+            return AEO_ERR_UNSUPPORTED;
+    }
+}
+
 template <class Epi>
 static int launch_finish(const aeo_plan_s* p, int nf_part, int nf_valid, const double2* part,
                          int lmax_a, int mmax_a, const Epi& ep, hipStream_t s) {
@@ -233,6 +503,14 @@ static int launch_finish(const aeo_plan_s* p, int nf_part, int nf_valid, const d
 int points_build_tables(aeo_plan_s* p, const double* d_lat, const double* d_lon, hipStream_t s) {
     hipLaunchKernelGGL(points_basis_kernel, dim3((p->npts + 255) / 256), dim3(256), 0, s,
                        p->npts, p->L, d_lat, d_lon, p->d_Yk, p->d_Yp);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+int points_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s) {
+    const int rows = (p->nprow > p->R2p) ? p->nprow : p->R2p;
+    hipLaunchKernelGGL(points_split_table_kernel, dim3((rows + 63) / 64), dim3(64), 0, s,
+                       p->pdev(), ns, p->d_Ysl, p->d_eYs, p->d_Yasl, p->d_eYa);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
@@ -290,21 +568,22 @@ int points_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     SpecK k{L, md->R, p->d_Cp, p->d_Cm};
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     Rk4 rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
+    const int ns = md->nslices;
     if (md->kind == kBVE) {
         JacLoader<true> ld{ys, nullptr, true, md->d_lap, f10, k};
         PwJac pw{md->d_gw, md->R};
-        int rc = launch_psyn(p, ld, pw, s);
+        int rc = launch_psyn_ns(p, ns, ld, pw, s);
         if (rc) return rc;
-        rc = launch_partial(p, md->d_gw, nullptr, 1, md->d_part, s);
+        rc = launch_partial_ns(p, ns, md->d_gw, nullptr, 1, md->d_part, s);
         if (rc) return rc;
         EpiBve ep{L, ys, forcing, md->d_nulap, rk};
         return launch_finish(p, 1, 1, md->d_part, cut, cut, ep, s);
     }
     SweLoader<true> ld{ys, md->d_lap, f10, k};
     PwSwe pw{md->d_gw, p->npts, md->R};
-    int rc = launch_psyn(p, ld, pw, s);
+    int rc = launch_psyn_ns(p, ns, ld, pw, s);
     if (rc) return rc;
-    rc = launch_partial(p, md->d_gw, nullptr, 4, md->d_part, s);
+    rc = launch_partial_ns(p, ns, md->d_gw, nullptr, 4, md->d_part, s);
     if (rc) return rc;
     EpiSwe ep{L, ys, md->d_lap, md->nu4 > 0.0 ? md->d_nu4lap2 : nullptr,
               md->has_topo ? md->d_lap_phi_s : nullptr,

@@ -131,6 +131,12 @@ class PointSetSphericalHarmonics(_TransformBase):
     def Y_matrix_real(self) -> torch.Tensor:
         return self.Y_matrix.real
 
+    def enable_split(self, nslices: int) -> None:
+        """Build the INT8 digit tables of the dense basis for split-precision
+        model stages (``set_precision`` of the models on this product space)."""
+        nat.check(nat.lib().aeo_plan_enable_split(self._plan, int(nslices)), "enable_split")
+        self.nslices = int(nslices)
+
 
 def _to_numpy(x) -> np.ndarray:
     if isinstance(x, torch.Tensor):

@@ -132,7 +132,7 @@ class ShallowWaterModel:
             self._model = ctypes.c_void_p()
 
     def set_precision(self, nslices: int = 0) -> None:
-        """0: FP64 Legendre contractions; k in {4..7}: split INT8 MFMA digits.
+        """0: FP64 Legendre / Y-basis contractions; k in {4..7}: split INT8 MFMA digits.
 
         Builds the product plan's digit tables on first use.  Everything
         else of the step stays FP64."""

@@ -91,12 +91,13 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
 
 int aeo_plan_destroy(aeo_plan_t plan);
 
-/* Build the split-precision tables of a Gauss plan: the FP64 Legendre tables
- * cut into `nslices` INT8 radix-254 digits per entry with one power-of-two
- * exponent per table row (2 <= nslices <= 8).  Models on this plan can then
- * run their Legendre contractions as INT8 MFMA products with exact INT32
- * accumulation and FP64 recombination (aeo_model_set_precision).  No
- * counterpart in the reference (its transforms are FP64 only). */
+/* Build the split-precision tables of a plan: the FP64 Legendre tables (Gauss)
+ * or the dense Y basis (point sets) cut into `nslices` INT8 radix-254 digits
+ * per entry with one power-of-two exponent per table row (2 <= nslices <= 8).
+ * Models on this plan can then run their Legendre / Y-basis contractions as
+ * INT8 MFMA products with exact INT32 accumulation and FP64 recombination
+ * (aeo_model_set_precision).  No counterpart in the reference (its transforms
+ * are FP64 only). */
 int aeo_plan_enable_split(aeo_plan_t plan, int nslices);
 
 /* Query: l_max, number of grid points, number of basis functions. */
@@ -160,7 +161,7 @@ int aeo_swe_create(aeo_plan_t prod, double radius, double omega, double gravity,
 
 int aeo_model_destroy(aeo_model_t model);
 
-/* 0 = FP64 Legendre contractions; k > 0 = split INT8 digits with k slices
+/* 0 = FP64 Legendre / Y-basis contractions; k > 0 = split INT8 digits with k slices
  * (the plan must have aeo_plan_enable_split(plan, k)).  Everything else of the
  * step (recurrence tables, FFTs, products, spectral algebra, RK4 update)
  * stays FP64.  No counterpart in the reference. */
