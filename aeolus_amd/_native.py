@@ -19,6 +19,7 @@ AEO_ERR_HIP = -3
 AEO_ERR_STATE = -4
 
 OP_LAP, OP_INVLAP, OP_DLAMBDA, OP_SINTH_DTH, OP_SINTH_DTH_T, OP_TRUNC23, OP_INVLAP_ZERO = range(7)
+GRID_STATS_BVE, GRID_STATS_SWE, GRID_STATS_MINMAX = range(3)
 
 _c_int = ctypes.c_int
 _c_double = ctypes.c_double
@@ -55,6 +56,7 @@ _SIGNATURES = {
     "aeo_model_rk4_adjoint": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp],
     "aeo_model_tendency_adjoint": [_vp, _vp, _vp, _vp, _vp],
     "aeo_spectral_diag": [_c_int, _c_double, _c_double, _vp, _c_int, _vp, _vp],
+    "aeo_grid_stats": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_model_profile": [_vp, _c_int, _c_int],
     "aeo_ensemble_create": [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "aeo_ensemble_destroy": [_vp],

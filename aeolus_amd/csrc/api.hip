@@ -895,4 +895,21 @@ int aeo_spectral_diag(int l_max, double radius, double omega, const double* zeta
                                 nfields, out, as_stream(stream));
 }
 
+int aeo_grid_stats(int mode, int npts, const double* w, const double* a, const double* b,
+                   const double* c, const double* d, const double* k, double* out, void* stream) {
+    AEO_REQUIRE(out && a && npts >= 1, AEO_ERR_INVALID, "aeo_grid_stats: bad arguments");
+    AEO_REQUIRE(mode == AEO_GRID_STATS_BVE || mode == AEO_GRID_STATS_SWE ||
+                    mode == AEO_GRID_STATS_MINMAX,
+                AEO_ERR_INVALID, "aeo_grid_stats: unknown mode");
+    if (mode == AEO_GRID_STATS_BVE)
+        AEO_REQUIRE(w && b && c, AEO_ERR_INVALID, "aeo_grid_stats: BVE needs w, zeta, u, v");
+    if (mode == AEO_GRID_STATS_SWE) {
+        AEO_REQUIRE(w && b && c && k, AEO_ERR_INVALID,
+                    "aeo_grid_stats: SWE needs w, u, v, phi and the constants k");
+        AEO_REQUIRE(k[2] != 0.0, AEO_ERR_INVALID, "aeo_grid_stats: gravity must be nonzero");
+    }
+    const double k0 = k ? k[0] : 0.0, k1 = k ? k[1] : 0.0, k2 = k ? k[2] : 1.0;
+    return launch_grid_stats(mode, npts, w, a, b, c, d, k0, k1, k2, out, as_stream(stream));
+}
+
 }  // extern "C"

@@ -252,6 +252,9 @@ int launch_scale(const double2* in, double2* out, double alpha, size_t n, hipStr
 int launch_transpose(const double2* in, double2* out, int n, int nmat, hipStream_t s);
 int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf, double* out,
                          hipStream_t s);
+int launch_grid_stats(int mode, int npts, const double* w, const double* a, const double* b,
+                      const double* c, const double* d, double k0, double k1, double k2,
+                      double* out, hipStream_t s);
 int launch_velocity_points(int n, int row_len, double R, const double* coslat,
                            const double* g_s, const double* g_lam, double* u, double* v, int mode,
                            hipStream_t s);

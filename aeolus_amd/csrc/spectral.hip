@@ -151,6 +151,98 @@ int launch_spectral_diag(int L, double R, double omega, const double2* z, int nf
     return AEO_OK;
 }
 
+// Per-step grid statistics of one sampling, one workgroup (fixed reduction
+// order, compensated sums; run/bve/diagnostics.py:162-211,
+// run/swe/diagnostics.py:113-173, phy/shallow_water.py:456-484).
+//   AEO_GRID_STATS_BVE: a = zeta, b = u, c = v
+//     out = {max |zeta|, sum w zeta^2, sum w, max sqrt(u^2 + v^2)}
+//   AEO_GRID_STATS_SWE: a = u, b = v, c = phi (anomaly), d = phi_s or null;
+//     k = {Phi0, mean phi_s, gravity}; Phi = Phi0 + phi
+//     out = {max sqrt(u^2 + v^2), min Phi, max Phi,
+//            sum w [Phi (u^2 + v^2)/2 + Phi^2/2 + Phi phi_s],
+//            min eta, max eta},  eta = (Phi - Phi0 + phi_s - mean phi_s) / g
+//   AEO_GRID_STATS_MINMAX: a = field; out = {min a, max a}
+constexpr int kStatsThreads = 512;
+
+__global__ __launch_bounds__(kStatsThreads) void grid_stats_kernel(
+    int mode, int npts, const double* __restrict__ w, const double* __restrict__ a,
+    const double* __restrict__ b, const double* __restrict__ c, const double* __restrict__ d,
+    double k0, double k1, double k2, double* __restrict__ out) {
+    // per thread: 4 extrema (e0 max, e1 min, e2 max, e3 min/max by mode) and 2 sums
+    double x0 = 0.0, x1 = INFINITY, x2 = -INFINITY, x3 = INFINITY, x4 = -INFINITY;
+    CSum s0 = {0, 0}, s1 = {0, 0};
+    for (int i = threadIdx.x; i < npts; i += kStatsThreads) {
+        if (mode == AEO_GRID_STATS_BVE) {
+            const double z = a[i], wi = w[i];
+            x0 = fmax(x0, fabs(z));
+            x4 = fmax(x4, b[i] * b[i] + c[i] * c[i]);
+            s0 = cs_add(s0, wi * z * z);
+            s1 = cs_add(s1, wi);
+        } else if (mode == AEO_GRID_STATS_SWE) {
+            const double u = a[i], v = b[i], ph = k0 + c[i], sp = d ? d[i] : 0.0;
+            const double q = u * u + v * v;
+            x0 = fmax(x0, q);
+            x1 = fmin(x1, ph);
+            x2 = fmax(x2, ph);
+            double e = 0.5 * ph * q + 0.5 * ph * ph;
+            if (d) e += ph * sp;
+            s0 = cs_add(s0, w[i] * e);
+            const double eta = d ? (ph - k0 + (sp - k1)) / k2 : (ph - k0) / k2;
+            x3 = fmin(x3, eta);
+            x4 = fmax(x4, eta);
+        } else {
+            x1 = fmin(x1, a[i]);
+            x2 = fmax(x2, a[i]);
+        }
+    }
+    __shared__ double r0[kStatsThreads], r1[kStatsThreads], r2[kStatsThreads], r3[kStatsThreads],
+        r4[kStatsThreads];
+    __shared__ CSum q0[kStatsThreads], q1[kStatsThreads];
+    const int t = threadIdx.x;
+    r0[t] = x0; r1[t] = x1; r2[t] = x2; r3[t] = x3; r4[t] = x4;
+    q0[t] = s0; q1[t] = s1;
+    __syncthreads();
+    for (int h = kStatsThreads / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            r0[t] = fmax(r0[t], r0[t + h]);
+            r1[t] = fmin(r1[t], r1[t + h]);
+            r2[t] = fmax(r2[t], r2[t + h]);
+            r3[t] = fmin(r3[t], r3[t + h]);
+            r4[t] = fmax(r4[t], r4[t + h]);
+            q0[t] = cs_merge(q0[t], q0[t + h]);
+            q1[t] = cs_merge(q1[t], q1[t + h]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (mode == AEO_GRID_STATS_BVE) {
+            out[0] = r0[0];
+            out[1] = q0[0].s + q0[0].c;
+            out[2] = q1[0].s + q1[0].c;
+            out[3] = sqrt(r4[0]);
+        } else if (mode == AEO_GRID_STATS_SWE) {
+            out[0] = sqrt(r0[0]);
+            out[1] = r1[0];
+            out[2] = r2[0];
+            out[3] = q0[0].s + q0[0].c;
+            out[4] = r3[0];
+            out[5] = r4[0];
+        } else {
+            out[0] = r1[0];
+            out[1] = r2[0];
+        }
+    }
+}
+
+int launch_grid_stats(int mode, int npts, const double* w, const double* a, const double* b,
+                      const double* c, const double* d, double k0, double k1, double k2,
+                      double* out, hipStream_t s) {
+    hipLaunchKernelGGL(grid_stats_kernel, dim3(1), dim3(kStatsThreads), 0, s, mode, npts, w, a,
+                       b, c, d, k0, k1, k2, out);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
 // mode 0: velocity (num/spectral_operators.py:517-538):
 //   u = g_s / (R c), v = g_lam / c, with |c| < 1e-6 -> 0
 // mode 1: gradient (num/spectral_operators.py:542-558):

@@ -203,6 +203,43 @@ class ShallowWaterModel:
         return res
 
     # -- validation (phy/shallow_water.py:428-539) -------------------------
+    def _stats(self, state: ShallowWaterState, w=None):
+        """Device statistics of one step (one host read): the SWE grid stats of
+        the state sampling (aeo_grid_stats) and the min/max of phi on the
+        product sampling when it differs."""
+        from ..run.diagnostics import grid_stats
+        phi_c = state.coeffs[PHI]
+        u, v = self.wind_on_state_grid(state)
+        phi_g = self.sh.inv_transform(phi_c)
+        mean_phi_s = (float(self.phi_s_lm[0, 0].real) / math.sqrt(4.0 * math.pi)
+                      if self._phi_s_state is not None else 0.0)
+        w = torch.as_tensor(self.sh.weights, dtype=torch.float64) if w is None else w
+        gs = grid_stats(nat.GRID_STATS_SWE, w, u, v, phi_g, self._phi_s_state,
+                        k=(self.phi0, mean_phi_s, self.gravity))
+        if self._ps.sh is not self.sh:
+            mm = grid_stats(nat.GRID_STATS_MINMAX, None, self._ps.sh.inv_transform(phi_c))[:2]
+        else:
+            mm = gs[1:3] - self.phi0
+        return u, v, phi_g, gs, mm
+
+    def characteristic_fields(self, state: ShallowWaterState) -> dict:
+        """State-grid u, v, wind speed and total geopotential, and the
+        total-geopotential extrema over every sampling
+        (``phy/shallow_water.py:456-468``)."""
+        u, v, phi_g, gs, mm = self._stats(state)
+        h = torch.cat([gs[1:3], mm]).cpu().tolist()
+        ext = (min(h[0], self.phi0 + h[2]), max(h[1], self.phi0 + h[3]))
+        return {"u": u, "v": v, "wind_speed": torch.sqrt(u * u + v * v),
+                "phi_total": self.phi0 + phi_g, "phi_total_extrema": ext}
+
+    def max_characteristic_speed(self, state: ShallowWaterState) -> float:
+        """max|u| + sqrt(max(Phi0 + phi)) over every sampling, clamped at 0
+        inside the root (``phy/shallow_water.py:470-484``); one host read."""
+        _, _, _, gs, mm = self._stats(state)
+        h = torch.cat([gs[0:3], mm]).cpu().tolist()
+        phi_max = max(h[2], self.phi0 + h[4])
+        return h[0] + math.sqrt(max(phi_max, 0.0))
+
     def total_geopotential_extrema(self, state: ShallowWaterState) -> tuple[float, float]:
         phi_c = state.coeffs[PHI]
         samplings = [self.sh] + ([self._ps.sh] if self._ps.sh is not self.sh else [])

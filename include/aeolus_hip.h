@@ -246,6 +246,27 @@ int aeo_ensemble_rk4(aeo_ensemble_t ens, const double* const* y, double dt, int 
 int aeo_spectral_diag(int l_max, double radius, double omega, const double* zeta, int nfields,
                       double* out, void* stream);
 
+enum aeo_grid_stats_mode {
+    AEO_GRID_STATS_BVE = 0,    /* a = zeta, b = u, c = v */
+    AEO_GRID_STATS_SWE = 1,    /* a = u, b = v, c = phi anomaly, d = phi_s or NULL */
+    AEO_GRID_STATS_MINMAX = 2  /* a = field */
+};
+
+/* Per-step grid statistics of one sampling of `npts` points with quadrature
+ * weights w (device FP64; one workgroup, fixed reduction order, compensated
+ * sums), written to the device array out:
+ *   BVE:    {max|zeta|, sum w zeta^2, sum w, max |u|}
+ *           (DiagnosticsRecorder.record, run/bve/diagnostics.py:162-211)
+ *   SWE:    k = {Phi0, mean phi_s, gravity}, Phi = Phi0 + phi:
+ *           {max|u|, min Phi, max Phi, sum w [Phi|u|^2/2 + Phi^2/2 + Phi phi_s],
+ *            min eta, max eta}, eta = (Phi - Phi0 + phi_s - mean phi_s)/gravity
+ *           (SWEDiagnosticsRecorder.record, run/swe/diagnostics.py:113-173,
+ *            characteristic_fields, phy/shallow_water.py:456-484)
+ *   MINMAX: {min a, max a} (total_geopotential_extrema, phy/shallow_water.py:404-426)
+ * One device-to-host read of `out` then gives the whole row. */
+int aeo_grid_stats(int mode, int npts, const double* w, const double* a, const double* b,
+                   const double* c, const double* d, const double* k, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

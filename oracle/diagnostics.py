@@ -10,6 +10,13 @@ rounded), the yardstick for the device's compensated sums.
 Pinned by (tests/test_oracle_models.py): the closed forms of
 ``tests/test_diagnostics.py:63-108`` (single mode, circulation, absolute
 enstrophy of f, padding ignored).
+
+``bve_record_row`` / ``swe_record_row`` restate one row of
+``DiagnosticsRecorder.record`` (``run/bve/diagnostics.py:162-211``) and
+``SWEDiagnosticsRecorder.record`` (``run/swe/diagnostics.py:113-173``);
+pinned (tests/test_oracle_diagnostics.py) by solid-body rotation (speed,
+energy, rms vorticity, CFL in closed form) and the resting layer (mass,
+energy, depth, free surface).
 """
 from __future__ import annotations
 
@@ -17,7 +24,7 @@ import math
 
 import numpy as np
 
-__all__ = ["mode_power", "spectral_diagnostics"]
+__all__ = ["mode_power", "spectral_diagnostics", "bve_record_row", "swe_record_row"]
 
 
 def mode_power(a: np.ndarray) -> np.ndarray:
@@ -42,4 +49,93 @@ def spectral_diagnostics(zeta: np.ndarray, radius: float, omega: float) -> dict:
         "energy": 0.5 * R ** 4 * math.fsum(inv * Pl),
         "enstrophy_rel": 0.5 * R ** 2 * math.fsum(Pl),
         "enstrophy_abs": 0.5 * R ** 2 * math.fsum(mode_power(q).ravel()),
+    }
+
+
+def bve_record_row(planet, zeta: np.ndarray, dt: float, length_scale) -> dict:
+    """One ``DiagnosticsRecorder.record`` row with the round-trip residual
+    (``run/bve/diagnostics.py:162-211``)."""
+    R, L = planet.R, planet.l_max
+    spec = spectral_diagnostics(zeta, R, planet.Omega)
+    P = mode_power(zeta)
+    Pl = np.array([math.fsum(row) for row in P])
+    l = np.arange(L + 1, dtype=np.float64)
+    Z_l = 0.5 * R ** 2 * Pl
+    E_l = np.zeros(L + 1)
+    E_l[1:] = 0.5 * R ** 4 * Pl[1:] / (l[1:] * (l[1:] + 1.0))
+    cut = (2 * L) // 3
+    sh, so = planet.sh, planet.so
+    z = sh.inv_transform(zeta)
+    w = np.asarray(sh.weights, dtype=np.float64).ravel()
+    psi = so.inv_laplacian(zeta)
+    u, v = so.velocity_from_streamfunction(psi, sh, planet.grid.point_latitudes)
+    max_speed = float(np.max(np.sqrt(u * u + v * v)))
+    back = sh.transform(z)
+    return {
+        "max_speed_ms": max_speed,
+        "cfl": max_speed * dt / length_scale if (length_scale and dt > 0) else float("nan"),
+        "circulation": spec["circulation"],
+        "energy": spec["energy"],
+        "enstrophy_rel": spec["enstrophy_rel"],
+        "enstrophy_abs": spec["enstrophy_abs"],
+        "zeta_max": float(np.max(np.abs(z))),
+        "zeta_rms": math.sqrt(math.fsum(w * z * z) / math.fsum(w)),
+        "energy_l1": float(E_l[1]) if L >= 1 else 0.0,
+        "high_l_enstrophy_frac": (math.fsum(Z_l[cut + 1:]) / spec["enstrophy_rel"]
+                                  if spec["enstrophy_rel"] > 0 else 0.0),
+        "roundtrip_residual": float(np.linalg.norm(back - zeta)) / (float(np.linalg.norm(zeta))
+                                                                    + 1e-300),
+    }
+
+
+def swe_record_row(p, y: np.ndarray, dt: float, length_scale, terrain_max_m: float = 0.0) -> dict:
+    """One ``SWEDiagnosticsRecorder.record`` row (``run/swe/diagnostics.py:113-173``,
+    ``characteristic_fields`` ``phy/shallow_water.py:456-468``); ``p`` is a
+    ``SWEParams``, ``y`` the (3, L+1, L+1) stack."""
+    pl = p.planet
+    R, so, sh = pl.R, pl.so, pl.sh
+    zeta, delta, phi = y[0], y[1], y[2]
+    psi, chi = p.inv_lap(zeta), p.inv_lap(delta)
+    c = pl.scoslat
+
+    def deriv(a):
+        return sh.inv_transform(so.d_lambda(a)), sh.inv_transform(so.sin_theta_d_theta(a)) / R
+
+    psi_lam, psi_snt = deriv(psi)
+    chi_lam, chi_snt = deriv(chi)
+    u = (psi_snt + chi_lam) / c
+    v = (psi_lam - chi_snt) / c
+    phi_total = p.phi0 + sh.inv_transform(phi)
+    lo, hi = float(phi_total.min()), float(phi_total.max())
+    if pl.psh is not pl.sh:
+        pt = p.phi0 + pl.psh.inv_transform(phi)
+        lo, hi = min(lo, float(pt.min())), max(hi, float(pt.max()))
+    max_wind = float(np.max(np.sqrt(u * u + v * v)))
+    max_char = max_wind + math.sqrt(max(hi, 0.0))
+    kinetic = 0.5 * phi_total * (u * u + v * v)
+    potential = 0.5 * phi_total ** 2
+    if p.phi_s_lm is not None:
+        phi_s = sh.inv_transform(p.phi_s_lm)
+        mean_phi_s = float(p.phi_s_lm[0, 0].real) / math.sqrt(4.0 * math.pi)
+        potential = potential + phi_total * phi_s
+        eta = (phi_total - p.phi0 + (phi_s - mean_phi_s)) / p.gravity
+    else:
+        eta = (phi_total - p.phi0) / p.gravity
+    w = np.asarray(sh.weights, dtype=np.float64).ravel() * R ** 2
+    return {
+        "max_wind_ms": max_wind,
+        "max_char_speed_ms": max_char,
+        "cfl": max_char * dt / length_scale if (length_scale and dt > 0) else float("nan"),
+        "phi_total_min": lo,
+        "phi_total_max": hi,
+        "total_mass": R ** 2 * (4.0 * math.pi * p.phi0
+                                + math.sqrt(4.0 * math.pi) * float(phi[0, 0].real)),
+        "total_energy": math.fsum(w * (kinetic + potential)),
+        "h_min_m": lo / p.gravity,
+        "eta_min_m": float(eta.min()),
+        "eta_max_m": float(eta.max()),
+        "terrain_max_m": float(terrain_max_m),
+        "zeta_l2": math.sqrt(R ** 2 * math.fsum(mode_power(zeta).ravel())),
+        "delta_l2": math.sqrt(R ** 2 * math.fsum(mode_power(delta).ravel())),
+        "phi_l2": math.sqrt(R ** 2 * math.fsum(mode_power(phi).ravel())),
     }
