@@ -5,8 +5,9 @@ spectra files they write, and the SWE characteristic speed
 (phy/shallow_water.py:470-484).
 
 Bars: every number within 1e-11 relative of the oracle (grid extrema, the
-compensated quadratures and the spectral sums); round-trip residuals both
-below 1e-12.
+compensated quadratures and the spectral sums); the round-trip residual
+below 1e-12 where the quadrature is exact (Gauss) and within 1e-10 relative
+of the oracle's on geodesic Voronoi weights, where it is a real residual.
 """
 import csv
 import math
@@ -57,8 +58,10 @@ def test_bve_recorder_matches_oracle(kind, L, nlat, nlon, res, tmp_path):
         row = rec.record(step * dt, torch.as_tensor(z, device="cuda"), dt, step)
         ref = bve_record_row(op, z, dt, ls)
         for k, v in ref.items():
-            if k == "roundtrip_residual":
-                assert row[k] < 1e-12 and v < 1e-12
+            if k == "roundtrip_residual" and v < 1e-12:   # exact quadrature (Gauss)
+                assert row[k] < 1e-12
+            elif k == "roundtrip_residual":                 # Voronoi weights: a real residual
+                assert _close(row[k], v, 1e-10), (row[k], v)
             else:
                 assert _close(row[k], v), (k, row[k], v)
     rec.close()
