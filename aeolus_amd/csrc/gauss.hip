@@ -180,6 +180,23 @@ __device__ inline void unpack_sd(const double2* Z, int n, int mmax, double w, do
 // Intermediate layout of every grid kernel: E/O/S/D[jp][m][column], one
 // column per (member, field) pair, `ncol` columns in all.
 
+// (row pair, member) of this workgroup from a 1-D grid of nh x nmem.  Member
+// batches: workgroups are dealt to the 8 XCDs round-robin by id, so the
+// decode gives each XCD runs of consecutive members of one row pair, whose
+// E/O and S/D columns are adjacent in memory ([jp][m][member][field]) and
+// share that XCD's L2; otherwise id = jp + nh * b.
+__device__ inline void grid_wg_index(int nh, int nmem, int& jp, int& b) {
+    const int id = blockIdx.x;
+    if (nmem > 1 && (nh & 7) == 0) {
+        const int x = id & 7, k = id >> 3;
+        b = k % nmem;
+        jp = (k / nmem) * 8 + x;
+    } else {
+        jp = id % nh;
+        b = id / nh;
+    }
+}
+
 // Prologue of the fused grid kernels: the twiddle table (n entries at twl)
 // and the NF even/odd columns of one member of a row pair (sE/sO[m * NF + f],
 // m <= L) into LDS.  Each thread issues all of its loads before its first
@@ -289,7 +306,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
                                                                int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
-    const int jp = blockIdx.x, b = blockIdx.y;
+    int jp, b;
+    grid_wg_index(g.nh, nmem, jp, b);
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const double invR = 1.0 / R;
@@ -333,7 +351,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
                                                                int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
-    const int jp = blockIdx.x, b = blockIdx.y;
+    int jp, b;
+    grid_wg_index(g.nh, nmem, jp, b);
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const double invR = 1.0 / R;
@@ -397,7 +416,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
                                                                int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
-    const int jp = blockIdx.x, b = blockIdx.y;
+    int jp, b;
+    grid_wg_index(g.nh, nmem, jp, b);
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const double invR = 1.0 / R;
@@ -613,7 +633,7 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
     if ((rc = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p)))) return rc;
-    hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh, 1), dim3(kGridThreads), jac_lds(p), s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
+    hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh), dim3(kGridThreads), jac_lds(p), s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
                        p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
@@ -671,7 +691,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
         int rc0 = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p));
         if (rc0) return rc0;
         pr.arm(kClsGrid);
-        launch_k(grid_jac_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), jac_lds(p), s,
+        launch_k(grid_jac_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), jac_lds(p), s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     } else {
@@ -679,7 +699,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
         int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);
         if (rc0) return rc0;
         pr.arm(kClsGrid);
-        launch_k(grid_swe_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), swe_lds, s,
+        launch_k(grid_swe_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), swe_lds, s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     }
@@ -748,7 +768,7 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     int rc0 = allow_lds((const void*)grid_adj_kernel_for(p), lds);
     if (rc0) return rc0;
     pr.arm(kClsGrid);
-    launch_k(grid_adj_kernel_for(p), dim3(p->nh, md->nmem), dim3(kGridThreads), lds, s,
+    launch_k(grid_adj_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), lds, s,
              p->gdev(), (const double2*)md->d_aE, (const double2*)md->d_aO, md->R, md->nmem,
              md->d_aS, md->d_aD);
     AEO_LAUNCH_CHECK();
