@@ -17,7 +17,9 @@ Measured in the same run (``secondary``):
     32x64/64x128/128x256/256x512, one hipGraph replay per two steps,
     autotuned tiles, FP64 and split, with the one-day error;
   * the adjoint: L = 170, 48 steps, split transforms (7 slices); cost of the
-    checkpointed forward + reverse sweep relative to the plain forward.
+    checkpointed forward + reverse sweep relative to the plain forward;
+  * the geodesic res 4 / L = 21 trajectory with FP64 and split (6 slices)
+    dense Y-basis contractions.
 
 Inputs are resident in HBM before every timed region; a "step" is one full
 RK4 step (4 fused stages).  Prints one JSON line.  ``--gpus N`` with N > 1
@@ -303,6 +305,32 @@ def ensemble_only(args):
                       "member_steps_per_s": 4 * nper * 1000.0 / ms}))
 
 
+def geodesic_modes(steps, warmup):
+    """Geodesic res 4 state / res 5 product grid, L = 21, 96 x 900 s (the C2
+    trajectory): ms/step and the error against FP64 for the dense Y-basis
+    contractions in FP64 and split precision (6 slices)."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.planet import Planet
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R_EARTH, OMEGA, grid_type="geodesic", grid_resolution=4,
+                                l_max=21, product_quadrature="fine")
+    model = BarotropicVorticity(pl)
+    z0 = torch.as_tensor(spectrum_l15(21, np.random.default_rng(21), rms=5e-5), device="cuda")
+    out = torch.empty_like(z0)
+    res, ref = {}, None
+    for ns in (0, 6):
+        model.set_precision(ns)
+        zT = model.rk4(z0, 900.0, 96)
+        ref = zT if ns == 0 else ref
+        ms = _timed(lambda k: model.rk4(z0, 900.0, k, out=out), min(steps, 96), warmup)
+        res[_mode_name(ns)] = {"ms_per_step": ms,
+                               "rel_l2_vs_fp64_96steps": 0.0 if ns == 0 else _rel(zT, ref)}
+    return res
+
+
 def adjoint_cost(model, ns):
     import torch
     from aeolus_amd.run.initial_conditions import spectrum_l15
@@ -406,6 +434,7 @@ def main():
         secondary["ensemble_256_members"] = {_mode_name(k): v for k, v in
                                              ensemble_modes(48, ns_split).items()}
         secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
+        secondary["geodesic_res4_L21_modes"] = geodesic_modes(args.steps, args.warmup)
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
         "value": 1000.0 / h["ms_step"],
