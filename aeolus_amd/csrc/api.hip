@@ -252,7 +252,7 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
     p->B = n_basis(l_max);
     p->npts = npoints;
     p->max_fields = max_fields;
-    p->nchunks = (npoints + 1023) / 1024;
+    p->nchunks = (npoints + kPtsChunk - 1) / kPtsChunk;
     std::vector<double> w(npoints);
     for (int i = 0; i < npoints; ++i) w[i] = weights ? weights[i] : 4.0 * M_PI / npoints;
     double *d_lat = nullptr, *d_lon = nullptr;
@@ -288,7 +288,7 @@ int aeo_plan_create_points(int npoints, int l_max, const double* lat, const doub
 static int enable_split_points(aeo_plan_s* plan, int nslices) {
     const int B = plan->B;
     const int K2 = rup(2 * B, 256);
-    AEO_REQUIRE(split_exact(nslices, K2) && split_exact(nslices, 1024), AEO_ERR_UNSUPPORTED,
+    AEO_REQUIRE(split_exact(nslices, K2) && split_exact(nslices, kPtsChunk), AEO_ERR_UNSUPPORTED,
                 "aeo_plan_enable_split: INT32 accumulation would not be exact at this size");
     if (plan->nslices == nslices) return AEO_OK;
     for (void* q : {(void*)plan->d_Ysl, (void*)plan->d_eYs, (void*)plan->d_Yasl,
@@ -299,7 +299,7 @@ static int enable_split_points(aeo_plan_s* plan, int nslices) {
     plan->nprow = rup(plan->npts, 16);
     plan->K2 = K2;
     plan->R2p = rup(2 * B, 16);
-    plan->Npad = plan->nchunks * 1024;
+    plan->Npad = plan->nchunks * kPtsChunk;
     int rc;
     if ((rc = dev_alloc(&plan->d_Ysl, (size_t)nslices * plan->nprow * plan->K2))) return rc;
     if ((rc = dev_alloc(&plan->d_eYs, (size_t)plan->nprow))) return rc;

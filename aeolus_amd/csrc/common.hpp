@@ -75,6 +75,7 @@ inline int n_basis(int L) { return (L + 1) * (L + 2) / 2; }
 // Block size used by the grid (FFT) kernels and the maximum number of
 // complex values one thread keeps in registers across an in-place FFT stage.
 constexpr int kGridThreads = 256;
+constexpr int kPtsChunk = 256;    // points per analysis partial sum (point-set plans)
 constexpr int kMaxPerThread = 24;
 constexpr int kMaxRadices = 16;
 

@@ -13,7 +13,8 @@
 namespace aeo {
 
 constexpr int kPtsThreads = 256;
-constexpr int kPtsChunk = 1024;  // points per analysis partial sum
+constexpr int kPtsSynPts = 64;                          // points per synthesis workgroup
+constexpr int kPtsSynSlices = kPtsThreads / kPtsSynPts;  // K (basis) slices per point
 
 __device__ inline void k_to_lm(int k, int& l, int& m) {
     int t = (int)((sqrt(8.0 * k + 1.0) - 1.0) * 0.5);
@@ -104,10 +105,13 @@ struct PwSwe {  // 12 synthesised fields -> w * (zeta_dot, curl, kinetic, phi_do
     }
 };
 
+// Workgroup = 64 points (lanes) x 4 slices of the basis index k (waves):
+// each wave sums its quarter of k, the four partials meet in LDS in a fixed
+// order, and the first wave applies the pointwise stage.
 template <class Loader, class Pw>
 __global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld, Pw pw) {
     constexpr int NF = Loader::NF;
-    extern __shared__ double2 lds[];  // [NF][B] coefficient values times c_k
+    extern __shared__ double2 lds[];  // [NF][B] coefficient values times c_k | partials
     const int B = p.B;
     for (int t = threadIdx.x; t < NF * B; t += kPtsThreads) {
         const int f = t / B, k = t - f * B;
@@ -117,14 +121,17 @@ __global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Lo
         lds[t] = (m == 0) ? v : cscale(v, 2.0);
     }
     __syncthreads();
-    const int i = blockIdx.x * kPtsThreads + threadIdx.x;
-    if (i >= p.npts) return;
+    const int lane = threadIdx.x % kPtsSynPts, sl = threadIdx.x / kPtsSynPts;
+    const int i = blockIdx.x * kPtsSynPts + lane;
+    const int ic = (i < p.npts) ? i : p.npts - 1;   // clamped: every load stays in range
+    const int kq = (B + kPtsSynSlices - 1) / kPtsSynSlices;
+    const int k0 = sl * kq, k1 = (k0 + kq < B) ? k0 + kq : B;
     double acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0;
-    const double* yr = p.Yk + i;
-    const double* yi = p.Yk + (size_t)B * p.npts + i;
-    for (int k = 0; k < B; ++k) {
+    const double* yr = p.Yk + ic;
+    const double* yi = p.Yk + (size_t)B * p.npts + ic;
+    for (int k = k0; k < k1; ++k) {
         const double a = yr[(size_t)k * p.npts];
         const double b = yi[(size_t)k * p.npts];
 #pragma unroll
@@ -134,7 +141,20 @@ __global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Lo
             acc[f] = fma(-b, c.y, acc[f]);
         }
     }
-    pw.template apply<NF>(i, acc, p.w[i], p.coslat[i]);
+    double* part = reinterpret_cast<double*>(lds + (size_t)NF * B);   // [slices][NF][64]
+#pragma unroll
+    for (int f = 0; f < NF; ++f) part[(sl * NF + f) * kPtsSynPts + lane] = acc[f];
+    __syncthreads();
+    if (sl == 0 && i < p.npts) {
+        double v[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            double s = part[f * kPtsSynPts + lane];
+            for (int q = 1; q < kPtsSynSlices; ++q) s += part[(q * NF + f) * kPtsSynPts + lane];
+            v[f] = s;
+        }
+        pw.template apply<NF>(i, v, p.w[i], p.coslat[i]);
+    }
 }
 
 // Partial analysis sums: part[chunk][f][k] = sum_{i in chunk} conj(Y_ik) g_i,
@@ -405,8 +425,11 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_split_kernel(
 // ---------------------------------------------------------------------------
 template <class Loader, class Pw>
 static int launch_psyn(const aeo_plan_s* p, const Loader& ld, const Pw& pw, hipStream_t s) {
-    const size_t lds = sizeof(double2) * Loader::NF * p->B;
-    dim3 grid((p->npts + kPtsThreads - 1) / kPtsThreads);
+    const size_t lds = sizeof(double2) * Loader::NF * p->B +
+                       sizeof(double) * kPtsSynSlices * Loader::NF * kPtsSynPts;
+    AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
+                "point synthesis: LDS demand exceeds 160 KiB at this l_max");
+    dim3 grid((p->npts + kPtsSynPts - 1) / kPtsSynPts);
     if (lds > 65536) {
         AEO_CHECK_HIP(hipFuncSetAttribute((const void*)points_syn_kernel<Loader, Pw>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
