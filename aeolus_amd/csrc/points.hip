@@ -13,7 +13,7 @@
 namespace aeo {
 
 constexpr int kPtsThreads = 256;
-constexpr int kPtsSynPts = 64;                          // points per synthesis workgroup
+constexpr int kPtsSynPts = 32;                          // points per synthesis workgroup
 constexpr int kPtsSynSlices = kPtsThreads / kPtsSynPts;  // K (basis) slices per point
 
 __device__ inline void k_to_lm(int k, int& l, int& m) {
