@@ -172,6 +172,7 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
     const double* v = vals + (size_t)f * p.npts;
     double2 acc = c2(0.0, 0.0);
     if (k < p.B) {
+#pragma unroll 8
         for (int i = i0 + q; i < i1; i += 4) {
             const double g = weights ? v[i] * weights[i] : v[i];
             const double2 y = p.Yp[(size_t)i * p.B + k];
@@ -190,29 +191,37 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
     }
 }
 
-// Sum the partials (fixed chunk order) and run the epilogue for every (l, m).
+// Sum the chunk partials and run the epilogue for every (l, m): one wave per
+// entry, lanes over chunks, a fixed shuffle tree (deterministic), lane 0
+// applies the epilogue.
 template <class Epi>
-__global__ void points_finish_kernel(int L, int B, int nchunks, int nf_part, int nf_valid,
-                                     const double2* __restrict__ part, int lmax_a, int mmax_a,
-                                     Epi ep) {
+__global__ __launch_bounds__(kPtsThreads) void points_finish_kernel(
+    int L, int B, int nchunks, int nf_part, int nf_valid, const double2* __restrict__ part,
+    int lmax_a, int mmax_a, Epi ep) {
     constexpr int NF = Epi::NF;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (kPtsThreads / 64) + (threadIdx.x >> 6);
     const int n = L + 1;
-    if (t >= n * n) return;
+    if (t >= n * n) return;   // whole waves exit together
     const int l = t / n, m = t - l * n;
     double2 acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = c2(0.0, 0.0);
     if (m <= l && l <= lmax_a && m <= mmax_a) {
         const int k = l * (l + 1) / 2 + m;
-        for (int c = 0; c < nchunks; ++c) {
+        for (int c = lane; c < nchunks; c += 64) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
+            for (int f = 0; f < NF; ++f)
                 if (f < nf_valid) acc[f] = cadd(acc[f], part[((size_t)c * nf_part + f) * B + k]);
-            }
         }
     }
-    ep.apply(l, m, acc);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        for (int off = 32; off > 0; off >>= 1) {
+            acc[f].x += __shfl_xor(acc[f].x, off);
+            acc[f].y += __shfl_xor(acc[f].y, off);
+        }
+    if (lane == 0) ep.apply(l, m, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -517,8 +526,9 @@ template <class Epi>
 static int launch_finish(const aeo_plan_s* p, int nf_part, int nf_valid, const double2* part,
                          int lmax_a, int mmax_a, const Epi& ep, hipStream_t s) {
     const int n = (p->L + 1) * (p->L + 1);
-    hipLaunchKernelGGL(points_finish_kernel<Epi>, dim3((n + 255) / 256), dim3(256), 0, s, p->L,
-                       p->B, p->nchunks, nf_part, nf_valid, part, lmax_a, mmax_a, ep);
+    constexpr int per = kPtsThreads / 64;   // entries (waves) per workgroup
+    hipLaunchKernelGGL(points_finish_kernel<Epi>, dim3((n + per - 1) / per), dim3(kPtsThreads), 0,
+                       s, p->L, p->B, p->nchunks, nf_part, nf_valid, part, lmax_a, mmax_a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
