@@ -102,3 +102,21 @@ def test_split_bve_day_vs_oracle_l85():
     err = rel_l2(sp, fp64)
     print(f"split(6) vs fp64 after 1 day at L=85: {err:.3e}")
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("L,nlat,nlon", [(20, 30, 60), (33, 51, 100)])
+def test_split_odd_grids_match_fp64(L, nlat, nlon):
+    """Odd latitude counts (an equator row, nh not a multiple of 16) and
+    non-power-of-two longitudes: padding rows and columns of the digit tiles."""
+    import torch
+    from aeolus_amd.physics import BarotropicState, BarotropicVorticity
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print(f"case split odd grid L={L} {nlat}x{nlon}", flush=True)
+    pl = _planet(L, nlat, nlon)
+    m = BarotropicVorticity(pl, viscosity=1e5)
+    z = torch.as_tensor(spectrum_l15(L, np.random.default_rng(L + 1), rms=5e-5), device="cuda")
+    ref = m.tendency(BarotropicState(z)).cpu().numpy()
+    y_ref = m.rk4(z, 600.0, 20).cpu().numpy()
+    m.set_precision(6)
+    assert rel_l2(m.tendency(BarotropicState(z)).cpu().numpy(), ref) < 5e-12
+    assert rel_l2(m.rk4(z, 600.0, 20).cpu().numpy(), y_ref) < 1e-11
