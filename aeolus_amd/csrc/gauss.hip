@@ -198,8 +198,9 @@ __device__ inline void grid_wg_index(int nh, int nmem, int& jp, int& b) {
 }
 
 // Prologue of the fused grid kernels: the twiddle table (n entries at twl)
-// and the NF even/odd columns of one member of a row pair (sE/sO[m * NF + f],
-// m <= L) into LDS.  Each thread issues all of its loads before its first
+// and the NF even/odd columns of one member of a row pair into LDS, field
+// major (sE/sO[f * (L+1) + m], m <= L), so the per-row spectrum packing reads
+// consecutive wavenumbers.  Each thread issues all of its loads before its first
 // LDS store, so the gather costs one memory latency.
 template <int NF>
 __device__ inline void grid_prologue(const GaussDev& g, const double2* __restrict__ E,
@@ -225,8 +226,9 @@ __device__ inline void grid_prologue(const GaussDev& g, const double2* __restric
         for (int u = 0; u < PER; ++u) {
             const int t = t0 + threadIdx.x + u * kGridThreads;
             if (t < tot) {
-                sE[t] = ve[u];
-                sO[t] = vo[u];
+                const int m = t / NF, f = t - m * NF;
+                sE[f * (g.L + 1) + m] = ve[u];
+                sO[f * (g.L + 1) + m] = vo[u];
             }
             if (t < n) twl[t] = vt[u];
         }
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
     const size_t ms = (size_t)nmem * 4;
-    // LDS: rows [4][rs] | twiddles [n] | E, O columns [L+1][4] each
+    // LDS: rows [4][rs] | twiddles [n] | E, O columns [4][L+1] each
     double2* sE = lds + 4 * rs + n;
     double2* sO = sE + (L + 1) * 4;
     const double2* twl = lds + 4 * rs;
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     __syncthreads();
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
-        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 4,
+        pack_spectrum(lds + r * rs, n, L, sE + r * (L + 1), sO + r * (L + 1), 1,
                       (r & 1) ? kSpecScale : kSpecDlam, invR, threadIdx.x, kGridThreads);
     }
     __syncthreads();
@@ -363,14 +365,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
                               kSpecScale, kSpecPlain, kSpecDlam, kSpecScale, kSpecPlain,
                               kSpecPlain, kSpecPlain};
-    // LDS: rows [12][rs] | twiddles [n] | E, O columns [L+1][10] each
+    // LDS: rows [12][rs] | twiddles [n] | E, O columns [10][L+1] each
     double2* sE = lds + 12 * rs + n;
     double2* sO = sE + (L + 1) * 10;
     const double2* twl = lds + 12 * rs;
     grid_prologue<10>(g, E, O, eo, ms, lds + 12 * rs, sE, sO);
     __syncthreads();
     for (int r = 0; r < 12; ++r) {
-        pack_spectrum(lds + r * rs, n, L, sE + src[r], sO + src[r], 10, mode[r], invR,
+        pack_spectrum(lds + r * rs, n, L, sE + src[r] * (L + 1), sO + src[r] * (L + 1), 1, mode[r], invR,
                       threadIdx.x, kGridThreads);
     }
     __syncthreads();
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
     const size_t ms = (size_t)nmem * 5;
-    // LDS: rows [5][rs] | twiddles [n] | E, O columns [L+1][5] each
+    // LDS: rows [5][rs] | twiddles [n] | E, O columns [5][L+1] each
     double2* sE = lds + 5 * rs + n;
     double2* sO = sE + (L + 1) * 5;
     const double2* twl = lds + 5 * rs;
@@ -431,7 +433,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     __syncthreads();
     for (int r = 0; r < 5; ++r) {
         const int mode = (r == 4) ? kSpecPlain : ((r & 1) ? kSpecScale : kSpecDlam);
-        pack_spectrum(lds + r * rs, n, L, sE + r, sO + r, 5, mode, invR, threadIdx.x, kGridThreads);
+        pack_spectrum(lds + r * rs, n, L, sE + r * (L + 1), sO + r * (L + 1), 1, mode, invR,
+                      threadIdx.x, kGridThreads);
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 5, g.fft, twl);
