@@ -139,6 +139,32 @@ __device__ inline void pack_spectrum(double2* Z, int n, int L, const double2* Er
                                      int tstride) {
     // Er[m * ms], Or[m * ms]: even/odd partial sums of one field at wavenumber m;
     // invR = 1/R (the (i m / R) factor is formed as i (m * invR))
+    if (n > 2 * L) {
+        // no aliasing: bin b holds m = b (b <= L) and m = n - b (1 <= n - b <= L);
+        // both loads are clamped into range and issued unconditionally, the
+        // masks drop what does not belong to the bin
+        for (int b = t0; b < n; b += tstride) {
+            const int m1 = (b <= L) ? b : L;
+            const int m2 = (b > 0 && n - b <= L) ? n - b : L;
+            const double2 e1 = Er[m1 * ms], o1 = Or[m1 * ms];
+            const double2 e2 = Er[m2 * ms], o2 = Or[m2 * ms];
+            double2 gn = cadd(e1, o1), gs = csub(e1, o1);
+            double2 hn = cadd(e2, o2), hs = csub(e2, o2);
+            if (mode == kSpecDlam) {
+                gn = dlam_q(gn, m1 * invR); gs = dlam_q(gs, m1 * invR);
+                hn = dlam_q(hn, m2 * invR); hs = dlam_q(hs, m2 * invR);
+            } else if (mode == kSpecScale) {
+                gn = cscale(gn, invR); gs = cscale(gs, invR);
+                hn = cscale(hn, invR); hs = cscale(hs, invR);
+            }
+            if (b == 0) { gn.y = 0.0; gs.y = 0.0; }
+            double2 z = c2(0.0, 0.0);
+            if (b <= L) z = c2(gn.x - gs.y, gn.y + gs.x);
+            if (b > 0 && n - b <= L) z = cadd(z, c2(hn.x + hs.y, -hn.y + hs.x));
+            Z[lp(b)] = z;
+        }
+        return;
+    }
     for (int b = t0; b < n; b += tstride) {
         double2 z = c2(0.0, 0.0);
         for (int m = b; m <= L; m += n) {
