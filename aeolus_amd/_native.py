@@ -63,6 +63,12 @@ _SIGNATURES = {
     "aeo_ensemble_rk4": [_vp, ctypes.POINTER(_vp), _c_double, _c_int, ctypes.POINTER(_vp), _c_int,
                          _vp],
     "aeo_model_profile_read": [_vp, _dp, ctypes.POINTER(_c_int)],
+    "aeo_vector_curl_div_work": [_vp, _c_int, ctypes.POINTER(ctypes.c_longlong)],
+    "aeo_vector_curl_div": [_vp, _c_double, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp],
+    "aeo_pe_create": [_vp, _vp, _c_int, _dp, _c_double, _c_double, _c_double, _c_double, _vp,
+                      ctypes.POINTER(_vp)],
+    "aeo_pe_destroy": [_vp],
+    "aeo_pe_tendency": [_vp, _vp, _vp, _vp],
 }
 
 _lib = None

@@ -5,24 +5,6 @@
 
 namespace aeo {
 
-// C+ and C- of num/spectral_operators.py:135-176 (clamped numerators and
-// denominators, zero for m > l, C+[L,:] = 0, C-[0,:] = 0).
-__device__ inline double c_plus(int l, int m, int L) {
-    if (m > l || l == L) return 0.0;
-    const double Lb = l, Mb = m;
-    const double nu = fmax((Lb + 1.0) * (Lb + 1.0) - Mb * Mb, 0.0);
-    const double de = fmax((2.0 * Lb + 1.0) * (2.0 * Lb + 3.0), 1.0);
-    return Lb * sqrt(nu / de);
-}
-
-__device__ inline double c_minus(int l, int m) {
-    if (m > l || l == 0) return 0.0;
-    const double Lb = l, Mb = m;
-    const double nu = fmax(Lb * Lb - Mb * Mb, 0.0);
-    const double de = fmax((2.0 * Lb - 1.0) * (2.0 * Lb + 1.0), 1.0);
-    return -(Lb + 1.0) * sqrt(nu / de);
-}
-
 // One thread per (field, l, m).  Formulas: num/spectral_operators.py:213-334
 // and phy/barotropic.py:68-93.
 __global__ void spec_op_kernel(int op, int L, double R, const double2* __restrict__ in,

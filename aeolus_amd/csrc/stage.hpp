@@ -36,6 +36,24 @@ __device__ inline double2 dlam(double2 a, int m, double R) {
     return c2(0.0 * a.x - q * a.y, 0.0 * a.y + q * a.x);
 }
 
+// C+ and C- of num/spectral_operators.py:135-176 (clamped numerators and
+// denominators, zero for m > l, C+[L,:] = 0, C-[0,:] = 0).
+__device__ inline double c_plus(int l, int m, int L) {
+    if (m > l || l == L) return 0.0;
+    const double Lb = l, Mb = m;
+    const double nu = fmax((Lb + 1.0) * (Lb + 1.0) - Mb * Mb, 0.0);
+    const double de = fmax((2.0 * Lb + 1.0) * (2.0 * Lb + 3.0), 1.0);
+    return Lb * sqrt(nu / de);
+}
+
+__device__ inline double c_minus(int l, int m) {
+    if (m > l || l == 0) return 0.0;
+    const double Lb = l, Mb = m;
+    const double nu = fmax(Lb * Lb - Mb * Mb, 0.0);
+    const double de = fmax((2.0 * Lb - 1.0) * (2.0 * Lb + 1.0), 1.0);
+    return -(Lb + 1.0) * sqrt(nu / de);
+}
+
 // (S a)_{l,m} for a field given by a functor base(l, m).
 template <class F>
 __device__ inline double2 s_apply(const F& base, int l, int m, const SpecK& k) {

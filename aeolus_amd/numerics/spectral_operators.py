@@ -2,13 +2,18 @@
 
 Mirrors ``SpectralOperators`` (``num/spectral_operators.py:53-658``); every
 operator runs on the HIP device through ``aeo_spec_op`` / ``aeo_jacobian`` /
-``aeo_velocity`` / ``aeo_gradient``.
+``aeo_velocity`` / ``aeo_gradient`` / ``aeo_vector_curl_div``.
 """
 from __future__ import annotations
+
+import ctypes
+import warnings
 
 import torch
 
 from .. import _native as nat
+from .latlon_grid import GaussLatLonSphericalHarmonics
+from .point_set_sh import PointSetSphericalHarmonics
 from .spherical_backend import make_backend
 
 __all__ = ["SpectralOperators"]
@@ -30,6 +35,7 @@ class SpectralOperators:
                            if self._product_space.geometry is not None else None)
         l = torch.arange(self.l_max + 1, dtype=torch.float64, device=self.device)
         self.lap_eigs = -l * (l + 1.0) / self.R ** 2
+        self._ext = None
 
     # -- helpers -----------------------------------------------------------
     def _coeffs(self, c) -> torch.Tensor:
@@ -100,6 +106,53 @@ class SpectralOperators:
         if return_spectral:
             return out
         return self.sh.inv_transform(out)
+
+    # -- weak-form vector analysis (num/spectral_operators.py:399-502) ------
+    def extended_product_transform(self):
+        """The product-sampling transform one degree higher (same points, same
+        weights), built once (``num/spectral_operators.py:480-502``)."""
+        if self._ext is None:
+            sh_p = self._product_space.sh
+            with warnings.catch_warnings():
+                # a Gauss grid sized for degree L is under-resolved at L + 1; the
+                # quadrature is the product sampling's either way
+                warnings.simplefilter("ignore")
+                if isinstance(sh_p, GaussLatLonSphericalHarmonics):
+                    self._ext = GaussLatLonSphericalHarmonics(sh_p.grid, self.l_max + 1)
+                else:
+                    self._ext = PointSetSphericalHarmonics(sh_p.latitudes, sh_p.longitudes,
+                                                           self.l_max + 1, weights=sh_p.weights,
+                                                           device=self.device)
+        return self._ext
+
+    def vector_curl_div_spectral(self, f_east, f_north, *, truncate: bool = True):
+        """``(curl_lm, div_lm)`` of a tangent field sampled on the product
+        points (``(N,)`` or ``(F, N)`` per component): degree-(L+1) analyses of
+        the half-metric components, ``div = (i m/R) a + S^T_ext b / R``,
+        ``curl = (i m/R) b - S^T_ext a / R``, 2/3-truncated when ``truncate``."""
+        ext = self.extended_product_transform()
+        fe = torch.as_tensor(f_east, dtype=torch.float64, device=self.device).contiguous()
+        fn = torch.as_tensor(f_north, dtype=torch.float64, device=self.device).contiguous()
+        N = ext.n_points
+        if fe.shape != fn.shape or fe.numel() == 0 or fe.numel() % N:
+            raise ValueError(f"f_east and f_north must both be (N,) or (F, N) with N = {N}, "
+                             f"got {tuple(fe.shape)} and {tuple(fn.shape)}")
+        nf = fe.numel() // N
+        n = self.l_max + 1
+        curl = torch.empty((nf, n, n), dtype=torch.complex128, device=self.device)
+        div = torch.empty_like(curl)
+        nw = ctypes.c_longlong()
+        lib = nat.lib()
+        nat.check(lib.aeo_vector_curl_div_work(ext._plan, nf, ctypes.byref(nw)),
+                  "vector_curl_div_spectral")
+        work = torch.empty(nw.value, dtype=torch.float64, device=self.device)
+        nat.check(lib.aeo_vector_curl_div(ext._plan, self.R, fe.data_ptr(), fn.data_ptr(), nf,
+                                          1 if truncate else 0, curl.data_ptr(), div.data_ptr(),
+                                          work.data_ptr(), nat.stream()),
+                  "vector_curl_div_spectral")
+        if fe.dim() == 1:
+            return curl[0], div[0]
+        return curl, div
 
     # -- state-grid velocity / gradient (num/spectral_operators.py:517-558)
     def velocity_from_streamfunction(self, psi_lm):

@@ -57,6 +57,7 @@ enum aeo_spec_op_code {
 typedef struct aeo_plan_s* aeo_plan_t;
 typedef struct aeo_model_s* aeo_model_t;
 typedef struct aeo_ensemble_s* aeo_ensemble_t;
+typedef struct aeo_pe_s* aeo_pe_t;
 
 /* ABI version (AEO_ABI_VERSION) of the loaded library. */
 int aeo_version(void);
@@ -266,6 +267,41 @@ enum aeo_grid_stats_mode {
  * One device-to-host read of `out` then gives the whole row. */
 int aeo_grid_stats(int mode, int npts, const double* w, const double* a, const double* b,
                    const double* c, const double* d, const double* k, double* out, void* stream);
+
+/* Weak-form vector analysis of `nfields` tangent fields (east, north
+ * components, device (nfields, npoints) each, on the points of `ext`):
+ *   a = analysis_{L+1}(F_east / cos lat), b = analysis_{L+1}(F_north / cos lat),
+ *   div_lm  = (i m / R) a_lm + (S^T_ext b)_lm / R,
+ *   curl_lm = (i m / R) b_lm - (S^T_ext a)_lm / R,
+ * (S^T_ext c)_lm = C+_lm c_{l+1,m} + C-_lm c_{l-1,m} with the unclipped C+ at
+ * l = L; outputs (nfields, L+1, L+1) complex128, 2/3-truncated when
+ * `truncate` != 0; cos lat clamped at 1e-8.  `ext` is a plan of degree L+1 on
+ * the product sampling (same points and weights).  `work`: device, at least
+ * aeo_vector_curl_div_work(ext, nfields) doubles.
+ * Replaces SpectralOperators.vector_curl_div_spectral and
+ * _vector_analysis_pieces (num/spectral_operators.py:399-502). */
+int aeo_vector_curl_div_work(aeo_plan_t ext, int nfields, long long* n_doubles);
+int aeo_vector_curl_div(aeo_plan_t ext, double radius, const double* f_east,
+                        const double* f_north, int nfields, int truncate, double* curl_out,
+                        double* div_out, double* work, void* stream);
+
+/* Dry hydrostatic primitive-equation model on a product plan `prod` (degree
+ * L) and its degree-(L+1) companion `ext` on the same points.
+ *   interfaces: host, nlev + 1 sigma interfaces, exactly 0 first and 1 last,
+ *               strictly increasing (phy/sigma_coordinate.py:43-81)
+ *   phi_surface_lm: device (L+1)^2 complex128 surface geopotential, or NULL
+ *               for zero (its band limit is the caller's check,
+ *               phy/primitive_equations.py:209-238)
+ * Replaces PrimitiveEquationsModel.__init__ (phy/primitive_equations.py:180-264). */
+int aeo_pe_create(aeo_plan_t prod, aeo_plan_t ext, int nlev, const double* interfaces,
+                  double radius, double omega, double r_dry, double cp_dry,
+                  const double* phi_surface_lm, aeo_pe_t* out);
+int aeo_pe_destroy(aeo_pe_t pe);
+
+/* d/dt of the (3 nlev + 1, L+1, L+1) complex128 stack
+ * [zeta_1..K, delta_1..K, T_1..K, ln p_s] (device; out must not alias y).
+ * Replaces PrimitiveEquationsModel.tendency (phy/primitive_equations.py:482-753). */
+int aeo_pe_tendency(aeo_pe_t pe, const double* y, double* out, void* stream);
 
 #ifdef __cplusplus
 }

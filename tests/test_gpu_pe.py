@@ -1,0 +1,240 @@
+"""HIP primitive-equation tendency and weak-form vector curl/divergence
+against the CPU oracle, plus the exact properties the reference tests.
+
+Bars: vector analysis within 1e-12 of the oracle's scale (and of the closed
+forms curl = lap psi, div = lap chi on the Gauss 3/2-rule grid); the PE
+tendency within 1e-11 of each row block's scale (zeta, delta, T, ln p_s) on
+the Gauss (L = 15, 32 x 64) and geodesic (L = 10, res 3) product samplings;
+exact rest bitwise zero; BVE degeneracy within 1e-12 of the barotropic
+tendency's scale; zero flow with structured T gives -lap(Phi) within 1e-12.
+"""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+T0, PS0 = 280.0, 1.0e5
+SQRT4PI = math.sqrt(4.0 * math.pi)
+NONUNIFORM = (0.0, 0.05, 0.15, 0.30, 0.50, 0.75, 1.0)
+CASES = {"latlon": dict(L=15, nlat=32, nlon=64, res=0), "geodesic": dict(L=10, nlat=0, nlon=0,
+                                                                          res=3)}
+_cache = {}
+
+
+def _planets(kind):
+    if kind not in _cache:
+        from aeolus_amd.planet import Planet
+        from oracle.models import Planet as OPlanet, planet_constants
+        c = CASES[kind]
+        R, om = planet_constants(24.0)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            pl = Planet.with_radius(R, om, grid_type=kind, nlat=c["nlat"], nlon=c["nlon"],
+                                    grid_resolution=c["res"], l_max=c["L"],
+                                    product_quadrature="fine")
+        op = OPlanet(l_max=c["L"], radius=R, omega=om, grid_type=kind, nlat=c["nlat"],
+                     nlon=c["nlon"], resolution=c["res"])
+        _cache[kind] = (pl, op)
+    return _cache[kind]
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _band_limited(L, K, seed):
+    from oracle import pe
+    y = pe.isothermal_rest_state(L, K, T0, PS0)
+    rng = np.random.default_rng(seed)
+    n, cut = L + 1, (2 * L) // 3
+
+    def pert(amp, zero_mono):
+        p = amp * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        p[:, 0] = p[:, 0].real
+        p *= np.tril(np.ones((n, n)))
+        p[cut + 1:, :] = 0.0
+        if zero_mono:
+            p[0, 0] = 0.0
+        return p
+    for k in range(K):
+        y[k] += pert(2e-6, True)
+        y[K + k] += pert(2e-6, True)
+        y[2 * K + k] += pert(0.5, False)
+    y[3 * K] += pert(1e-3, True)
+    return y
+
+
+def _potentials(L, seed, top):
+    rng = np.random.default_rng(seed)
+    a = (rng.standard_normal((L + 1, L + 1)) + 1j * rng.standard_normal((L + 1, L + 1))) * 1e6
+    a = np.tril(a)
+    a[:, 0] = a[:, 0].real
+    a[0, :] = 0.0
+    a[top + 1:, :] = 0.0
+    return a
+
+
+def _vector(op, psi, chi):
+    so, psh, c = op.so, op.psh, op.pcoslat
+    p_lam = psh.inv_transform(so.d_lambda(psi))
+    p_snt = psh.inv_transform(so.sin_theta_d_theta(psi)) / so.R
+    c_lam = psh.inv_transform(so.d_lambda(chi))
+    c_snt = psh.inv_transform(so.sin_theta_d_theta(chi)) / so.R
+    return (p_snt + c_lam) / c, (p_lam - c_snt) / c
+
+
+@pytest.mark.parametrize("kind", ["latlon", "geodesic"])
+def test_vector_curl_div_matches_oracle(kind):
+    import torch
+    from oracle import pe
+    print(f"case vector curl/div {kind}", flush=True)
+    pl, op = _planets(kind)
+    L = op.l_max
+    psi, chi = _potentials(L, 1, L - 1), _potentials(L, 2, L - 1)
+    fe, fn = _vector(op, psi, chi)
+    ext = pe.extended_transform(op)
+    for trunc in (False, True):
+        rc, rd = pe.vector_curl_div(op, ext, fe, fn, truncate=trunc)
+        gc, gd = pl.so.vector_curl_div_spectral(torch.as_tensor(fe, device="cuda"),
+                                                torch.as_tensor(fn, device="cuda"),
+                                                truncate=trunc)
+        scale = max(np.abs(rc).max(), np.abs(rd).max())
+        ec, ed = np.abs(_np(gc) - rc).max() / scale, np.abs(_np(gd) - rd).max() / scale
+        print(f"  truncate={trunc}: curl {ec:.2e} div {ed:.2e}", flush=True)
+        assert ec < 1e-12 and ed < 1e-12
+    if kind == "latlon":
+        gc, gd = pl.so.vector_curl_div_spectral(torch.as_tensor(fe, device="cuda"),
+                                                torch.as_tensor(fn, device="cuda"), truncate=False)
+        lap = op.so.lap_eigs[:, None]
+        scale = max(np.abs(lap * psi).max(), np.abs(lap * chi).max())
+        assert np.abs(_np(gc) - lap * psi).max() < 1e-12 * scale
+        assert np.abs(_np(gd) - lap * chi).max() < 1e-12 * scale
+    # batched call: each field as alone
+    fe2 = torch.as_tensor(np.stack([fe, 2.0 * fn]), device="cuda")
+    fn2 = torch.as_tensor(np.stack([fn, -fe]), device="cuda")
+    bc, bd = pl.so.vector_curl_div_spectral(fe2, fn2)
+    for f in range(2):
+        sc, sd = pl.so.vector_curl_div_spectral(fe2[f], fn2[f])
+        assert torch.allclose(bc[f], sc, rtol=0, atol=1e-14 * float(sc.abs().max()))
+        assert torch.allclose(bd[f], sd, rtol=0, atol=1e-14 * float(sd.abs().max()))
+
+
+def test_vector_curl_div_zero_field_is_bitwise_zero():
+    import torch
+    pl, _ = _planets("latlon")
+    z = torch.zeros(pl.so.extended_product_transform().n_points, dtype=torch.float64,
+                    device="cuda")
+    c, d = pl.so.vector_curl_div_spectral(z, z)
+    assert float(c.abs().max()) == 0.0 and float(d.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("kind,sigma", [("latlon", 5), ("geodesic", 5), ("latlon", "nonuniform"),
+                                        ("latlon", 1)])
+def test_pe_tendency_matches_oracle(kind, sigma):
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
+    from oracle import pe
+    print(f"case pe tendency {kind} sigma={sigma}", flush=True)
+    pl, op = _planets(kind)
+    L = op.l_max
+    interfaces = NONUNIFORM if sigma == "nonuniform" else tuple(
+        k / sigma for k in range(sigma + 1))
+    K = len(interfaces) - 1
+    phi_s = None
+    if sigma == "nonuniform":
+        # band-limited terrain at the 2/3 cut
+        phi_s = _potentials(L, 9, (2 * L) // 3) * 1e-3
+        phi_s[:, (2 * L) // 3 + 1:] = 0.0
+        phi_s[0, 0] = 4000.0
+    m = PrimitiveEquationsModel(pl, SigmaGrid(interfaces), surface_geopotential_lm=phi_s)
+    om = pe.PEModel(op, pe.Sigma(interfaces), phi_surface_lm=phi_s)
+    y = _band_limited(L, K, 3)
+    got = _np(m.tendency(torch.as_tensor(y, device="cuda")))
+    ref = pe.pe_tendency(om, y)
+    blocks = {"zeta": slice(0, K), "delta": slice(K, 2 * K), "T": slice(2 * K, 3 * K),
+              "lnps": slice(3 * K, 3 * K + 1)}
+    errs = {b: np.abs(got[s] - ref[s]).max() / np.abs(ref[s]).max() for b, s in blocks.items()}
+    print("  block rel errors", {b: f"{e:.2e}" for b, e in errs.items()}, flush=True)
+    for b, e in errs.items():
+        assert e < 1e-11, b
+    assert not got[:2 * K, 0, :].any()
+
+
+@pytest.mark.parametrize("kind", ["latlon", "geodesic"])
+def test_pe_exact_rest_is_bitwise_zero(kind):
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid, isothermal_rest_state
+    from aeolus_amd.run.engine import rk4_step_array
+    pl, op = _planets(kind)
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(5))
+    st = isothermal_rest_state(op.l_max, 5, temperature=T0, surface_pressure=PS0, device="cuda")
+    out = m.tendency(st.coeffs)
+    assert tuple(out.shape) == tuple(st.coeffs.shape)
+    assert float(out.abs().max()) == 0.0
+    y1 = rk4_step_array(m.tendency, st.coeffs, 0.0, 600.0)
+    assert torch.equal(y1, st.coeffs)
+
+
+@pytest.mark.parametrize("kind", ["latlon", "geodesic"])
+def test_pe_bve_degeneracy(kind):
+    """delta = 0, per-level uniform T, uniform ln p_s: each level's zeta
+    tendency is the barotropic model's (the fused BVE kernels)."""
+    import torch
+    from aeolus_amd.physics import (BarotropicState, BarotropicVorticity,
+                                    PrimitiveEquationsModel, SigmaGrid)
+    from oracle import pe
+    pl, op = _planets(kind)
+    L = op.l_max
+    z = _potentials(L, 5, L) * 1e-11
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(4))
+    y = pe.isothermal_rest_state(L, 4, T0, PS0)
+    for k in range(4):
+        y[k] += z
+        y[8 + k, 0, 0] = (T0 - 6.0 * k) * SQRT4PI
+    out = _np(m.tendency(torch.as_tensor(y, device="cuda")))
+    bve = _np(BarotropicVorticity(pl, viscosity=0.0).tendency(
+        BarotropicState(torch.as_tensor(z, device="cuda"))))
+    scale = np.abs(bve).max()
+    errs = [np.abs(out[k] - bve).max() / scale for k in range(4)]
+    print(f"case pe bve degeneracy {kind}: {max(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-12
+
+
+def test_pe_zero_flow_structured_temperature():
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
+    from oracle import pe
+    pl, op = _planets("latlon")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(5))
+    y = pe.isothermal_rest_state(15, 5, T0, PS0)
+    for k in range(5):
+        y[10 + k, 0, 0] = (T0 - 7.0 * k) * SQRT4PI
+        y[10 + k, 4, 2] = 3.0 - 1.0j
+        y[10 + k, 2, 0] = 1.5 * (k + 1)
+    out = _np(m.tendency(torch.as_tensor(y, device="cuda")))
+    assert not out[:5].any()
+    phi, _ = pe.hydrostatic(pe.Sigma.uniform(5), y[10:15], np.zeros((16, 16), complex), pe.R_DRY)
+    expected = -op.so.lap_eigs[None, :, None] * phi
+    scale = np.abs(expected).max()
+    assert scale > 0 and np.abs(out[5:10] - expected).max() < 1e-12 * scale
+
+
+def test_pe_rejects_bad_inputs():
+    import torch
+    from aeolus_amd.physics import (PrimitiveEquationsModel, PrimitiveEquationsStateError,
+                                    SigmaGrid)
+    pl, _ = _planets("latlon")
+    with pytest.raises(ValueError):
+        PrimitiveEquationsModel(pl, SigmaGrid.uniform(3), r_dry=-1.0)
+    with pytest.raises(ValueError):
+        PrimitiveEquationsModel(pl, SigmaGrid.uniform(3), cp_dry=100.0)
+    high = np.zeros((16, 16), complex)
+    high[14, 3] = 1.0
+    with pytest.raises(ValueError):
+        PrimitiveEquationsModel(pl, SigmaGrid.uniform(3), surface_geopotential_lm=high)
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(3))
+    with pytest.raises(PrimitiveEquationsStateError):
+        m.tendency(torch.zeros((13, 16, 16), dtype=torch.complex128, device="cuda"))
