@@ -261,7 +261,8 @@ __device__ inline void grid_prologue(const GaussDev& g, const double2* __restric
     }
 }
 
-// Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh); block 256.
+// Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh, field
+// batches of row_cap rows); block 256.
 template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
                                                                const double2* __restrict__ E,
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
     const int cap = row_cap(n);
     const double2* twl = lds + (size_t)cap * rs;
     fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
-    for (int f0 = 0; f0 < nf; f0 += cap) {
+    for (int f0 = blockIdx.y * cap; f0 < nf; f0 += cap * gridDim.y) {
         const int nr = (nf - f0 < cap) ? nf - f0 : cap;
         for (int r = 0; r < nr; ++r) {
             pack_spectrum(lds + r * rs, n, L, E + row + f0 + r, O + row + f0 + r, nf, kSpecPlain,
@@ -294,7 +295,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
     }
 }
 
-// Analysis of nf grid fields: rows -> S/D (m <= mmax).  grid (nh); block 256.
+// Analysis of nf grid fields: rows -> S/D (m <= mmax).  grid (nh, field
+// batches of row_cap rows); block 256.
 template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
                                                                const double* __restrict__ f_in,
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
     const int cap = row_cap(n);
     const double2* twl = lds + (size_t)cap * rs;
     fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
-    for (int f0 = 0; f0 < nf; f0 += cap) {
+    for (int f0 = blockIdx.y * cap; f0 < nf; f0 += cap * gridDim.y) {
         const int nr = (nf - f0 < cap) ? nf - f0 : cap;
         for (int t = threadIdx.x; t < nr * n; t += kGridThreads) {
             const int r = t / n, k = t - r * n;
@@ -595,6 +597,12 @@ static size_t row_lds(const aeo_plan_s* p) {
     return sizeof(double2) * ((size_t)row_cap(p->nlon) * lds_row_stride(p->nlon) + p->nlon);
 }
 
+// Workgroups along the field axis of the plain transforms' grid kernels.
+static unsigned field_batches(const aeo_plan_s* p, int nf) {
+    const int cap = row_cap(p->nlon);
+    return (unsigned)((nf + cap - 1) / cap);
+}
+
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
     const int L = p->L;
     const size_t fs = (size_t)(L + 1) * (L + 1);
@@ -612,7 +620,8 @@ int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStrea
         if (rc) return rc;
         int rc2 = allow_lds((const void*)grid_syn_kernel_for(p), row_lds(p));
         if (rc2) return rc2;
-        hipLaunchKernelGGL(grid_syn_kernel_for(p), dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+        hipLaunchKernelGGL(grid_syn_kernel_for(p), dim3(p->nh, field_batches(p, nb)),
+                           dim3(kGridThreads), row_lds(p), s,
                            p->gdev(), p->d_E, p->d_O, nb, f + (size_t)f0 * p->nlat * p->nlon);
         AEO_LAUNCH_CHECK();
     }
@@ -626,7 +635,8 @@ int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
         int rc2 = allow_lds((const void*)grid_ana_kernel_for(p), row_lds(p));
         if (rc2) return rc2;
-        hipLaunchKernelGGL(grid_ana_kernel_for(p), dim3(p->nh), dim3(kGridThreads), row_lds(p), s,
+        hipLaunchKernelGGL(grid_ana_kernel_for(p), dim3(p->nh, field_batches(p, nb)),
+                           dim3(kGridThreads), row_lds(p), s,
                            p->gdev(), f + (size_t)f0 * p->nlat * p->nlon, nb, L, p->d_S, p->d_D);
         AEO_LAUNCH_CHECK();
         StageArgs sa{};

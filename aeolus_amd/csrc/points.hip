@@ -76,6 +76,11 @@ struct PwStore {  // generic synthesis
     }
 };
 
+__device__ inline PwStore at_group(const PwStore& x, int g) {
+    const int f0 = g * RawLoader::NF;
+    return PwStore{x.out + (size_t)f0 * x.npts, x.npts, x.nf - f0};
+}
+
 struct PwJac {  // v = (a_l, S a, b_l, S b) -> w J, with a_s = Syn(S a) / R
     double* gw;
     double R;
@@ -109,8 +114,10 @@ struct PwSwe {  // 12 synthesised fields -> w * (zeta_dot, curl, kinetic, phi_do
 // each wave sums its quarter of k, the four partials meet in LDS in a fixed
 // order, and the first wave applies the pointwise stage.
 template <class Loader, class Pw>
-__global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld, Pw pw) {
+__global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld0, Pw pw0) {
     constexpr int NF = Loader::NF;
+    const Loader ld = at_group(ld0, (int)blockIdx.y);
+    const Pw pw = at_group(pw0, (int)blockIdx.y);
     extern __shared__ double2 lds[];  // [NF][B] coefficient values times c_k | partials
     const int B = p.B;
     for (int t = threadIdx.x; t < NF * B; t += kPtsThreads) {
@@ -197,8 +204,13 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
 template <class Epi>
 __global__ __launch_bounds__(kPtsThreads) void points_finish_kernel(
     int L, int B, int nchunks, int nf_part, int nf_valid, const double2* __restrict__ part,
-    int lmax_a, int mmax_a, Epi ep) {
+    int lmax_a, int mmax_a, Epi ep0) {
     constexpr int NF = Epi::NF;
+    // field group blockIdx.y: its partials start NF * B into each chunk
+    const int g = blockIdx.y;
+    const Epi ep = at_group(ep0, g);
+    part += (size_t)g * NF * B;
+    nf_valid = (nf_valid - g * NF < NF) ? nf_valid - g * NF : NF;
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * (kPtsThreads / 64) + (threadIdx.x >> 6);
     const int n = L + 1;
@@ -433,12 +445,13 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_split_kernel(
 // Launchers
 // ---------------------------------------------------------------------------
 template <class Loader, class Pw>
-static int launch_psyn(const aeo_plan_s* p, const Loader& ld, const Pw& pw, hipStream_t s) {
+static int launch_psyn(const aeo_plan_s* p, const Loader& ld, const Pw& pw, hipStream_t s,
+                       int groups = 1) {
     const size_t lds = sizeof(double2) * Loader::NF * p->B +
                        sizeof(double) * kPtsSynSlices * Loader::NF * kPtsSynPts;
     AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
                 "point synthesis: LDS demand exceeds 160 KiB at this l_max");
-    dim3 grid((p->npts + kPtsSynPts - 1) / kPtsSynPts);
+    dim3 grid((p->npts + kPtsSynPts - 1) / kPtsSynPts, groups);
     if (lds > 65536) {
         AEO_CHECK_HIP(hipFuncSetAttribute((const void*)points_syn_kernel<Loader, Pw>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -524,10 +537,11 @@ static int launch_partial_ns(const aeo_plan_s* p, int ns, const double* vals, co
 
 template <class Epi>
 static int launch_finish(const aeo_plan_s* p, int nf_part, int nf_valid, const double2* part,
-                         int lmax_a, int mmax_a, const Epi& ep, hipStream_t s) {
+                         int lmax_a, int mmax_a, const Epi& ep, hipStream_t s, int groups = 1) {
     const int n = (p->L + 1) * (p->L + 1);
     constexpr int per = kPtsThreads / 64;   // entries (waves) per workgroup
-    hipLaunchKernelGGL(points_finish_kernel<Epi>, dim3((n + per - 1) / per), dim3(kPtsThreads), 0,
+    hipLaunchKernelGGL(points_finish_kernel<Epi>, dim3((n + per - 1) / per, groups),
+                       dim3(kPtsThreads), 0,
                        s, p->L, p->B, p->nchunks, nf_part, nf_valid, part, lmax_a, mmax_a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
@@ -550,14 +564,11 @@ int points_build_split_tables(aeo_plan_s* p, int ns, hipStream_t s) {
 
 int points_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
     const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
-    for (int g0 = 0; g0 < nf; g0 += RawLoader::NF) {
-        const int nb = (nf - g0 < RawLoader::NF) ? nf - g0 : RawLoader::NF;
-        RawLoader ld{a + g0 * fs, p->L, nb};
-        PwStore pw{f + (size_t)g0 * p->npts, p->npts, nb};
-        int rc = launch_psyn(p, ld, pw, s);
-        if (rc) return rc;
-    }
-    return AEO_OK;
+    // one launch: grid y runs over groups of RawLoader::NF fields
+    RawLoader ld{a, p->L, nf};
+    PwStore pw{f, p->npts, nf};
+    (void)fs;
+    return launch_psyn(p, ld, pw, s, (nf + RawLoader::NF - 1) / RawLoader::NF);
 }
 
 int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream_t s) {
@@ -566,14 +577,11 @@ int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStrea
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
         int rc = launch_partial(p, f + (size_t)f0 * p->npts, p->d_w, nb, p->d_part, s);
         if (rc) return rc;
-        for (int g0 = 0; g0 < nb; g0 += EpiStore::NF) {
-            const int ng = (nb - g0 < EpiStore::NF) ? nb - g0 : EpiStore::NF;
-            EpiStore ep{a + (size_t)(f0 + g0) * fs, p->L, ng};
-            // partials of this group start at field g0: view them as an nb-field array
-            // offset by g0 * B within each chunk.
-            rc = launch_finish(p, nb, ng, p->d_part + (size_t)g0 * p->B, p->L, p->L, ep, s);
-            if (rc) return rc;
-        }
+        // one launch: grid y runs over groups of EpiStore::NF fields
+        EpiStore ep{a + (size_t)f0 * fs, p->L, nb};
+        rc = launch_finish(p, nb, nb, p->d_part, p->L, p->L, ep, s,
+                           (nb + EpiStore::NF - 1) / EpiStore::NF);
+        if (rc) return rc;
     }
     return AEO_OK;
 }

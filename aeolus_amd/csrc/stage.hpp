@@ -254,6 +254,24 @@ struct EpiStore {
     }
 };
 
+// Field group g of a launch whose grid y index runs over groups of NF fields:
+// a loader or epilogue covering all of the launch's fields is narrowed to its
+// group; any other loader or epilogue covers one group and is used as given.
+template <class T>
+__device__ inline T at_group(const T& x, int) { return x; }
+
+__device__ inline RawLoader at_group(const RawLoader& x, int g) {
+    const int f0 = g * RawLoader::NF;
+    const int nf = (x.nf - f0 < RawLoader::NF) ? x.nf - f0 : RawLoader::NF;
+    return RawLoader{x.a + (size_t)f0 * (x.L + 1) * (x.L + 1), x.L, nf};
+}
+
+__device__ inline EpiStore at_group(const EpiStore& x, int g) {
+    const int f0 = g * EpiStore::NF;
+    const int nf = (x.nf - f0 < EpiStore::NF) ? x.nf - f0 : EpiStore::NF;
+    return EpiStore{x.out + (size_t)f0 * (x.L + 1) * (x.L + 1), x.L, nf};
+}
+
 struct EpiStore1 {
     static constexpr int NF = 1;
     double2* out;

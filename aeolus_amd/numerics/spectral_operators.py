@@ -108,21 +108,25 @@ class SpectralOperators:
         return self.sh.inv_transform(out)
 
     # -- weak-form vector analysis (num/spectral_operators.py:399-502) ------
+    def product_transform(self, l_max: int, max_fields: int = 16):
+        """A new transform of degree ``l_max`` on the product sampling (same
+        points, same weights) that takes ``max_fields`` fields per launch."""
+        sh_p = self._product_space.sh
+        with warnings.catch_warnings():
+            # a Gauss grid sized for degree L is under-resolved at L + 1; the
+            # quadrature is the product sampling's either way
+            warnings.simplefilter("ignore")
+            if isinstance(sh_p, GaussLatLonSphericalHarmonics):
+                return GaussLatLonSphericalHarmonics(sh_p.grid, l_max, max_fields=max_fields)
+            return PointSetSphericalHarmonics(sh_p.latitudes, sh_p.longitudes, l_max,
+                                              weights=sh_p.weights, max_fields=max_fields,
+                                              device=self.device)
+
     def extended_product_transform(self):
-        """The product-sampling transform one degree higher (same points, same
-        weights), built once (``num/spectral_operators.py:480-502``)."""
+        """The product-sampling transform one degree higher, built once
+        (``num/spectral_operators.py:480-502``)."""
         if self._ext is None:
-            sh_p = self._product_space.sh
-            with warnings.catch_warnings():
-                # a Gauss grid sized for degree L is under-resolved at L + 1; the
-                # quadrature is the product sampling's either way
-                warnings.simplefilter("ignore")
-                if isinstance(sh_p, GaussLatLonSphericalHarmonics):
-                    self._ext = GaussLatLonSphericalHarmonics(sh_p.grid, self.l_max + 1)
-                else:
-                    self._ext = PointSetSphericalHarmonics(sh_p.latitudes, sh_p.longitudes,
-                                                           self.l_max + 1, weights=sh_p.weights,
-                                                           device=self.device)
+            self._ext = self.product_transform(self.l_max + 1)
         return self._ext
 
     def vector_curl_div_spectral(self, f_east, f_north, *, truncate: bool = True):

@@ -152,11 +152,14 @@ class PrimitiveEquationsModel:
         self.f_lm = torch.zeros((n, n), dtype=torch.complex128, device=self.device)
         self.f_lm[1, 0] = 2.0 * self.Omega * math.sqrt(4.0 * math.pi / 3.0)
         self._ps = self.so.backend.product_space(self.so.product_quadrature)
-        self._ext = self.so.extended_product_transform()
+        # own product-sampling plans sized so that every batch of one tendency
+        # is a single synthesis / analysis launch sequence
+        self._prod = self.so.product_transform(self.l_max, max_fields=10 * self.nlev + 2)
+        self._ext = self.so.product_transform(self.l_max + 1, max_fields=2 * self.nlev)
         interfaces = np.ascontiguousarray(sigma.interfaces_array())
         handle = ctypes.c_void_p()
         nat.check(nat.lib().aeo_pe_create(
-            self._ps.sh._plan, self._ext._plan, self.nlev, nat.host_ptr(interfaces), self.R,
+            self._prod._plan, self._ext._plan, self.nlev, nat.host_ptr(interfaces), self.R,
             self.Omega, self.r_dry, self.cp_dry,
             None if surface_geopotential_lm is None else self.phi_surface_lm.data_ptr(),
             ctypes.byref(handle)), "PrimitiveEquationsModel")

@@ -66,6 +66,8 @@ def _args():
                    help="force the headline precision mode (0 = FP64, k = split slices)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 4 device copies of 256 MiB (scales profiler byte counters)")
+    p.add_argument("--pe-only", action="store_true",
+                   help="only the primitive-equation tendency / RK4 timings (one JSON line)")
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the 256-member uneven-L ensemble, graph replay (for profiler "
                         "runs; precision from --precision, default 6 slices)")
@@ -331,6 +333,58 @@ def geodesic_modes(steps, warmup):
     return res
 
 
+def _pe_state(L, K, seed):
+    """Isothermal rest (260 K, 101325 Pa) plus seeded perturbations band-limited
+    at the 2/3 cut (zeta, delta 2e-6 1/s, T 0.5 K, ln p_s 1e-3)."""
+    n, cut = L + 1, (2 * L) // 3
+    rng = np.random.default_rng(seed)
+    y = np.zeros((3 * K + 1, n, n), dtype=np.complex128)
+    y[2 * K:3 * K, 0, 0] = 260.0 * math.sqrt(4.0 * math.pi)
+    y[3 * K, 0, 0] = math.log(101325.0) * math.sqrt(4.0 * math.pi)
+
+    def pert(amp, zero_mono):
+        p = amp * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        p[:, 0] = p[:, 0].real
+        p *= np.tril(np.ones((n, n)))
+        p[cut + 1:, :] = 0.0
+        if zero_mono:
+            p[0, 0] = 0.0
+        return p
+    for k in range(K):
+        y[k] += pert(2e-6, True)
+        y[K + k] += pert(2e-6, True)
+        y[2 * K + k] += pert(0.5, False)
+    y[3 * K] += pert(1e-3, True)
+    return y
+
+
+def pe_modes(steps):
+    """Dry primitive-equation tendency (8 uniform sigma levels) and one RK4
+    step through rk4_step_array: on the reference's default PE configuration
+    (geodesic res 3 state / res 4 product, L = 10) and on a Gauss 128 x 256
+    grid at L = 85 (3/2-rule product)."""
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
+    from aeolus_amd.planet import Planet
+    from aeolus_amd.run.engine import rk4_step_array
+    res = {}
+    for name, kw, L in (("geodesic_res3_L10_K8", dict(grid_type="geodesic", grid_resolution=3), 10),
+                        ("gauss_128x256_L85_K8", dict(grid_type="latlon", nlat=128, nlon=256), 85)):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            pl = Planet.with_radius(R_EARTH, OMEGA, l_max=L, product_quadrature="fine", **kw)
+        m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(8))
+        y = torch.as_tensor(_pe_state(L, 8, 7), device="cuda")
+        ms_t = _timed(lambda k: [m.tendency(y) for _ in range(k)], steps, 5)
+        ms_rk = _timed(lambda k: [rk4_step_array(m.tendency, y, 0.0, 300.0) for _ in range(k)],
+                       max(steps // 4, 1), 2)
+        res[name] = {"ms_per_tendency": ms_t, "ms_per_rk4_step": ms_rk,
+                     "product_points": m._ext.n_points, "levels": 8,
+                     "fields_per_tendency": {"synthesised": 82, "analysed_L": 33,
+                                             "analysed_L_plus_1": 16}}
+    return res
+
+
 def adjoint_cost(model, ns):
     import torch
     from aeolus_amd.run.initial_conditions import spectrum_l15
@@ -406,6 +460,10 @@ def main():
     if args.ensemble_only:
         ensemble_only(args)
         return
+    if args.pe_only:
+        print(json.dumps({"workload": "dry primitive equations, 8 sigma levels",
+                          "pe_modes": pe_modes(min(args.steps, 100))}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "every figure of this benchmark is one GPU's; "
                                    "run with --gpus 1"}))
@@ -435,6 +493,7 @@ def main():
                                              ensemble_modes(48, ns_split).items()}
         secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
         secondary["geodesic_res4_L21_modes"] = geodesic_modes(args.steps, args.warmup)
+        secondary["primitive_equations_K8"] = pe_modes(min(args.steps, 100))
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
         "value": 1000.0 / h["ms_step"],
