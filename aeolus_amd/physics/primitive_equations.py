@@ -152,6 +152,10 @@ class PrimitiveEquationsModel:
         self.f_lm = torch.zeros((n, n), dtype=torch.complex128, device=self.device)
         self.f_lm[1, 0] = 2.0 * self.Omega * math.sqrt(4.0 * math.pi / 3.0)
         self._ps = self.so.backend.product_space(self.so.product_quadrature)
+        coslat = getattr(self.grid, "coslat", None)
+        if coslat is None:
+            coslat = torch.cos(torch.as_tensor(self.grid.point_latitudes))
+        self._state_coslat = torch.clamp(torch.as_tensor(coslat, device=self.device), min=1e-8)
         # own product-sampling plans sized so that every batch of one tendency
         # is a single synthesis / analysis launch sequence
         self._prod = self.so.product_transform(self.l_max, max_fields=10 * self.nlev + 2)
@@ -192,3 +196,97 @@ class PrimitiveEquationsModel:
 
     def tendency_state(self, state: PrimitiveEquationsState) -> PrimitiveEquationsState:
         return PrimitiveEquationsState(self.tendency(state.coeffs))
+
+    # -- state-grid fields (phy/primitive_equations.py:266-313, 472-480) ----
+    def _inv_laplacian(self, coeffs) -> torch.Tensor:
+        """lap^-1 per (L+1, L+1) slice with l = 0 zeroed (``:266-270``)."""
+        c = coeffs.to(torch.complex128).contiguous()
+        out = torch.empty_like(c)
+        nat.check(nat.lib().aeo_spec_op(nat.OP_INVLAP_ZERO, self.l_max, self.R, c.data_ptr(),
+                                        out.data_ptr(), c.numel() // (self.l_max + 1) ** 2,
+                                        nat.stream()), "inv_laplacian")
+        return out
+
+    def wind_on_state_grid(self, state: PrimitiveEquationsState):
+        """Per-level (u, v), each ``(nlev, N)``, on the state sampling: the
+        Helmholtz wind ``k x grad psi + grad chi`` (``:280-307``)."""
+        K = self.nlev
+        y = self._stack(state.coeffs)
+        pc = self._inv_laplacian(y[:2 * K])                     # psi_1..K, chi_1..K
+        lam = self.sh.inv_transform(self.so.d_lambda_coeffs(pc))
+        snt = self.sh.inv_transform(self.so.sin_theta_d_theta_coeffs(pc)) / self.R
+        c = self._state_coslat
+        return (snt[:K] + lam[K:]) / c, (lam[:K] - snt[K:]) / c
+
+    def temperature_on_state_grid(self, state: PrimitiveEquationsState) -> torch.Tensor:
+        return self.sh.inv_transform(self._stack(state.coeffs)[2 * self.nlev:3 * self.nlev])
+
+    def surface_pressure_on_state_grid(self, state: PrimitiveEquationsState) -> torch.Tensor:
+        """``exp(ln p_s)`` on the state grid (``:472-480``)."""
+        return torch.exp(self.sh.inv_transform(self._stack(state.coeffs)[3 * self.nlev]))
+
+    def _t_extrema_dev(self, y) -> torch.Tensor:
+        """Device ``[min, max]`` of T over every level and every sampling."""
+        from ..run.diagnostics import grid_stats
+        t = y[2 * self.nlev:3 * self.nlev]
+        samplings = [self.sh] + ([self._ps.sh] if self._ps.sh is not self.sh else [])
+        mm = [grid_stats(nat.GRID_STATS_MINMAX, None, sh.inv_transform(t))[:2] for sh in samplings]
+        lo = torch.stack([x[0] for x in mm]).min()
+        hi = torch.stack([x[1] for x in mm]).max()
+        return torch.stack([lo, hi])
+
+    def temperature_extrema(self, state: PrimitiveEquationsState) -> tuple[float, float]:
+        """(min, max) of T over every level on the state and product samplings
+        (``:764-782``); one host read."""
+        lo, hi = self._t_extrema_dev(self._stack(state.coeffs)).tolist()
+        return lo, hi
+
+    def max_characteristic_speed(self, state: PrimitiveEquationsState) -> float:
+        """``max_k max|V_k| + sqrt(gamma R_d max(T_max, 0))`` (``:784-806``);
+        one host read."""
+        y = self._stack(state.coeffs)
+        u, v = self.wind_on_state_grid(state)
+        vals = torch.stack([torch.sqrt(u * u + v * v).max(), self._t_extrema_dev(y)[1]]).tolist()
+        return vals[0] + math.sqrt(self.gamma * self.r_dry * max(vals[1], 0.0))
+
+    #: monopole tolerance relative to the per-level spectral norm (``:808-811``)
+    _MONOPOLE_RTOL = 1e-10
+
+    def validate_state(self, state: PrimitiveEquationsState, *, context: str = "") -> None:
+        """Raise :class:`PrimitiveEquationsStateError` on any hard violation
+        (``:813-854``): shape, finiteness, per-level zeta / delta monopoles,
+        strict temperature positivity over every sampling, finite p_s."""
+        where = f" {context}" if context else ""
+        n = self.l_max + 1
+        expected = (3 * self.nlev + 1, n, n)
+        c = state.coeffs
+        if tuple(c.shape) != expected:
+            raise PrimitiveEquationsStateError(
+                f"state shape {tuple(c.shape)} does not match the model's expected "
+                f"{expected}{where}")
+        y = torch.as_tensor(c, device=self.device).to(torch.complex128).contiguous()
+        K = self.nlev
+        if not bool(torch.isfinite(y).all()):
+            raise PrimitiveEquationsStateError(
+                f"primitive-equation state contains NaN/Inf coefficients{where}")
+        mono = y[:2 * K, 0, 0].abs()
+        scale = torch.linalg.norm(y[:2 * K].reshape(2 * K, -1), dim=1)
+        t_min = self._t_extrema_dev(y)[0]
+        ps_ok = torch.isfinite(torch.exp(self.sh.inv_transform(y[3 * K]))).all()
+        host = torch.cat([mono, scale, t_min[None], ps_ok[None].to(torch.float64)]).tolist()
+        for idx in range(2 * K):
+            name, k = ("zeta", idx) if idx < K else ("delta", idx - K)
+            m, sc = host[idx], host[2 * K + idx]
+            if m > self._MONOPOLE_RTOL * max(sc, 1e-30):
+                raise PrimitiveEquationsStateError(
+                    f"{name} monopole is nonzero at level {k + 1}{where}: |a00| = {m:g} "
+                    f"(field norm {sc:g}); the global mean of {name} must stay exactly zero "
+                    "on every level")
+        if not host[4 * K] > 0.0:
+            raise PrimitiveEquationsStateError(
+                f"temperature is not strictly positive{where}: min(T) = {host[4 * K]:g} K over "
+                "the state/product samplings")
+        if host[4 * K + 1] != 1.0:
+            raise PrimitiveEquationsStateError(
+                f"surface pressure exp(ln p_s) overflowed to a non-finite value on the state "
+                f"grid{where}")

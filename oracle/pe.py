@@ -17,7 +17,8 @@
   (``phy/primitive_equations.py:482-753``): product-grid reconstruction,
   thermodynamic and ln(p_s) tendencies, vector-invariant momentum tendencies
   with the spectral hydrostatic geopotential; :func:`isothermal_rest_state`
-  follows ``:139-161``.
+  follows ``:139-161``; :func:`state_wind`, :func:`temperature_extrema` and
+  :func:`max_characteristic_speed` follow ``:280-307, 764-806``.
 
 Pinned by (tests/test_oracle_pe.py): the reference's sigma-coordinate hand
 values (``tests/test_sigma_coordinate.py:76-242, 355-441``: alpha and log
@@ -28,7 +29,10 @@ closed forms ``curl F = lap psi``, ``div F = lap chi`` on the Gauss 3/2-rule
 grid to 1e-12 (``tests/test_pe_tendency.py:439-493``), and the tendency's
 exact properties: isothermal rest is bitwise zero, BVE degeneracy against
 :func:`oracle.models.bve_tendency` to 1e-12, zero flow with structured T
-gives ``-lap Phi`` (``tests/test_pe_tendency.py:802-1000``).
+gives ``-lap Phi`` (``tests/test_pe_tendency.py:802-1000``); the resting
+state's characteristic speed is the Lamb bound ``sqrt(gamma R_d T0)``
+(``tests/test_primitive_equations.py:428-441``) and a solid-body rotation
+``psi = -a R^2 Omega' sin(lat)`` has ``u = a Omega' R cos(lat)``, ``v = 0``.
 """
 from __future__ import annotations
 
@@ -42,7 +46,8 @@ from .transforms import DenseTransform, GaussTransform
 __all__ = ["R_DRY", "CP_DRY", "Sigma", "hydrostatic", "column_mass_tendency",
            "interface_sigma_dot", "vertical_advection", "omega_over_p",
            "layer_mass_residual", "energy_exchange", "vertical_sbp", "extended_transform",
-           "vector_curl_div", "isothermal_rest_state", "PEModel", "pe_tendency"]
+           "vector_curl_div", "isothermal_rest_state", "PEModel", "pe_tendency", "state_wind",
+           "temperature_extrema", "max_characteristic_speed"]
 
 R_DRY = 287.04          # phy/primitive_equations.py:52
 CP_DRY = 1004.64        # phy/primitive_equations.py:53
@@ -331,6 +336,37 @@ class PEModel:
         zd[:, 0, :] = 0.0
         dd[:, 0, :] = 0.0
         return zd, dd
+
+
+def state_wind(model: PEModel, y):
+    """Per-level (u, v) on the state sampling, clamped ``cos lat``
+    (``phy/primitive_equations.py:280-307``)."""
+    p, so, K = model.planet, model.planet.so, model.K
+    us, vs = [], []
+    for k in range(K):
+        psi, chi = model.inv_lap(y[k]), model.inv_lap(y[K + k])
+        p_lam = p.sh.inv_transform(so.d_lambda(psi))
+        p_snt = p.sh.inv_transform(so.sin_theta_d_theta(psi)) / so.R
+        c_lam = p.sh.inv_transform(so.d_lambda(chi))
+        c_snt = p.sh.inv_transform(so.sin_theta_d_theta(chi)) / so.R
+        us.append((p_snt + c_lam) / p.scoslat)
+        vs.append((p_lam - c_snt) / p.scoslat)
+    return np.stack(us), np.stack(vs)
+
+
+def temperature_extrema(model: PEModel, y):
+    """(min, max) of T over every level on the state and product samplings (``:764-782``)."""
+    p, K = model.planet, model.K
+    vals = [sh.inv_transform(y[2 * K + k]) for sh in (p.sh, p.psh) for k in range(K)]
+    return min(float(v.min()) for v in vals), max(float(v.max()) for v in vals)
+
+
+def max_characteristic_speed(model: PEModel, y) -> float:
+    """``max|V| + sqrt(gamma R_d max(T_max, 0))`` (``:784-806``)."""
+    u, v = state_wind(model, y)
+    _, t_max = temperature_extrema(model, y)
+    gamma = model.cp_dry / (model.cp_dry - model.r_dry)
+    return float(np.sqrt(u * u + v * v).max()) + math.sqrt(gamma * model.r_dry * max(t_max, 0.0))
 
 
 def pe_tendency(model: PEModel, y) -> np.ndarray:

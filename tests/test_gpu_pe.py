@@ -6,7 +6,10 @@ forms curl = lap psi, div = lap chi on the Gauss 3/2-rule grid); the PE
 tendency within 1e-11 of each row block's scale (zeta, delta, T, ln p_s) on
 the Gauss (L = 15, 32 x 64) and geodesic (L = 10, res 3) product samplings;
 exact rest bitwise zero; BVE degeneracy within 1e-12 of the barotropic
-tendency's scale; zero flow with structured T gives -lap(Phi) within 1e-12.
+tendency's scale; zero flow with structured T gives -lap(Phi) within 1e-12;
+state-grid wind, temperature extrema and characteristic speed within 1e-12 of
+the oracle (Lamb bound at rest); one validated RK4 step within 1e-11 of the
+oracle's increment.
 """
 import math
 import warnings
@@ -238,3 +241,84 @@ def test_pe_rejects_bad_inputs():
     m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(3))
     with pytest.raises(PrimitiveEquationsStateError):
         m.tendency(torch.zeros((13, 16, 16), dtype=torch.complex128, device="cuda"))
+
+
+@pytest.mark.parametrize("kind", ["latlon", "geodesic"])
+def test_pe_state_wind_extrema_and_speed_match_oracle(kind):
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, PrimitiveEquationsState, SigmaGrid
+    from oracle import pe
+    pl, op = _planets(kind)
+    L = op.l_max
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(5))
+    om = pe.PEModel(op, pe.Sigma.uniform(5))
+    y = _band_limited(L, 5, 11)
+    st = PrimitiveEquationsState(torch.as_tensor(y, device="cuda"))
+    u, v = m.wind_on_state_grid(st)
+    ru, rv = pe.state_wind(om, y)
+    scale = max(np.abs(ru).max(), np.abs(rv).max())
+    assert np.abs(_np(u) - ru).max() < 1e-12 * scale
+    assert np.abs(_np(v) - rv).max() < 1e-12 * scale
+    lo, hi = m.temperature_extrema(st)
+    rlo, rhi = pe.temperature_extrema(om, y)
+    assert lo == pytest.approx(rlo, rel=1e-12) and hi == pytest.approx(rhi, rel=1e-12)
+    assert m.max_characteristic_speed(st) == pytest.approx(pe.max_characteristic_speed(om, y),
+                                                           rel=1e-12)
+    rest = pe.isothermal_rest_state(L, 5, T0, PS0)
+    lamb = math.sqrt(m.gamma * m.r_dry * T0)
+    assert m.max_characteristic_speed(
+        PrimitiveEquationsState(torch.as_tensor(rest, device="cuda"))) == pytest.approx(
+        lamb, rel=1e-9)
+    m.validate_state(st)
+
+
+def test_pe_validation_rejects_invalid_states():
+    """The reference's hard failures (``tests/test_primitive_equations.py:446-492``)."""
+    import torch
+    from aeolus_amd.physics import (PrimitiveEquationsModel, PrimitiveEquationsState,
+                                    PrimitiveEquationsStateError, SigmaGrid,
+                                    isothermal_rest_state)
+    pl, _ = _planets("latlon")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(5))
+
+    def rest():
+        return isothermal_rest_state(15, 5, temperature=T0, surface_pressure=PS0, device="cuda")
+    m.validate_state(rest())
+    cases = []
+    b = rest(); b.temperature[1, 0, 0] = -50.0 * SQRT4PI; cases.append((b, "temperature"))
+    b = rest(); b.temperature[0, 0, 0] = 0.0; cases.append((b, "temperature"))
+    b = rest(); b.coeffs[0, 3, 2] = math.nan; cases.append((b, "NaN/Inf"))
+    b = rest(); b.zeta[2, 0, 0] = 1e-3; cases.append((b, "zeta monopole .* level 3"))
+    b = rest(); b.delta[0, 0, 0] = 1e-3; cases.append((b, "delta monopole"))
+    b = rest(); b.ln_ps[0, 0] = 1000.0 * SQRT4PI; cases.append((b, "surface pressure"))
+    cases.append((PrimitiveEquationsState.zeros(15, 6, device="cuda"), "shape"))
+    for st, msg in cases:
+        with pytest.raises(PrimitiveEquationsStateError, match=msg):
+            m.validate_state(st)
+
+
+def test_pe_rk4_step_with_stage_validation_matches_oracle():
+    """One RK4 step through rk4_step_array with validate_state on every stage
+    (the reference's run_pe step) against the oracle's RK4 of its tendency."""
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, PrimitiveEquationsState, SigmaGrid
+    from aeolus_amd.run.engine import rk4_step_array
+    from oracle import pe
+    from oracle.models import rk4_step_array as ork4
+    pl, op = _planets("latlon")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(5))
+    om = pe.PEModel(op, pe.Sigma.uniform(5))
+    y = _band_limited(15, 5, 13)
+    seen = []
+
+    def validator(ys):
+        seen.append(1)
+        m.validate_state(PrimitiveEquationsState(ys))
+    got = _np(rk4_step_array(m.tendency, torch.as_tensor(y, device="cuda"), 0.0, 300.0,
+                             stage_validator=validator))
+    ref = ork4(lambda s: pe.pe_tendency(om, s), y, 300.0)
+    assert len(seen) == 3
+    for s in (slice(0, 5), slice(5, 10), slice(10, 15), slice(15, 16)):
+        # 1e-11 of the step's increment, plus a few ulps of the state it is added to
+        d = ref[s] - y[s]
+        assert np.abs(got[s] - ref[s]).max() < 1e-11 * np.abs(d).max() + 4e-16 * np.abs(y[s]).max()

@@ -265,3 +265,22 @@ def test_monopoles_continuity_and_energy_exchange(latlon):
     conv, work = pe.energy_exchange(m.sigma, f["temperature"], f["phi_surface"], f["g_full"],
                                     f["v_grad_lnps"], m.r_dry)
     assert np.abs(conv - work).max() < 1e-12 * max(np.abs(conv).max(), np.abs(work).max())
+
+
+def test_state_wind_and_characteristic_speed(latlon):
+    """Solid-body rotation: zeta = 2 w sin(lat) gives u = w R cos(lat), v = 0;
+    the resting atmosphere's speed is the Lamb bound sqrt(gamma R_d T0)."""
+    m = pe.PEModel(latlon, pe.Sigma.uniform(3))
+    y = pe.isothermal_rest_state(15, 3, T0, PS0)
+    w = 1e-5
+    y[1, 1, 0] = 2.0 * w * math.sqrt(4.0 * math.pi / 3.0)
+    u, v = pe.state_wind(m, y)
+    lat = latlon.grid.point_latitudes
+    np.testing.assert_allclose(u[1], w * latlon.R * np.cos(lat), rtol=1e-10, atol=1e-9)
+    assert np.abs(v).max() < 1e-9 and np.abs(u[0]).max() == 0.0
+    gamma = pe.CP_DRY / (pe.CP_DRY - pe.R_DRY)
+    rest = pe.isothermal_rest_state(15, 3, T0, PS0)
+    lamb = math.sqrt(gamma * pe.R_DRY * T0)
+    assert pe.max_characteristic_speed(m, rest) == pytest.approx(lamb, rel=1e-9)
+    assert pe.max_characteristic_speed(m, y) == pytest.approx(
+        w * latlon.R * np.cos(lat).max() + lamb, rel=1e-9)
