@@ -6,7 +6,7 @@
 //   1. pe_prep_kernel     every coefficient field the product grid needs, per
 //                         level: U, V (the wind numerators), zeta, delta, T,
 //                         T_lambda, T_s, eta, eta_lambda, eta_s, plus the two
-//                         ln p_s derivatives (one thread per (l, m), all levels);
+//                         ln p_s derivatives (one thread per (l, m, level));
 //   2. synthesis          of those 10K + 2 fields on the product plan;
 //   3. pe_column_kernel   one thread per product point walks its column:
 //                         continuity (G, d ln p_s/dt, sigma_dot), omega/p,
@@ -16,7 +16,7 @@
 //   4. analysis           of 4K + 1 product fields on the product plan;
 //   5. analysis           of the 2K half-metric components on the degree-(L+1)
 //                         plan over the same points;
-//   6. pe_assemble_kernel one thread per (l, m): weak-form curl/div of the
+//   6. pe_assemble_kernel one thread per (l, m, level): weak-form curl/div of the
 //                         residual vector, 2/3 truncations, the spectral
 //                         hydrostatic geopotential and -lap(Phi + E), the
 //                         pinned l = 0 momentum rows.
@@ -36,6 +36,10 @@ enum PeSyn { kSynU = 0, kSynV, kSynZeta, kSynDelta, kSynT, kSynTLam, kSynTS, kSy
 //   [0K) Dsigma_k  [1K) alpha_k  [2K) beta_k / Dsigma_k (k >= 1)  [3K) 2 Dsigma_k
 //   [4K) R_d beta_k (k >= 1)  [5K) R_d alpha_k  [6K, 7K] sigma_{k-1/2}, k = 0..K
 constexpr int kSigConsts = 7;
+
+// One wave per workgroup for the column kernel: tens of thousands of columns
+// spread over every CU.
+constexpr int kColThreads = 64;
 
 __device__ inline double point_coslat(const double* c, int row_len, int i) {
     return fmax(row_len > 0 ? c[i / row_len] : c[i], 1e-8);
@@ -72,11 +76,13 @@ __global__ void pe_prep_kernel(int L, int K, double R, double f10, const double2
                                double2* __restrict__ prep) {
     const int n = L + 1, fs = n * n;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;            // level, or K for the ln p_s fields
     if (r >= fs) return;
     const int l = r / n, m = r - l * n;
-    const int nf = kSynPerLevel * K + 2;
+    const int nf = (k < K) ? kSynPerLevel : 2;
+    double2* o = prep + (size_t)kSynPerLevel * k * fs + r;
     if (m > l) {
-        for (int f = 0; f < nf; ++f) prep[(size_t)f * fs + r] = c2(0.0, 0.0);
+        for (int f = 0; f < nf; ++f) o[(size_t)f * fs] = c2(0.0, 0.0);
         return;
     }
     const bool lo = l >= 1, hi = l <= L - 1;
@@ -96,37 +102,36 @@ __global__ void pe_prep_kernel(int L, int K, double R, double f10, const double2
     };
     const double2 zero = c2(0.0, 0.0);
     auto at = [&](int row, int ll) { return y[(size_t)row * fs + ll * n + m]; };
-    for (int k = 0; k < K; ++k) {
-        const double2 z = at(k, l), zm = lo ? at(k, l - 1) : zero, zp = hi ? at(k, l + 1) : zero;
-        const double2 d = at(K + k, l), dm = lo ? at(K + k, l - 1) : zero,
-                      dp = hi ? at(K + k, l + 1) : zero;
-        const double2 t = at(2 * K + k, l), tm = lo ? at(2 * K + k, l - 1) : zero,
-                      tp = hi ? at(2 * K + k, l + 1) : zero;
-        const double2 ps = ilap(z, l), psm = lo ? ilap(zm, l - 1) : zero,
-                      psp = hi ? ilap(zp, l + 1) : zero;
-        const double2 ch = ilap(d, l), chm = lo ? ilap(dm, l - 1) : zero,
-                      chp = hi ? ilap(dp, l + 1) : zero;
-        // eta = zeta + f, f = 2 Omega sqrt(4 pi / 3) at (1, 0) only
-        auto eta = [&](double2 a, int ll) {
-            return (ll == 1 && m == 0) ? c2(a.x + f10, a.y + 0.0) : a;
-        };
-        double2* o = prep + (size_t)kSynPerLevel * k * fs + r;
-        o[kSynU * fs] = cadd(cdivr(sop(psm, psp), R), dlam(ch, m, R));
-        o[kSynV * fs] = csub(dlam(ps, m, R), cdivr(sop(chm, chp), R));
-        o[kSynZeta * fs] = z;
-        o[kSynDelta * fs] = d;
-        o[kSynT * fs] = t;
-        o[kSynTLam * fs] = dlam(t, m, R);
-        o[kSynTS * fs] = cdivr(sop(tm, tp), R);
-        o[kSynEta * fs] = eta(z, l);
-        o[kSynEtaLam * fs] = dlam(z, m, R);
-        o[kSynEtaS * fs] = cdivr(sop(lo ? eta(zm, l - 1) : zero, hi ? eta(zp, l + 1) : zero), R);
+    if (k == K) {
+        const double2 q = at(3 * K, l), qm = lo ? at(3 * K, l - 1) : zero,
+                      qp = hi ? at(3 * K, l + 1) : zero;
+        o[0] = dlam(q, m, R);
+        o[fs] = cdivr(sop(qm, qp), R);
+        return;
     }
-    const double2 q = at(3 * K, l), qm = lo ? at(3 * K, l - 1) : zero,
-                  qp = hi ? at(3 * K, l + 1) : zero;
-    double2* o = prep + (size_t)kSynPerLevel * K * fs + r;
-    o[0] = dlam(q, m, R);
-    o[fs] = cdivr(sop(qm, qp), R);
+    const double2 z = at(k, l), zm = lo ? at(k, l - 1) : zero, zp = hi ? at(k, l + 1) : zero;
+    const double2 d = at(K + k, l), dm = lo ? at(K + k, l - 1) : zero,
+                  dp = hi ? at(K + k, l + 1) : zero;
+    const double2 t = at(2 * K + k, l), tm = lo ? at(2 * K + k, l - 1) : zero,
+                  tp = hi ? at(2 * K + k, l + 1) : zero;
+    const double2 ps = ilap(z, l), psm = lo ? ilap(zm, l - 1) : zero,
+                  psp = hi ? ilap(zp, l + 1) : zero;
+    const double2 ch = ilap(d, l), chm = lo ? ilap(dm, l - 1) : zero,
+                  chp = hi ? ilap(dp, l + 1) : zero;
+    // eta = zeta + f, f = 2 Omega sqrt(4 pi / 3) at (1, 0) only
+    auto eta = [&](double2 a, int ll) {
+        return (ll == 1 && m == 0) ? c2(a.x + f10, a.y + 0.0) : a;
+    };
+    o[kSynU * fs] = cadd(cdivr(sop(psm, psp), R), dlam(ch, m, R));
+    o[kSynV * fs] = csub(dlam(ps, m, R), cdivr(sop(chm, chp), R));
+    o[kSynZeta * fs] = z;
+    o[kSynDelta * fs] = d;
+    o[kSynT * fs] = t;
+    o[kSynTLam * fs] = dlam(t, m, R);
+    o[kSynTS * fs] = cdivr(sop(tm, tp), R);
+    o[kSynEta * fs] = eta(z, l);
+    o[kSynEtaLam * fs] = dlam(z, m, R);
+    o[kSynEtaS * fs] = cdivr(sop(lo ? eta(zm, l - 1) : zero, hi ? eta(zp, l + 1) : zero), R);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,30 +226,32 @@ __global__ void pe_assemble_kernel(int L, int K, double R, const double* __restr
                                    const double2* __restrict__ spx, double2* __restrict__ out) {
     const int n = L + 1, fs = n * n, fx = (L + 2) * (L + 2);
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;            // level, or K for the ln p_s row
     if (r >= fs) return;
     const int l = r / n, m = r - l * n;
     const int cut = (2 * L) / 3;
+    if (k == K) {
+        out[(size_t)3 * K * fs + r] = trunc23(sp[(size_t)4 * K * fs + r], l, m, cut);
+        return;
+    }
     const double lap = -(double)l * (l + 1.0) / (R * R);
     const double* rb = sig + 4 * K;
     const double* ra = sig + 5 * K;
     const double2* T = y + (size_t)2 * K * fs + r;
-    // spectral Simmons-Burridge geopotential, bottom-up (phy/sigma_coordinate.py:164-209)
+    // spectral Simmons-Burridge geopotential: the interface recursion from the
+    // surface up to level k, in the reference's order (phy/sigma_coordinate.py:164-209)
     double2 pb = phis[r];
-    for (int k = K - 1; k >= 0; --k) {
-        if (k < K - 1) pb = cadd(pb, cscale(T[(size_t)(k + 1) * fs], rb[k + 1]));
-        const double2 phi = cadd(pb, cscale(T[(size_t)k * fs], ra[k]));
-        double2 curl_z, div_z;
-        weak_curl_div(spx + (size_t)k * fx, spx + (size_t)(K + k) * fx, l, m, L, R, &curl_z,
-                      &div_z);
-        const double2 zd = trunc23(csub(sp[(size_t)(K + k) * fs + r], curl_z), l, m, cut);
-        const double2 ke = trunc23(sp[(size_t)(3 * K + k) * fs + r], l, m, cut);
-        const double2 dd = csub(trunc23(csub(sp[(size_t)(2 * K + k) * fs + r], div_z), l, m, cut),
-                                cscale(cadd(ke, phi), lap));
-        out[(size_t)k * fs + r] = l == 0 ? c2(0.0, 0.0) : zd;
-        out[(size_t)(K + k) * fs + r] = l == 0 ? c2(0.0, 0.0) : dd;
-        out[(size_t)(2 * K + k) * fs + r] = trunc23(sp[(size_t)k * fs + r], l, m, cut);
-    }
-    out[(size_t)3 * K * fs + r] = trunc23(sp[(size_t)4 * K * fs + r], l, m, cut);
+    for (int j = K - 1; j > k; --j) pb = cadd(pb, cscale(T[(size_t)j * fs], rb[j]));
+    const double2 phi = cadd(pb, cscale(T[(size_t)k * fs], ra[k]));
+    double2 curl_z, div_z;
+    weak_curl_div(spx + (size_t)k * fx, spx + (size_t)(K + k) * fx, l, m, L, R, &curl_z, &div_z);
+    const double2 zd = trunc23(csub(sp[(size_t)(K + k) * fs + r], curl_z), l, m, cut);
+    const double2 ke = trunc23(sp[(size_t)(3 * K + k) * fs + r], l, m, cut);
+    const double2 dd = csub(trunc23(csub(sp[(size_t)(2 * K + k) * fs + r], div_z), l, m, cut),
+                            cscale(cadd(ke, phi), lap));
+    out[(size_t)k * fs + r] = l == 0 ? c2(0.0, 0.0) : zd;
+    out[(size_t)(K + k) * fs + r] = l == 0 ? c2(0.0, 0.0) : dd;
+    out[(size_t)(2 * K + k) * fs + r] = trunc23(sp[(size_t)k * fs + r], l, m, cut);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,7 +440,7 @@ int aeo_pe_tendency(aeo_pe_t pe, const double* y, double* out, void* stream) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int L = pe->L, K = pe->K, n = L + 1;
     const int nsyn = kSynPerLevel * K + 2;
-    hipLaunchKernelGGL(pe_prep_kernel, dim3(blocks_for((size_t)n * n)), dim3(256), 0, s, L, K,
+    hipLaunchKernelGGL(pe_prep_kernel, dim3(blocks_for((size_t)n * n), K + 1), dim3(256), 0, s, L, K,
                        pe->R, pe->f10, reinterpret_cast<const double2*>(y), pe->d_prep);
     AEO_LAUNCH_CHECK();
     int rc = aeo_synthesis(pe->prod, reinterpret_cast<const double*>(pe->d_prep), nsyn,
@@ -441,7 +448,8 @@ int aeo_pe_tendency(aeo_pe_t pe, const double* y, double* out, void* stream) {
     if (rc) return rc;
     int row_len = 0;
     const double* cl = plan_coslat(pe->prod, &row_len);
-    hipLaunchKernelGGL(pe_column_kernel, dim3(blocks_for(pe->npts)), dim3(256), 0, s, pe->npts,
+    hipLaunchKernelGGL(pe_column_kernel, dim3((pe->npts + kColThreads - 1) / kColThreads),
+                       dim3(kColThreads), 0, s, pe->npts,
                        row_len, cl, K, pe->d_sig, pe->kappa, pe->r_dry, pe->d_syn, pe->d_an,
                        pe->d_ex);
     AEO_LAUNCH_CHECK();
@@ -451,7 +459,8 @@ int aeo_pe_tendency(aeo_pe_t pe, const double* y, double* out, void* stream) {
     if ((rc = aeo_analysis(pe->ext, pe->d_ex, 2 * K, reinterpret_cast<double*>(pe->d_spx),
                            stream)))
         return rc;
-    hipLaunchKernelGGL(pe_assemble_kernel, dim3(blocks_for((size_t)n * n)), dim3(256), 0, s, L, K,
+    hipLaunchKernelGGL(pe_assemble_kernel, dim3(blocks_for((size_t)n * n), K + 1), dim3(256), 0, s,
+                       L, K,
                        pe->R, pe->d_sig, reinterpret_cast<const double2*>(y), pe->d_phis,
                        pe->d_sp, pe->d_spx, reinterpret_cast<double2*>(out));
     AEO_LAUNCH_CHECK();

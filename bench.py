@@ -66,7 +66,8 @@ def _args():
                    help="force the headline precision mode (0 = FP64, k = split slices)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 4 device copies of 256 MiB (scales profiler byte counters)")
-    p.add_argument("--pe-only", action="store_true",
+    p.add_argument("--pe-only", nargs="?", const="all", default=None,
+                   choices=["all", "geodesic", "gauss"],
                    help="only the primitive-equation tendency / RK4 timings (one JSON line)")
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the 256-member uneven-L ensemble, graph replay (for profiler "
@@ -358,7 +359,7 @@ def _pe_state(L, K, seed):
     return y
 
 
-def pe_modes(steps):
+def pe_modes(steps, which="all"):
     """Dry primitive-equation tendency (8 uniform sigma levels) and one RK4
     step through rk4_step_array: on the reference's default PE configuration
     (geodesic res 3 state / res 4 product, L = 10) and on a Gauss 128 x 256
@@ -370,6 +371,8 @@ def pe_modes(steps):
     res = {}
     for name, kw, L in (("geodesic_res3_L10_K8", dict(grid_type="geodesic", grid_resolution=3), 10),
                         ("gauss_128x256_L85_K8", dict(grid_type="latlon", nlat=128, nlon=256), 85)):
+        if which != "all" and not name.startswith(which):
+            continue
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
             pl = Planet.with_radius(R_EARTH, OMEGA, l_max=L, product_quadrature="fine", **kw)
@@ -426,10 +429,29 @@ def cpu_baseline(seconds: float):
         if time.perf_counter() - t0 >= seconds:
             break
     el = time.perf_counter() - t0
+    pe_ms = _pe_oracle_ms(min(3.0, seconds / 4))
     return {"value": n / el, "unit": "RK4 steps/s", "cores": int(blas),
+            "pe_geodesic_res3_L10_K8_ms_per_tendency": pe_ms,
             "host_cores_available": len(os.sched_getaffinity(0)), "kind": "port",
             "sample": f"{n} RK4 steps of the L=170 Gauss 256x512 BVE step (oracle NumPy "
                       f"FFT + stable-Legendre restatement, FP64), after 1 warm-up step"}
+
+
+def _pe_oracle_ms(seconds: float) -> float:
+    """Oracle primitive-equation tendency (NumPy, dense stable-recurrence basis)
+    on the reference's default PE case: geodesic res 3 (res 4 product), L = 10,
+    8 uniform levels; ms per tendency over about `seconds`."""
+    from oracle import pe
+    from oracle.models import Planet as OPlanet
+    op = OPlanet(l_max=10, radius=R_EARTH, omega=OMEGA, grid_type="geodesic", resolution=3)
+    m = pe.PEModel(op, pe.Sigma.uniform(8))
+    y = _pe_state(10, 8, 7)
+    pe.pe_tendency(m, y)
+    n, t0 = 0, time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < seconds:
+        pe.pe_tendency(m, y)
+        n += 1
+    return (time.perf_counter() - t0) * 1000.0 / n
 
 
 def _measured_traffic(kernel: str, precision: str):
@@ -462,7 +484,7 @@ def main():
         return
     if args.pe_only:
         print(json.dumps({"workload": "dry primitive equations, 8 sigma levels",
-                          "pe_modes": pe_modes(min(args.steps, 100))}))
+                          "pe_modes": pe_modes(min(args.steps, 100), args.pe_only)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "every figure of this benchmark is one GPU's; "
