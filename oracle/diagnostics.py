@@ -17,6 +17,11 @@ enstrophy of f, padding ignored).
 pinned (tests/test_oracle_diagnostics.py) by solid-body rotation (speed,
 energy, rms vorticity, CFL in closed form) and the resting layer (mass,
 energy, depth, free surface).
+
+``pe_record_row`` restates one row of ``PEDiagnosticsRecorder.record``
+(``run/pe/diagnostics.py:86-129``); pinned (tests/test_oracle_pe.py) by the
+resting atmosphere (uniform p_s: mass ``4 pi R^2 p_s`` and zero drift, zero
+wind, the Lamb-bound speed and its Courant number).
 """
 from __future__ import annotations
 
@@ -24,7 +29,8 @@ import math
 
 import numpy as np
 
-__all__ = ["mode_power", "spectral_diagnostics", "bve_record_row", "swe_record_row"]
+__all__ = ["mode_power", "spectral_diagnostics", "bve_record_row", "swe_record_row",
+           "pe_record_row"]
 
 
 def mode_power(a: np.ndarray) -> np.ndarray:
@@ -139,3 +145,24 @@ def swe_record_row(p, y: np.ndarray, dt: float, length_scale, terrain_max_m: flo
         "delta_l2": math.sqrt(R ** 2 * math.fsum(mode_power(delta).ravel())),
         "phi_l2": math.sqrt(R ** 2 * math.fsum(mode_power(phi).ravel())),
     }
+
+
+def pe_record_row(model, y: np.ndarray, dt: float, length_scale, mass0=None) -> dict:
+    """One PE diagnostics row (``run/pe/diagnostics.py:86-129``); ``model`` is an
+    ``oracle.pe.PEModel``, ``mass0`` the first row's total mass (None: this row)."""
+    from . import pe
+    p, K = model.planet, model.K
+    t_min, t_max = pe.temperature_extrema(model, y)
+    ps = np.exp(p.sh.inv_transform(y[3 * K]))
+    u, v = pe.state_wind(model, y)
+    zeta = np.stack([p.sh.inv_transform(y[k]) for k in range(K)])
+    delta = np.stack([p.sh.inv_transform(y[K + k]) for k in range(K)])
+    max_char = pe.max_characteristic_speed(model, y)
+    courant = max_char * dt / length_scale if (length_scale and dt > 0) else float("nan")
+    mass = math.fsum(p.sh.weights * p.R ** 2 * ps)
+    m0 = mass if mass0 is None else mass0
+    return {"t_min": t_min, "t_max": t_max, "ps_min": float(ps.min()), "ps_max": float(ps.max()),
+            "max_wind_ms": float(np.sqrt(u * u + v * v).max()),
+            "max_abs_zeta": float(np.abs(zeta).max()), "max_abs_delta": float(np.abs(delta).max()),
+            "max_char_speed_ms": max_char, "courant": courant, "total_mass": mass,
+            "mass_rel_drift": (mass - m0) / m0 if m0 else 0.0}

@@ -322,3 +322,48 @@ def test_pe_rk4_step_with_stage_validation_matches_oracle():
         # 1e-11 of the step's increment, plus a few ulps of the state it is added to
         d = ref[s] - y[s]
         assert np.abs(got[s] - ref[s]).max() < 1e-11 * np.abs(d).max() + 4e-16 * np.abs(y[s]).max()
+
+
+def test_run_pe_matches_oracle(tmp_path):
+    """run_pe (fixed 300 s step, count snapshots, stage validation) against
+    the oracle stepped with the same dt sequence: snapshots within 1e-12 of
+    each row block's scale, every diagnostics row within 1e-10."""
+    import csv
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, PrimitiveEquationsState, SigmaGrid
+    from aeolus_amd.run import IntegrationScheduler, run_pe
+    from oracle import pe
+    from oracle.diagnostics import pe_record_row
+    from oracle.models import rk4_step_array as ork4
+    print("case run_pe latlon L=15 K=4", flush=True)
+    pl, op = _planets("latlon")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(4))
+    om = pe.PEModel(op, pe.Sigma.uniform(4))
+    y0 = _band_limited(15, 4, 17)
+    times = [0.0, 1200.0, 2700.0]
+    run_pe(m, PrimitiveEquationsState(torch.as_tensor(y0, device="cuda")), dt_seconds=300.0,
+           t_end_days=0.03125, out_dir=tmp_path, snapshot_times=times)
+    got = np.load(tmp_path / "pe_coeffs.npy")
+    assert np.array_equal(np.load(tmp_path / "pe_snapshot_times.npy"), np.asarray(times))
+    rows = list(csv.DictReader(open(tmp_path / "diagnostics" / "timeseries.csv")))
+    sched = IntegrationScheduler(2700.0, mode="count", snapshot_times=times)
+    y, snaps, ref_rows = y0, [], [pe_record_row(om, y0, 0.0, pl.grid.cfl_length_scale)]
+    while (ev := sched.next_event(300.0)) is not None:
+        if ev[0] == "store":
+            snaps.append(y)
+            continue
+        y = ork4(lambda s: pe.pe_tendency(om, s), y, ev[1])
+        ref_rows.append(pe_record_row(om, y, ev[1], pl.grid.cfl_length_scale,
+                                      mass0=ref_rows[0]["total_mass"]))
+    assert got.shape == (3, 13, 16, 16) and len(rows) == len(ref_rows) == 10
+    for i, s in enumerate(snaps):
+        for b in (slice(0, 4), slice(4, 8), slice(8, 12), slice(12, 13)):
+            assert np.abs(got[i][b] - s[b]).max() <= 1e-12 * np.abs(s[b]).max(), (i, b)
+    for r, rr in zip(rows, ref_rows):
+        for key, val in rr.items():
+            if key == "mass_rel_drift":
+                assert abs(float(r[key]) - val) < 1e-13, key
+            elif key == "courant" and float(r["dt_s"]) == 0.0:
+                assert math.isnan(float(r[key]))
+            else:
+                assert float(r[key]) == pytest.approx(val, rel=1e-10, abs=1e-300), key

@@ -284,3 +284,18 @@ def test_state_wind_and_characteristic_speed(latlon):
     assert pe.max_characteristic_speed(m, rest) == pytest.approx(lamb, rel=1e-9)
     assert pe.max_characteristic_speed(m, y) == pytest.approx(
         w * latlon.R * np.cos(lat).max() + lamb, rel=1e-9)
+
+
+def test_pe_record_row_resting_atmosphere(latlon):
+    """Uniform p_s: total mass 4 pi R^2 p_s, zero drift and wind, Lamb speed."""
+    from oracle.diagnostics import pe_record_row
+    m = pe.PEModel(latlon, pe.Sigma.uniform(4))
+    y = pe.isothermal_rest_state(15, 4, T0, PS0)
+    row = pe_record_row(m, y, 300.0, 1.0e5)
+    assert row["total_mass"] == pytest.approx(4.0 * math.pi * latlon.R ** 2 * PS0, rel=1e-13)
+    assert row["mass_rel_drift"] == 0.0 and row["max_wind_ms"] == 0.0
+    assert row["ps_min"] == pytest.approx(PS0, rel=1e-13)
+    assert row["t_min"] == pytest.approx(T0, rel=1e-13)
+    lamb = math.sqrt(pe.CP_DRY / (pe.CP_DRY - pe.R_DRY) * pe.R_DRY * T0)
+    assert row["max_char_speed_ms"] == pytest.approx(lamb, rel=1e-12)
+    assert row["courant"] == pytest.approx(lamb * 300.0 / 1.0e5, rel=1e-12)
