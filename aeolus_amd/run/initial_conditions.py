@@ -88,3 +88,54 @@ def swe_spectra(l_max: int, radius: float, gravity: float, seed: int, *, u_rms: 
     u2 = float(np.sum(w * (power_per_degree(zeta) + power_per_degree(delta)))) / (4.0 * math.pi)
     s = u_rms / math.sqrt(u2)
     return np.stack([zeta * s, delta * s, gravity * h])
+
+
+# ---------------------------------------------------------------------------
+# Primitive-equation presets (run/pe/initial_conditions.py:57-131): built on
+# the exact isothermal rest state in spectral space (no grid round trip).
+# ---------------------------------------------------------------------------
+THERMAL_WAVE_DEGREE = 2
+THERMAL_WAVE_ORDER = 2
+
+
+def _pe_isothermal_rest(model, *, temperature, surface_pressure, thermal_amplitude=0.0):
+    from ..physics.primitive_equations import isothermal_rest_state
+    return isothermal_rest_state(model.l_max, model.nlev, temperature=temperature,
+                                 surface_pressure=surface_pressure, device=model.device)
+
+
+def _pe_thermal_wave(model, *, temperature, surface_pressure, thermal_amplitude):
+    """Rest plus one real coefficient ``thermal_amplitude`` (K) on the (2, 2)
+    mode at every level: unbalanced by design."""
+    if model.l_max < THERMAL_WAVE_DEGREE:
+        raise ValueError(f"thermal_wave needs l_max >= {THERMAL_WAVE_DEGREE}, got {model.l_max}")
+    st = _pe_isothermal_rest(model, temperature=temperature, surface_pressure=surface_pressure)
+    st.temperature[:, THERMAL_WAVE_DEGREE, THERMAL_WAVE_ORDER] = float(thermal_amplitude)
+    return st
+
+
+def _pe_orographic_isothermal_rest(model, *, temperature, surface_pressure,
+                                   thermal_amplitude=0.0):
+    """Rest balanced over the model's surface geopotential:
+    ``ln p_s = ln p_ref - Phi_s / (R_d T0)`` on the spectral coefficients."""
+    st = _pe_isothermal_rest(model, temperature=temperature, surface_pressure=surface_pressure)
+    st.coeffs[3 * model.nlev] -= model.phi_surface_lm / (model.r_dry * float(temperature))
+    return st
+
+
+PE_INITIAL_CONDITIONS = {
+    "isothermal_rest": _pe_isothermal_rest,
+    "thermal_wave": _pe_thermal_wave,
+    "orographic_isothermal_rest": _pe_orographic_isothermal_rest,
+}
+
+
+def make_pe_ic(name: str, model, *, temperature: float, surface_pressure: float,
+               thermal_amplitude: float = 0.0):
+    """A named primitive-equation initial state (``run/pe/initial_conditions.py:115-131``)."""
+    if name not in PE_INITIAL_CONDITIONS:
+        raise ValueError(f"Unknown pe initial condition: {name}. "
+                         f"Available: {sorted(PE_INITIAL_CONDITIONS)}")
+    return PE_INITIAL_CONDITIONS[name](model, temperature=temperature,
+                                       surface_pressure=surface_pressure,
+                                       thermal_amplitude=thermal_amplitude)

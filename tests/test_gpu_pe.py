@@ -367,3 +367,64 @@ def test_run_pe_matches_oracle(tmp_path):
                 assert math.isnan(float(r[key]))
             else:
                 assert float(r[key]) == pytest.approx(val, rel=1e-10, abs=1e-300), key
+
+
+def _terrain_model(kind):
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid, Topography
+    pl, _ = _planets(kind)
+    L = pl.sh.l_max
+    topo = Topography.mountain(pl, height_m=1500.0, lat_deg=25.0, lon_deg=60.0, width_deg=25.0,
+                               l_cut=(2 * L) // 3)
+    phi_s = topo.surface_geopotential_lm(9.80616)
+    return PrimitiveEquationsModel(pl, SigmaGrid.uniform(4), surface_geopotential_lm=phi_s)
+
+
+@pytest.mark.parametrize("kind,rtol", [("latlon", 1e-13), ("geodesic", 1e-2)])
+def test_pe_orographic_rest_is_balanced(kind, rtol):
+    """Isothermal rest over band-limited terrain: the pressure-gradient terms
+    cancel (``tests/test_pe_orographic_rest.py:471-502``): momentum blocks
+    within rtol of max|lap Phi_s|, T and ln p_s blocks bitwise zero."""
+    from aeolus_amd.run.initial_conditions import make_pe_ic
+    m = _terrain_model(kind)
+    st = make_pe_ic("orographic_isothermal_rest", m, temperature=T0, surface_pressure=PS0)
+    out = m.tendency(st.coeffs)
+    scale = float((m.lap_eigs[:, None] * m.phi_surface_lm).abs().max())
+    K = m.nlev
+    res = {"zeta": float(out[:K].abs().max()), "delta": float(out[K:2 * K].abs().max()),
+           "T": float(out[2 * K:3 * K].abs().max()), "lnps": float(out[3 * K].abs().max())}
+    print(f"case orographic rest {kind}: scale {scale:.3e}", res, flush=True)
+    assert scale > 0 and res["T"] == 0.0 and res["lnps"] == 0.0
+    assert res["zeta"] <= rtol * scale and res["delta"] <= rtol * scale
+
+
+def test_pe_orographic_rest_preserved_by_run(tmp_path):
+    """Five fixed 300 s steps through run_pe keep the Gauss orographic rest:
+    max wind <= 1e-12 m/s, mass drift <= 1e-14 (``tests/test_pe_orographic_rest.py:84-101``)."""
+    import csv
+    from aeolus_amd.run import run_pe
+    from aeolus_amd.run.initial_conditions import make_pe_ic
+    m = _terrain_model("latlon")
+    st = make_pe_ic("orographic_isothermal_rest", m, temperature=T0, surface_pressure=PS0)
+    run_pe(m, st, dt_seconds=300.0, t_end_days=1500.0 / 86400.0, out_dir=tmp_path,
+           snapshot_times=[0.0])
+    rows = list(csv.DictReader(open(tmp_path / "diagnostics" / "timeseries.csv")))
+    assert len(rows) == 6
+    for r in rows:
+        assert float(r["max_wind_ms"]) <= 1e-12
+        assert abs(float(r["mass_rel_drift"])) <= 1e-14
+
+
+def test_pe_thermal_wave_preset():
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
+    from aeolus_amd.run.initial_conditions import make_pe_ic
+    pl, _ = _planets("latlon")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(3))
+    st = make_pe_ic("thermal_wave", m, temperature=260.0, surface_pressure=101325.0,
+                    thermal_amplitude=1.0)
+    assert torch.all(st.temperature[:, 2, 2] == 1.0)
+    m.validate_state(st)
+    out = m.tendency(st.coeffs)
+    assert float(out[3:6].abs().max()) > 0.0          # the unbalanced T drives divergence
+    with pytest.raises(ValueError):
+        make_pe_ic("nope", m, temperature=260.0, surface_pressure=101325.0)
