@@ -240,28 +240,39 @@ class ShallowWaterModel:
         phi_max = max(h[2], self.phi0 + h[4])
         return h[0] + math.sqrt(max(phi_max, 0.0))
 
-    def total_geopotential_extrema(self, state: ShallowWaterState) -> tuple[float, float]:
-        phi_c = state.coeffs[PHI]
+    def _phi_extrema_dev(self, phi_c) -> torch.Tensor:
+        """Device ``[min, max]`` of phi over the state and product samplings."""
+        from ..run.diagnostics import grid_stats
         samplings = [self.sh] + ([self._ps.sh] if self._ps.sh is not self.sh else [])
-        lo, hi = math.inf, -math.inf
-        for sh in samplings:
-            g = sh.inv_transform(phi_c)
-            lo, hi = min(lo, float(g.min())), max(hi, float(g.max()))
+        mm = [grid_stats(nat.GRID_STATS_MINMAX, None, sh.inv_transform(phi_c))[:2]
+              for sh in samplings]
+        return torch.stack([torch.stack([x[0] for x in mm]).min(),
+                            torch.stack([x[1] for x in mm]).max()])
+
+    def total_geopotential_extrema(self, state: ShallowWaterState) -> tuple[float, float]:
+        """(min, max) of Phi0 + phi over every sampling (``:494-539``); one host read."""
+        lo, hi = self._phi_extrema_dev(self._stack(state.coeffs)[PHI]).tolist()
         return self.phi0 + lo, self.phi0 + hi
 
     def validate_state(self, state: ShallowWaterState, *, context: str = "") -> None:
+        """Finiteness, per-prognostic monopoles and strictly positive thickness
+        over every sampling (``:428-454``).  Every check is reduced on the
+        device; the verdict is one host read."""
         where = f" {context}" if context else ""
-        coeffs = state.coeffs
-        if not bool(torch.isfinite(coeffs).all()):
+        coeffs = self._stack(state.coeffs)
+        finite = torch.isfinite(coeffs).all()
+        mono = coeffs[:, 0, 0].abs()
+        scale = torch.linalg.norm(coeffs.reshape(3, -1), dim=1)
+        lo_dev = self._phi_extrema_dev(coeffs[PHI])[0]
+        host = torch.cat([finite[None].to(torch.float64), mono, scale, lo_dev[None]]).tolist()
+        if host[0] != 1.0:
             raise ShallowWaterStateError(f"shallow-water state contains NaN/Inf coefficients{where}")
         for idx, name in enumerate(PROGNOSTICS):
-            field = coeffs[idx]
-            mono = float(torch.abs(field[0, 0]))
-            scale = float(torch.linalg.norm(field))
-            if mono > self._MONOPOLE_RTOL * max(scale, 1e-30):
+            m, sc = host[1 + idx], host[4 + idx]
+            if m > self._MONOPOLE_RTOL * max(sc, 1e-30):
                 raise ShallowWaterStateError(
-                    f"{name} monopole is nonzero{where}: |a00| = {mono:g} (field norm {scale:g})")
-        lo, _ = self.total_geopotential_extrema(state)
+                    f"{name} monopole is nonzero{where}: |a00| = {m:g} (field norm {sc:g})")
+        lo = self.phi0 + host[7]
         if not lo > 0.0:
             raise ShallowWaterStateError(
                 f"fluid thickness is not strictly positive{where}: min(Phi0 + phi) = {lo:g} "

@@ -154,3 +154,26 @@ def test_bve_batch_members_match_single_runs(nmem):
 def BarotropicState_(c):
     from aeolus_amd.physics import BarotropicState
     return BarotropicState(c)
+
+
+def test_swe_validate_state_rejections():
+    """SWE hard validation (``phy/shallow_water.py:428-454``): NaN/Inf, each
+    prognostic's monopole, non-positive thickness on the state or product
+    sampling; a valid state passes."""
+    import torch
+    from aeolus_amd.physics import ShallowWaterModel, ShallowWaterState, ShallowWaterStateError
+    from aeolus_amd.run.initial_conditions import swe_spectra
+    pl, _ = _planets("latlon", 21, 32, 64)
+    m = ShallowWaterModel(pl, gravity=9.80616, mean_depth=1e4)
+    y0 = torch.as_tensor(swe_spectra(21, R_C1, 9.80616, seed=4), device="cuda")
+    m.validate_state(ShallowWaterState(y0))
+    cases = []
+    y = y0.clone(); y[0, 5, 3] = float("nan"); cases.append((y, "NaN/Inf"))
+    for f, name in enumerate(("zeta", "delta", "phi")):
+        y = y0.clone(); y[f, 0, 0] = 1.0; cases.append((y, f"{name} monopole"))
+    y = y0.clone(); y[2, 1, 0] = -3.0 * m.phi0; cases.append((y, "not strictly positive"))
+    for y, msg in cases:
+        with pytest.raises(ShallowWaterStateError, match=msg):
+            m.validate_state(ShallowWaterState(y))
+    lo, hi = m.total_geopotential_extrema(ShallowWaterState(y0))
+    assert 0.0 < lo < m.phi0 < hi
