@@ -597,6 +597,11 @@ static size_t row_lds(const aeo_plan_s* p) {
     return sizeof(double2) * ((size_t)row_cap(p->nlon) * lds_row_stride(p->nlon) + p->nlon);
 }
 
+// Fields per Legendre workgroup of a plain transform batch: each workgroup
+// streams its m column of the table once for all of its fields; a tile whose
+// LDS demand is too large is halved by the caller.
+static int batch_tile(int nf) { return nf >= 4 ? 4 : (nf >= 2 ? 2 : 1); }
+
 // Workgroups along the field axis of the plain transforms' grid kernels.
 static unsigned field_batches(const aeo_plan_s* p, int nf) {
     const int cap = row_cap(p->nlon);
@@ -616,7 +621,9 @@ int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStrea
         sa.E = p->d_E;
         sa.O = p->d_O;
         sa.k = col_consts(p, 1.0, nullptr, 0.0);
-        int rc = launch_stage<KindRaw, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
+        int rc = AEO_ERR_UNSUPPORTED;
+        for (int mb = batch_tile(nb); rc == AEO_ERR_UNSUPPORTED && mb >= 1; mb /= 2)
+            rc = launch_stage<KindRaw, EpiNoneS, false, true>(p, mb, sa, EpiNoneS{}, s);
         if (rc) return rc;
         int rc2 = allow_lds((const void*)grid_syn_kernel_for(p), row_lds(p));
         if (rc2) return rc2;
@@ -648,7 +655,9 @@ int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream
         sa.mmax_a = L;
         sa.k = col_consts(p, 1.0, nullptr, 0.0);
         EpiStore1S ep{a + (size_t)f0 * fs, L, fs};
-        int rc = launch_stage<KindNone, EpiStore1S, true, false>(p, 1, sa, ep, s);
+        int rc = AEO_ERR_UNSUPPORTED;
+        for (int mb = batch_tile(nb); rc == AEO_ERR_UNSUPPORTED && mb >= 1; mb /= 2)
+            rc = launch_stage<KindNone, EpiStore1S, true, false>(p, mb, sa, ep, s);
         if (rc) return rc;
     }
     return AEO_OK;
