@@ -33,7 +33,7 @@ enum PeSyn { kSynU = 0, kSynV, kSynZeta, kSynDelta, kSynT, kSynTLam, kSynTS, kSy
              kSynEtaLam, kSynEtaS, kSynPerLevel };
 
 // Sigma constants, [7K + 1] doubles:
-//   [0K) Dsigma_k  [1K) alpha_k  [2K) beta_k / Dsigma_k (k >= 1)  [3K) 2 Dsigma_k
+//   [0K) Dsigma_k  [1K) alpha_k  [2K) beta_k / Dsigma_k (k >= 1)  [3K) 1 / (2 Dsigma_k)
 //   [4K) R_d beta_k (k >= 1)  [5K) R_d alpha_k  [6K, 7K] sigma_{k-1/2}, k = 0..K
 constexpr int kSigConsts = 7;
 
@@ -145,40 +145,42 @@ __global__ void pe_column_kernel(int npts, int row_len, const double* __restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     const size_t N = npts;
-    const double c = point_coslat(coslat, row_len, i);
+    // one division per column: every 1/cos(lat) factor below is a product
+    const double ic = 1.0 / point_coslat(coslat, row_len, i);
     auto F = [&](int f) { return syn[(size_t)f * N + i]; };
     const double* th = sig;
     const double* al = sig + K;
     const double* cb = sig + 2 * K;
-    const double* th2 = sig + 3 * K;
+    const double* ith2 = sig + 3 * K;   // 1 / (2 Dsigma_k)
     const double* si = sig + 6 * K;
     const double ll = F(kSynPerLevel * K), ls = F(kSynPerLevel * K + 1);
 
     // column total of Dsigma_k G_k, G_k = delta_k + V_k . grad ln p_s
     double total = 0.0;
+#pragma unroll 8
     for (int k = 0; k < K; ++k) {
         const int b = kSynPerLevel * k;
-        const double u = F(b + kSynU) / c, v = F(b + kSynV) / c;
-        const double G = F(b + kSynDelta) + (u * ll - v * ls) / c;
+        const double u = F(b + kSynU) * ic, v = F(b + kSynV) * ic;
+        const double G = F(b + kSynDelta) + (u * ll - v * ls) * ic;
         total = (k == 0) ? th[0] * G : total + th[k] * G;
     }
 
     double partial_prev = 0.0, sd_top = 0.0;
     double u0 = 0.0, v0 = 0.0, t0 = 0.0;
-    double u1 = F(kSynU) / c, v1 = F(kSynV) / c, t1 = F(kSynT);
+    double u1 = F(kSynU) * ic, v1 = F(kSynV) * ic, t1 = F(kSynT);
+    // unrolled so that several levels' loads are in flight at once (a column
+    // is one thread; there are few waves per CU to hide the latency)
+#pragma unroll 4
     for (int k = 0; k < K; ++k) {
         const int b = kSynPerLevel * k;
         const double d = F(b + kSynDelta), z = F(b + kSynZeta);
-        const double A = (u1 * ll - v1 * ls) / c;
+        const double A = (u1 * ll - v1 * ls) * ic;
         const double G = d + A;
         const double partial = (k == 0) ? th[0] * G : partial_prev + th[k] * G;
         const double sd_bot = (k == K - 1) ? total - total : si[k + 1] * total - partial;
-        double u2 = 0.0, v2 = 0.0, t2 = 0.0;
-        if (k + 1 < K) {
-            u2 = F(b + kSynPerLevel + kSynU) / c;
-            v2 = F(b + kSynPerLevel + kSynV) / c;
-            t2 = F(b + kSynPerLevel + kSynT);
-        }
+        // next level (clamped so the loads are unconditional; unused at the bottom)
+        const int bn = kSynPerLevel * ((k + 1 < K) ? k + 1 : k);
+        const double u2 = F(bn + kSynU) * ic, v2 = F(bn + kSynV) * ic, t2 = F(bn + kSynT);
         // centred (sigma_dot dX/dsigma)_k; boundary interfaces never read
         auto vadv = [&](double xm, double x0, double xp) {
             double acc = 0.0;
@@ -189,26 +191,26 @@ __global__ void pe_column_kernel(int npts, int row_len, const double* __restrict
                 acc = have ? acc + up : up;
                 have = true;
             }
-            return have ? acc / th2[k] : 0.0;
+            return have ? acc * ith2[k] : 0.0;
         };
         const double vu = vadv(u0, u1, u2), vv = vadv(v0, v1, v2), vt = vadv(t0, t1, t2);
         const double wp = (k == 0) ? A - al[0] * G : (A - cb[k] * partial_prev) - al[k] * G;
         // thermodynamic: -V.grad T - V_adv(T) + kappa T omega/p
-        const double advT = (u1 * F(b + kSynTLam) - v1 * F(b + kSynTS)) / c;
+        const double advT = (u1 * F(b + kSynTLam) - v1 * F(b + kSynTS)) * ic;
         an[(size_t)k * N + i] = (-advT - vt) + (kappa * t1) * wp;
         // eta V: div = u.grad eta + eta delta, k.curl = eta zeta + (grad eta x V).k
         const double eg = F(b + kSynEta), el = F(b + kSynEtaLam), es = F(b + kSynEtaS);
-        const double div_ev = (u1 * el - v1 * es) / c + eg * d;
-        const double curl_ev = eg * z + (v1 * el + u1 * es) / c;
+        const double div_ev = (u1 * el - v1 * es) * ic + eg * d;
+        const double curl_ev = eg * z + (v1 * el + u1 * es) * ic;
         an[(size_t)(K + k) * N + i] = -div_ev;
         an[(size_t)(2 * K + k) * N + i] = curl_ev;
         an[(size_t)(3 * K + k) * N + i] = 0.5 * (u1 * u1 + v1 * v1);
         // residual vector Z = sigma_dot dV/dsigma + R_d T grad ln p_s, half-metric
         const double rt = r_dry * t1;
-        const double ze = vu + rt * (ll / c);
-        const double zn = vv + rt * (-ls / c);
-        ex[(size_t)k * N + i] = ze / c;
-        ex[(size_t)(K + k) * N + i] = zn / c;
+        const double ze = vu + rt * (ll * ic);
+        const double zn = vv + rt * (-ls * ic);
+        ex[(size_t)k * N + i] = ze * ic;
+        ex[(size_t)(K + k) * N + i] = zn * ic;
         u0 = u1; v0 = v1; t0 = t1;
         u1 = u2; v1 = v2; t1 = t2;
         sd_top = sd_bot;
@@ -383,7 +385,7 @@ int aeo_pe_create(aeo_plan_t prod, aeo_plan_t ext, int nlev, const double* inter
         sig[k] = th;
         sig[K + k] = alpha;
         sig[2 * K + k] = k == 0 ? 0.0 : beta / th;
-        sig[3 * K + k] = 2.0 * th;
+        sig[3 * K + k] = 1.0 / (2.0 * th);
         sig[4 * K + k] = k == 0 ? 0.0 : r_dry * beta;
         sig[5 * K + k] = r_dry * alpha;
     }
