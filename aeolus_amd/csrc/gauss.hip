@@ -262,18 +262,18 @@ __device__ inline void grid_prologue(const GaussDev& g, const double2* __restric
 }
 
 // Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh, field
-// batches of row_cap rows); block 256.
+// batches of `cap` rows, cap <= row_cap); block 256.
 template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
                                                                const double2* __restrict__ E,
                                                                const double2* __restrict__ O,
-                                                               int nf, double* __restrict__ f_out) {
+                                                               int nf, int cap,
+                                                               double* __restrict__ f_out) {
     extern __shared__ double2 lds[];
     const int jp = blockIdx.x;
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t row = (size_t)jp * (L + 1) * nf;
-    const int cap = row_cap(n);
     const double2* twl = lds + (size_t)cap * rs;
     fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
     for (int f0 = blockIdx.y * cap; f0 < nf; f0 += cap * gridDim.y) {
@@ -296,11 +296,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_syn_kernel(GaussDev g,
 }
 
 // Analysis of nf grid fields: rows -> S/D (m <= mmax).  grid (nh, field
-// batches of row_cap rows); block 256.
+// batches of `cap` rows, cap <= row_cap); block 256.
 template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
                                                                const double* __restrict__ f_in,
-                                                               int nf, int mmax,
+                                                               int nf, int cap, int mmax,
                                                                double2* __restrict__ S,
                                                                double2* __restrict__ D) {
     extern __shared__ double2 lds[];
@@ -309,7 +309,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_ana_kernel(GaussDev g,
     const int jN = jp, jS = g.nlat - 1 - jp;
     const size_t row = (size_t)jp * (L + 1) * nf;
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
-    const int cap = row_cap(n);
     const double2* twl = lds + (size_t)cap * rs;
     fft_twiddles_to_lds(g.fft, lds + (size_t)cap * rs);
     for (int f0 = blockIdx.y * cap; f0 < nf; f0 += cap * gridDim.y) {
@@ -592,9 +591,18 @@ static size_t grid_lds(const aeo_plan_s* p, int rows, int nf) {
 
 static size_t jac_lds(const aeo_plan_s* p) { return grid_lds(p, 4, 4); }
 
-static size_t row_lds(const aeo_plan_s* p) {
+// Rows per workgroup of the plain transforms' grid kernels for a batch of nf
+// fields: enough workgroups (nh x batches >= 512) to fill the chip, at most
+// row_cap rows.
+static int rows_per_wg(const aeo_plan_s* p, int nf) {
+    const int want = (p->nh * nf + 511) / 512;
+    const int cap = row_cap(p->nlon);
+    return want < 1 ? 1 : (want > cap ? cap : want);
+}
+
+static size_t row_lds(const aeo_plan_s* p, int rows) {
     // padded rows of one LDS pass + the twiddle table
-    return sizeof(double2) * ((size_t)row_cap(p->nlon) * lds_row_stride(p->nlon) + p->nlon);
+    return sizeof(double2) * ((size_t)rows * lds_row_stride(p->nlon) + p->nlon);
 }
 
 // Fields per Legendre workgroup of a plain transform batch: each workgroup
@@ -603,10 +611,7 @@ static size_t row_lds(const aeo_plan_s* p) {
 static int batch_tile(int nf) { return nf >= 4 ? 4 : (nf >= 2 ? 2 : 1); }
 
 // Workgroups along the field axis of the plain transforms' grid kernels.
-static unsigned field_batches(const aeo_plan_s* p, int nf) {
-    const int cap = row_cap(p->nlon);
-    return (unsigned)((nf + cap - 1) / cap);
-}
+static unsigned field_batches(int nf, int rows) { return (unsigned)((nf + rows - 1) / rows); }
 
 int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStream_t s) {
     const int L = p->L;
@@ -625,11 +630,12 @@ int gauss_synthesis(aeo_plan_s* p, const double2* a, int nf, double* f, hipStrea
         for (int mb = batch_tile(nb); rc == AEO_ERR_UNSUPPORTED && mb >= 1; mb /= 2)
             rc = launch_stage<KindRaw, EpiNoneS, false, true>(p, mb, sa, EpiNoneS{}, s);
         if (rc) return rc;
-        int rc2 = allow_lds((const void*)grid_syn_kernel_for(p), row_lds(p));
+        const int rows = rows_per_wg(p, nb);
+        int rc2 = allow_lds((const void*)grid_syn_kernel_for(p), row_lds(p, rows));
         if (rc2) return rc2;
-        hipLaunchKernelGGL(grid_syn_kernel_for(p), dim3(p->nh, field_batches(p, nb)),
-                           dim3(kGridThreads), row_lds(p), s,
-                           p->gdev(), p->d_E, p->d_O, nb, f + (size_t)f0 * p->nlat * p->nlon);
+        hipLaunchKernelGGL(grid_syn_kernel_for(p), dim3(p->nh, field_batches(nb, rows)),
+                           dim3(kGridThreads), row_lds(p, rows), s, p->gdev(), p->d_E, p->d_O,
+                           nb, rows, f + (size_t)f0 * p->nlat * p->nlon);
         AEO_LAUNCH_CHECK();
     }
     return AEO_OK;
@@ -640,11 +646,12 @@ int gauss_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStream
     const size_t fs = (size_t)(L + 1) * (L + 1);
     for (int f0 = 0; f0 < nf; f0 += p->max_fields) {
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
-        int rc2 = allow_lds((const void*)grid_ana_kernel_for(p), row_lds(p));
+        const int rows = rows_per_wg(p, nb);
+        int rc2 = allow_lds((const void*)grid_ana_kernel_for(p), row_lds(p, rows));
         if (rc2) return rc2;
-        hipLaunchKernelGGL(grid_ana_kernel_for(p), dim3(p->nh, field_batches(p, nb)),
-                           dim3(kGridThreads), row_lds(p), s,
-                           p->gdev(), f + (size_t)f0 * p->nlat * p->nlon, nb, L, p->d_S, p->d_D);
+        hipLaunchKernelGGL(grid_ana_kernel_for(p), dim3(p->nh, field_batches(nb, rows)),
+                           dim3(kGridThreads), row_lds(p, rows), s, p->gdev(),
+                           f + (size_t)f0 * p->nlat * p->nlon, nb, rows, L, p->d_S, p->d_D);
         AEO_LAUNCH_CHECK();
         StageArgs sa{};
         sa.nmem = nb;
