@@ -428,3 +428,36 @@ def test_pe_thermal_wave_preset():
     assert float(out[3:6].abs().max()) > 0.0          # the unbalanced T drives divergence
     with pytest.raises(ValueError):
         make_pe_ic("nope", m, temperature=260.0, surface_pressure=101325.0)
+
+
+def test_pe_create_rejects_mismatched_plans():
+    """aeo_pe_create checks that the extended plan is degree L + 1 on the
+    product plan's points; aeo_vector_curl_div checks its arguments."""
+    import ctypes
+    import torch
+    from aeolus_amd import _native as nat
+    pl, _ = _planets("latlon")
+    so = pl.so
+    prod = so.product_transform(15, max_fields=8)
+    interfaces = np.asarray([0.0, 0.5, 1.0])
+    h = ctypes.c_void_p()
+    for ext in (so.product_transform(17), so.product_transform(15)):
+        rc = nat.lib().aeo_pe_create(prod._plan, ext._plan, 2, nat.host_ptr(interfaces), 6.4e6,
+                                     7.29e-5, 287.04, 1004.64, None, ctypes.byref(h))
+        assert rc == nat.AEO_ERR_INVALID and not h.value
+    ext = so.product_transform(16)
+    for bad in (np.asarray([0.1, 0.5, 1.0]), np.asarray([0.0, 0.5, 0.9]),
+                np.asarray([0.0, 0.6, 0.6])):
+        rc = nat.lib().aeo_pe_create(prod._plan, ext._plan, 2, nat.host_ptr(bad), 6.4e6,
+                                     7.29e-5, 287.04, 1004.64, None, ctypes.byref(h))
+        assert rc == nat.AEO_ERR_INVALID and not h.value
+    rc = nat.lib().aeo_pe_create(prod._plan, ext._plan, 2, nat.host_ptr(interfaces), 6.4e6,
+                                 7.29e-5, 287.04, 200.0, None, ctypes.byref(h))
+    assert rc == nat.AEO_ERR_INVALID
+    z = torch.zeros(ext.n_points, dtype=torch.float64, device="cuda")
+    out = torch.empty((16, 16), dtype=torch.complex128, device="cuda")
+    rc = nat.lib().aeo_vector_curl_div(ext._plan, -1.0, z.data_ptr(), z.data_ptr(), 1, 1,
+                                       out.data_ptr(), out.data_ptr(), z.data_ptr(), nat.stream())
+    assert rc == nat.AEO_ERR_INVALID
+    with pytest.raises(ValueError):
+        so.vector_curl_div_spectral(z[:-1], z[:-1])
