@@ -198,6 +198,29 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
     }
 }
 
+// Plain analysis output for few chunks: one thread per (field, l, m) sums the
+// chunk partials in chunk order (padding m > l written as 0).  grid
+// ceil(nf (L+1)^2 / 256); used when nchunks <= kPtsSeqChunks, where the
+// wave-per-entry tree below is bound by its shuffle chain.
+constexpr int kPtsSeqChunks = 32;
+
+__global__ __launch_bounds__(kPtsThreads) void points_store_sum_kernel(
+    int L, int B, int nchunks, int nf, const double2* __restrict__ part,
+    double2* __restrict__ out) {
+    const int n = L + 1, fs = n * n;
+    const size_t t = (size_t)blockIdx.x * kPtsThreads + threadIdx.x;
+    if (t >= (size_t)nf * fs) return;
+    const int f = (int)(t / fs), r = (int)(t - (size_t)f * fs);
+    const int l = r / n, m = r - l * n;
+    double2 acc = c2(0.0, 0.0);
+    if (m <= l) {
+        const int k = l * (l + 1) / 2 + m;
+#pragma unroll 8
+        for (int c = 0; c < nchunks; ++c) acc = cadd(acc, part[((size_t)c * nf + f) * B + k]);
+    }
+    out[t] = acc;
+}
+
 // Sum the chunk partials and run the epilogue for every (l, m): one wave per
 // entry, lanes over chunks, a fixed shuffle tree (deterministic), lane 0
 // applies the epilogue.
@@ -577,6 +600,15 @@ int points_analysis(aeo_plan_s* p, const double* f, int nf, double2* a, hipStrea
         const int nb = (nf - f0 < p->max_fields) ? nf - f0 : p->max_fields;
         int rc = launch_partial(p, f + (size_t)f0 * p->npts, p->d_w, nb, p->d_part, s);
         if (rc) return rc;
+        if (p->nchunks <= kPtsSeqChunks) {
+            const size_t tot = (size_t)nb * fs;
+            hipLaunchKernelGGL(points_store_sum_kernel, dim3((unsigned)((tot + kPtsThreads - 1) /
+                                                                        kPtsThreads)),
+                               dim3(kPtsThreads), 0, s, p->L, p->B, p->nchunks, nb, p->d_part,
+                               a + (size_t)f0 * fs);
+            AEO_LAUNCH_CHECK();
+            continue;
+        }
         // one launch: grid y runs over groups of EpiStore::NF fields
         EpiStore ep{a + (size_t)f0 * fs, p->L, nb};
         rc = launch_finish(p, nb, nb, p->d_part, p->L, p->L, ep, s,
