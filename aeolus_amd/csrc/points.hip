@@ -171,17 +171,21 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_kernel(
     PointsDev p, const double* __restrict__ vals, const double* __restrict__ weights,
     int nf, double2* __restrict__ part) {
     __shared__ double2 red[4][64];
+    __shared__ double gs[kPtsChunk];   // the chunk's weighted values, one vector load each
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int k = blockIdx.x * 64 + lane;
     const int chunk = blockIdx.y, f = blockIdx.z;
     const int i0 = chunk * kPtsChunk;
     const int i1 = (i0 + kPtsChunk < p.npts) ? i0 + kPtsChunk : p.npts;
     const double* v = vals + (size_t)f * p.npts;
+    for (int t = threadIdx.x; t < i1 - i0; t += kPtsThreads)
+        gs[t] = weights ? v[i0 + t] * weights[i0 + t] : v[i0 + t];
+    __syncthreads();
     double2 acc = c2(0.0, 0.0);
     if (k < p.B) {
 #pragma unroll 8
         for (int i = i0 + q; i < i1; i += 4) {
-            const double g = weights ? v[i] * weights[i] : v[i];
+            const double g = gs[i - i0];
             const double2 y = p.Yp[(size_t)i * p.B + k];
             acc.x = fma(y.x, g, acc.x);
             acc.y = fma(-y.y, g, acc.y);
