@@ -138,6 +138,7 @@ __global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Lo
     for (int f = 0; f < NF; ++f) acc[f] = 0.0;
     const double* yr = p.Yk + ic;
     const double* yi = p.Yk + (size_t)B * p.npts + ic;
+#pragma unroll 8
     for (int k = k0; k < k1; ++k) {
         const double a = yr[(size_t)k * p.npts];
         const double b = yi[(size_t)k * p.npts];
