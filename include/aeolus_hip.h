@@ -277,7 +277,9 @@ int aeo_grid_stats(int mode, int npts, const double* w, const double* a, const d
  * l = L; outputs (nfields, L+1, L+1) complex128, 2/3-truncated when
  * `truncate` != 0; cos lat clamped at 1e-8.  `ext` is a plan of degree L+1 on
  * the product sampling (same points and weights).  `work`: device, at least
- * aeo_vector_curl_div_work(ext, nfields) doubles.
+ * aeo_vector_curl_div_work(ext, nfields) doubles.  The analysis uses the
+ * plan's own workspace: calls on one plan must not overlap on different
+ * streams.
  * Replaces SpectralOperators.vector_curl_div_spectral and
  * _vector_analysis_pieces (num/spectral_operators.py:399-502). */
 int aeo_vector_curl_div_work(aeo_plan_t ext, int nfields, long long* n_doubles);
@@ -300,6 +302,8 @@ int aeo_pe_destroy(aeo_pe_t pe);
 
 /* d/dt of the (3 nlev + 1, L+1, L+1) complex128 stack
  * [zeta_1..K, delta_1..K, T_1..K, ln p_s] (device; out must not alias y).
+ * The model's scratch and its two plans' workspaces are shared by its calls:
+ * calls on one model must not overlap on different streams.
  * Replaces PrimitiveEquationsModel.tendency (phy/primitive_equations.py:482-753). */
 int aeo_pe_tendency(aeo_pe_t pe, const double* y, double* out, void* stream);
 
