@@ -4,7 +4,9 @@ against the CPU oracle, plus the exact properties the reference tests.
 Bars: vector analysis within 1e-12 of the oracle's scale (and of the closed
 forms curl = lap psi, div = lap chi on the Gauss 3/2-rule grid); the PE
 tendency within 1e-11 of each row block's scale (zeta, delta, T, ln p_s) on
-the Gauss (L = 15, 32 x 64) and geodesic (L = 10, res 3) product samplings;
+the Gauss (L = 15, 32 x 64; L = 42, 64 x 128 with 10 non-uniform levels) and
+geodesic (L = 10, res 3) product samplings; at the measured Gauss L = 85
+configuration, exact rest and BVE degeneracy;
 exact rest bitwise zero; BVE degeneracy within 1e-12 of the barotropic
 tendency's scale; zero flow with structured T gives -lap(Phi) within 1e-12;
 state-grid wind, temperature extrema and characteristic speed within 1e-12 of
@@ -461,3 +463,62 @@ def test_pe_create_rejects_mismatched_plans():
     assert rc == nat.AEO_ERR_INVALID
     with pytest.raises(ValueError):
         so.vector_curl_div_spectral(z[:-1], z[:-1])
+
+
+def test_pe_tendency_larger_grid_matches_oracle():
+    """Gauss 64 x 128 (3/2-rule product 64 x 128), L = 42, 10 non-uniform
+    levels against the oracle: each row block within 1e-11 of its scale."""
+    import torch
+    from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
+    from aeolus_amd.planet import Planet
+    from oracle import pe
+    from oracle.models import Planet as OPlanet, planet_constants
+    print("case pe tendency latlon L=42 K=10", flush=True)
+    R, om = planet_constants(24.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=64, nlon=128, l_max=42,
+                                product_quadrature="fine")
+    op = OPlanet(l_max=42, radius=R, omega=om, grid_type="latlon", nlat=64, nlon=128)
+    s = np.concatenate([[0.0], np.cumsum(np.linspace(1.0, 2.0, 10))])
+    interfaces = tuple(s / s[-1])
+    interfaces = interfaces[:-1] + (1.0,)
+    m = PrimitiveEquationsModel(pl, SigmaGrid(interfaces))
+    om_ = pe.PEModel(op, pe.Sigma(interfaces))
+    y = _band_limited(42, 10, 23)
+    got = _np(m.tendency(torch.as_tensor(y, device="cuda")))
+    ref = pe.pe_tendency(om_, y)
+    for b, sl in (("zeta", slice(0, 10)), ("delta", slice(10, 20)), ("T", slice(20, 30)),
+                  ("lnps", slice(30, 31))):
+        e = np.abs(got[sl] - ref[sl]).max() / np.abs(ref[sl]).max()
+        print(f"  {b}: {e:.2e}", flush=True)
+        assert e < 1e-11, b
+
+
+def test_pe_properties_at_benchmark_size():
+    """Gauss 128 x 256, L = 85, 8 levels (the measured PE configuration):
+    exact rest is bitwise zero and the BVE-degenerate state reproduces the
+    fused barotropic kernels' tendency within 1e-12 of its scale."""
+    import torch
+    from aeolus_amd.physics import (BarotropicState, BarotropicVorticity,
+                                    PrimitiveEquationsModel, SigmaGrid, isothermal_rest_state)
+    from aeolus_amd.planet import Planet
+    from oracle.models import planet_constants
+    print("case pe properties latlon L=85 K=8", flush=True)
+    R, om = planet_constants(24.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=128, nlon=256, l_max=85,
+                                product_quadrature="fine")
+    m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(8))
+    st = isothermal_rest_state(85, 8, temperature=T0, surface_pressure=PS0, device="cuda")
+    assert float(m.tendency(st.coeffs).abs().max()) == 0.0
+    z = torch.as_tensor(_potentials(85, 31, 85) * 1e-11, device="cuda")
+    y = st.coeffs.clone()
+    y[:8] += z
+    out = m.tendency(y)
+    bve = BarotropicVorticity(pl, viscosity=0.0).tendency(BarotropicState(z))
+    scale = float(bve.abs().max())
+    err = max(float((out[k] - bve).abs().max()) for k in range(8)) / scale
+    print(f"  bve degeneracy {err:.2e}", flush=True)
+    assert err <= 1e-12
