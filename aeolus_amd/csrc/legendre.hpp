@@ -1219,7 +1219,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 }
                 continue;
             }
-            // round 1: blocks 2, 3 -> blocks 0, 1
+            // tree step 1: blocks 2, 3 -> blocks 0, 1
             if (kb >= 2) {
 #pragma unroll
                 for (int bb = 0; bb < MB; ++bb)
@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                         accO[bb][f] = cadd(accO[bb][f], slot(kb, 1, bb, f));
                     }
             }
-            // round 2: block 1 -> block 0 (block 1 reuses its own slot)
+            // tree step 2: block 1 -> block 0 (block 1 reuses its own slot)
             if (kb == 1) {
 #pragma unroll
                 for (int bb = 0; bb < MB; ++bb)

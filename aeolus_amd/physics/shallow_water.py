@@ -257,7 +257,7 @@ class ShallowWaterModel:
     def validate_state(self, state: ShallowWaterState, *, context: str = "") -> None:
         """Finiteness, per-prognostic monopoles and strictly positive thickness
         over every sampling (``:428-454``).  Every check is reduced on the
-        device; the verdict is one host read."""
+        device; the outcome is one host read."""
         where = f" {context}" if context else ""
         coeffs = self._stack(state.coeffs)
         finite = torch.isfinite(coeffs).all()
