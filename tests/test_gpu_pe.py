@@ -137,7 +137,7 @@ def test_vector_curl_div_zero_field_is_bitwise_zero():
 
 
 @pytest.mark.parametrize("kind,sigma", [("latlon", 5), ("geodesic", 5), ("latlon", "nonuniform"),
-                                        ("latlon", 1)])
+                                        ("latlon", 1), ("latlon", 32), ("geodesic", 32)])
 def test_pe_tendency_matches_oracle(kind, sigma):
     import torch
     from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
