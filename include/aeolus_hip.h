@@ -76,7 +76,9 @@ const char* aeo_last_error(void);
  *             (Gauss weight * 2 pi / nlon), symmetric
  *   coslat_rows host, nlat: cos(latitude) of each row (the kernels clamp it at
  *             1e-8 for products and mask |cos| < 1e-6 for velocities)
- *   max_fields: largest number of fields one execute call may pass (>= 1). */
+ *   max_fields: fields per launch sequence (>= 1); the workspaces are sized
+ *             for it, and execute calls with more fields run in chunks of
+ *             max_fields. */
 int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
                           const double* w_rows, const double* coslat_rows,
                           int max_fields, aeo_plan_t* out);
@@ -85,7 +87,8 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
  * Replaces PointSetSphericalHarmonics.__init__ (num/fast_geodesic_sh.py:15-66)
  * and, with Voronoi weights, GeodesicSphericalHarmonics (num/optimized_geodesic_sh.py:42-133).
  *   lat, lon, weights, coslat: host, npoints each (weights NULL = uniform 4 pi / N,
- *   num/fast_geodesic_sh.py:35-37; coslat = cos(lat), clamped in the kernels). */
+ *   num/fast_geodesic_sh.py:35-37; coslat = cos(lat), clamped in the kernels);
+ *   max_fields as for aeo_plan_create_gauss. */
 int aeo_plan_create_points(int npoints, int l_max, const double* lat, const double* lon,
                            const double* weights, const double* coslat, int max_fields,
                            aeo_plan_t* out);
