@@ -382,6 +382,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     extern __shared__ double2 lds[];
     int jp, b;
     grid_wg_index(g.nh, nmem, jp, b);
+    // two workgroups per (row pair, member): half 0 forms the products
+    // 0, 1 (eta flux divergence and curl) from 9 grid rows, half 1 the
+    // products 2, 3 (kinetic energy, phi flux divergence) from 8; both
+    // transform the wind rows 0-3 themselves
+    const int half = blockIdx.y;
     const int n = g.nlon, L = g.L, rs = lds_row_stride(n);
     const int jN = jp, jS = g.nlat - 1 - jp;
     const double invR = 1.0 / R;
@@ -392,18 +397,24 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
                               kSpecScale, kSpecPlain, kSpecDlam, kSpecScale, kSpecPlain,
                               kSpecPlain, kSpecPlain};
+    // grid rows of each half, in LDS row order
+    constexpr int rows0[9] = {0, 1, 2, 3, 4, 5, 6, 10, 11};
+    constexpr int rows1[9] = {0, 1, 2, 3, 7, 8, 9, 11, 11};
+    const int nr = half == 0 ? 9 : 8;
     // LDS: rows [12][rs] | twiddles [n] | E, O columns [10][L+1] each
     double2* sE = lds + 12 * rs + n;
     double2* sO = sE + (L + 1) * 10;
     const double2* twl = lds + 12 * rs;
     grid_prologue<10>(g, E, O, eo, ms, lds + 12 * rs, sE, sO);
     __syncthreads();
-    for (int r = 0; r < 12; ++r) {
-        pack_spectrum(lds + r * rs, n, L, sE + src[r] * (L + 1), sO + src[r] * (L + 1), 1, mode[r], invR,
-                      threadIdx.x, kGridThreads);
+    for (int i = 0; i < nr; ++i) {
+        // LDS row i <- grid row rows0[i] (half 0) or rows1[i] (half 1)
+        const int r = half == 0 ? (i < 7 ? i : i + 3) : (i < 4 ? i : (i < 7 ? i + 3 : 11));
+        pack_spectrum(lds + i * rs, n, L, sE + src[r] * (L + 1), sO + src[r] * (L + 1), 1, mode[r],
+                      invR, threadIdx.x, kGridThreads);
     }
     __syncthreads();
-    fft_rows<+1, P2>(lds, 12, g.fft, twl);
+    fft_rows<+1, P2>(lds, nr, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     const double icN = 1.0 / cN, icS = 1.0 / cS;
@@ -411,23 +422,37 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
         const int e = lp(k);
         double gn[12], gs[12], on[4], os[4];
 #pragma unroll
-        for (int r = 0; r < 12; ++r) {
-            const double2 z = lds[r * rs + e];
-            gn[r] = z.x;
-            gs[r] = z.y;
+        for (int r = 0; r < 12; ++r) gn[r] = gs[r] = 0.0;
+        // constant row indices in each branch keep gn / gs in registers
+        if (half == 0) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const double2 z = lds[i * rs + e];
+                gn[rows0[i]] = z.x;
+                gs[rows0[i]] = z.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const double2 z = lds[i * rs + e];
+                gn[rows1[i]] = z.x;
+                gs[rows1[i]] = z.y;
+            }
         }
         swe_point_ic(gn, icN, on);
         swe_point_ic(gs, icS, os);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) lds[r * rs + e] = c2(on[r], os[r]);
+        const double2 p0 = half == 0 ? c2(on[0], os[0]) : c2(on[2], os[2]);
+        const double2 p1 = half == 0 ? c2(on[1], os[1]) : c2(on[3], os[3]);
+        lds[e] = p0;
+        lds[rs + e] = p1;
     }
     __syncthreads();
-    fft_rows<-1, P2>(lds, 4, g.fft, twl);
+    fft_rows<-1, P2>(lds, 2, g.fft, twl);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
-    for (int r = 0; r < 4; ++r)
-        unpack_sd(lds + r * rs, n, mmax, w, S + so + r, D + so + r, (size_t)nmem * 4,
-                  threadIdx.x, kGridThreads);
+    for (int r = 0; r < 2; ++r)
+        unpack_sd(lds + r * rs, n, mmax, w, S + so + 2 * half + r, D + so + 2 * half + r,
+                  (size_t)nmem * 4, threadIdx.x, kGridThreads);
 }
 
 // Adjoint of the BVE Jacobian on one north/south row pair.  E/O columns
@@ -754,7 +779,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
         int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);
         if (rc0) return rc0;
         pr.arm(kClsGrid);
-        launch_k(grid_swe_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), swe_lds, s,
+        launch_k(grid_swe_kernel_for(p), dim3(p->nh * md->nmem, 2), dim3(kGridThreads), swe_lds, s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     }
