@@ -483,33 +483,40 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     const double2* twl = lds + 5 * rs;
     grid_prologue<5>(g, E, O, eo, ms, lds + 5 * rs, sE, sO);
     __syncthreads();
-    for (int r = 0; r < 5; ++r) {
+    // two workgroups per (row pair, member): half 0 forms cotangents 0, 1
+    // from rows b_l, b_s, jt; half 1 cotangents 2, 3 from a_l, a_s, jt
+    const int half = blockIdx.y;
+    for (int i = 0; i < 3; ++i) {
+        const int r = half == 0 ? i + 2 : (i < 2 ? i : 4);
         const int mode = (r == 4) ? kSpecPlain : ((r & 1) ? kSpecScale : kSpecDlam);
-        pack_spectrum(lds + r * rs, n, L, sE + r * (L + 1), sO + r * (L + 1), 1, mode, invR,
+        pack_spectrum(lds + i * rs, n, L, sE + r * (L + 1), sO + r * (L + 1), 1, mode, invR,
                       threadIdx.x, kGridThreads);
     }
     __syncthreads();
-    fft_rows<+1, P2>(lds, 5, g.fft, twl);
+    fft_rows<+1, P2>(lds, 3, g.fft, twl);
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
-        const double2 al = lds[e], as = lds[rs + e], bl = lds[2 * rs + e], bs = lds[3 * rs + e];
-        const double2 jt = lds[4 * rs + e];
+        const double2 xl = lds[e], xs = lds[rs + e];      // (b_l, b_s) or (a_l, a_s)
+        const double2 jt = lds[2 * rs + e];
         const double tN = jt.x * qN, tS = jt.y * qS;
-        lds[e] = c2(-tN * bs.x, -tS * bs.y);
-        lds[rs + e] = c2(tN * bl.x, tS * bl.y);
-        lds[2 * rs + e] = c2(tN * as.x, tS * as.y);
-        lds[3 * rs + e] = c2(-tN * al.x, -tS * al.y);
+        if (half == 0) {
+            lds[e] = c2(-tN * xs.x, -tS * xs.y);
+            lds[rs + e] = c2(tN * xl.x, tS * xl.y);
+        } else {
+            lds[e] = c2(tN * xs.x, tS * xs.y);
+            lds[rs + e] = c2(-tN * xl.x, -tS * xl.y);
+        }
     }
     __syncthreads();
-    fft_rows<-1, P2>(lds, 4, g.fft, twl);
+    fft_rows<-1, P2>(lds, 2, g.fft, twl);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
-    for (int r = 0; r < 4; ++r)
-        unpack_sd(lds + r * rs, n, L, w, S + so + r, D + so + r, (size_t)nmem * 4, threadIdx.x,
-                  kGridThreads);
+    for (int r = 0; r < 2; ++r)
+        unpack_sd(lds + r * rs, n, L, w, S + so + 2 * half + r, D + so + 2 * half + r,
+                  (size_t)nmem * 4, threadIdx.x, kGridThreads);
 }
 
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).
@@ -848,7 +855,7 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     int rc0 = allow_lds((const void*)grid_adj_kernel_for(p), lds);
     if (rc0) return rc0;
     pr.arm(kClsGrid);
-    launch_k(grid_adj_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), lds, s,
+    launch_k(grid_adj_kernel_for(p), dim3(p->nh * md->nmem, 2), dim3(kGridThreads), lds, s,
              p->gdev(), (const double2*)md->d_aE, (const double2*)md->d_aO, md->R, md->nmem,
              md->d_aS, md->d_aD);
     AEO_LAUNCH_CHECK();
