@@ -2,7 +2,8 @@
 
 The numerical classes mirror the reference interface (transform objects,
 backends, ``SpectralOperators``, ``BarotropicVorticity``,
-``ShallowWaterModel``, ``rk4_step`` / ``rk4_step_array``) with PyTorch
+``ShallowWaterModel``, ``PrimitiveEquationsModel`` with ``SigmaGrid``,
+``rk4_step`` / ``rk4_step_array`` and the run drivers) with PyTorch
 device tensors in place of CuPy arrays; every computation runs in the HIP
 library ``aeolus_amd/lib/libaeolus_hip.so`` (see ``include/aeolus_hip.h``).
 """
