@@ -522,3 +522,68 @@ def test_pe_properties_at_benchmark_size():
     err = max(float((out[k] - bve).abs().max()) for k in range(8)) / scale
     print(f"  bve degeneracy {err:.2e}", flush=True)
     assert err <= 1e-12
+
+
+def test_vector_curl_div_closed_forms_at_L85():
+    """Gauss 128 x 256 (3/2-rule product), L = 85, random potentials in every
+    mode up to L - 1: curl / div of k x grad(psi) + grad(chi) recover
+    lap(psi), lap(chi) within 5e-12 (the formulation's own round-off at this
+    size: the oracle's closed-form errors for these potentials are 6.6e-13 /
+    1.8e-12) and equal the oracle's within 1e-12."""
+    import torch
+    from aeolus_amd.planet import Planet
+    from oracle.models import planet_constants
+    R, om = planet_constants(24.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=128, nlon=256, l_max=85,
+                                product_quadrature="fine")
+    so = pl.so
+    sh_p = so._product_space.sh
+    psi = torch.as_tensor(_potentials(85, 41, 84), device="cuda")
+    chi = torch.as_tensor(_potentials(85, 42, 84), device="cuda")
+    c = so._product_space.coslat
+
+    def derivs(a):
+        return (sh_p.inv_transform(so.d_lambda_coeffs(a)),
+                sh_p.inv_transform(so.sin_theta_d_theta_coeffs(a)) / so.R)
+    p_l, p_s = derivs(psi)
+    c_l, c_s = derivs(chi)
+    fe, fn = (p_s + c_l) / c, (p_l - c_s) / c
+    curl, div = so.vector_curl_div_spectral(fe, fn, truncate=False)
+    lap = so.lap_eigs[:, None]
+    scale = max(float((lap * psi).abs().max()), float((lap * chi).abs().max()))
+    ec = float((curl - lap * psi).abs().max()) / scale
+    ed = float((div - lap * chi).abs().max()) / scale
+    print(f"case vector curl/div L=85 closed forms: curl {ec:.2e} div {ed:.2e}", flush=True)
+    assert ec < 5e-12 and ed < 5e-12
+    from oracle import pe
+    from oracle.models import Planet as OPlanet
+    op = OPlanet(l_max=85, radius=R, omega=om, grid_type="latlon", nlat=128, nlon=256)
+    rc, rd = pe.vector_curl_div(op, pe.extended_transform(op), _np(fe), _np(fn), truncate=False)
+    assert np.abs(_np(curl) - rc).max() < 1e-12 * scale
+    assert np.abs(_np(div) - rd).max() < 1e-12 * scale
+
+
+def test_vector_curl_div_coarse_quadrature_matches_oracle():
+    """The coarse product quadrature (the state sampling) builds the
+    extended plan on the state grid; against the oracle within 1e-12."""
+    import torch
+    from aeolus_amd.planet import Planet
+    from oracle import pe
+    from oracle.models import Planet as OPlanet, planet_constants
+    R, om = planet_constants(24.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=32, nlon=64, l_max=15,
+                                product_quadrature="coarse")
+    op = OPlanet(l_max=15, radius=R, omega=om, grid_type="latlon", nlat=32, nlon=64,
+                 product_quadrature="coarse")
+    psi, chi = _potentials(15, 3, 14), _potentials(15, 4, 14)
+    fe, fn = _vector(op, psi, chi)
+    rc, rd = pe.vector_curl_div(op, pe.extended_transform(op), fe, fn)
+    gc, gd = pl.so.vector_curl_div_spectral(torch.as_tensor(fe, device="cuda"),
+                                            torch.as_tensor(fn, device="cuda"))
+    scale = max(np.abs(rc).max(), np.abs(rd).max())
+    assert np.abs(_np(gc) - rc).max() < 1e-12 * scale
+    assert np.abs(_np(gd) - rd).max() < 1e-12 * scale
