@@ -362,15 +362,16 @@ def _pe_state(L, K, seed):
 def pe_modes(steps, which="all"):
     """Dry primitive-equation tendency (8 uniform sigma levels) and one RK4
     step through rk4_step_array: on the reference's default PE configuration
-    (geodesic res 3 state / res 4 product, L = 10) and on a Gauss 128 x 256
-    grid at L = 85 (3/2-rule product)."""
+    (geodesic res 3 state / res 4 product, L = 10) and on Gauss 128 x 256 /
+    L = 85 and 256 x 512 / L = 170 grids (3/2-rule products)."""
     import torch
     from aeolus_amd.physics import PrimitiveEquationsModel, SigmaGrid
     from aeolus_amd.planet import Planet
     from aeolus_amd.run.engine import rk4_step_array
     res = {}
     for name, kw, L in (("geodesic_res3_L10_K8", dict(grid_type="geodesic", grid_resolution=3), 10),
-                        ("gauss_128x256_L85_K8", dict(grid_type="latlon", nlat=128, nlon=256), 85)):
+                        ("gauss_128x256_L85_K8", dict(grid_type="latlon", nlat=128, nlon=256), 85),
+                        ("gauss_256x512_L170_K8", dict(grid_type="latlon", nlat=256, nlon=512), 170)):
         if which != "all" and not name.startswith(which):
             continue
         with warnings.catch_warnings():

@@ -495,25 +495,27 @@ def test_pe_tendency_larger_grid_matches_oracle():
         assert e < 1e-11, b
 
 
-def test_pe_properties_at_benchmark_size():
-    """Gauss 128 x 256, L = 85, 8 levels (the measured PE configuration):
-    exact rest is bitwise zero and the BVE-degenerate state reproduces the
-    fused barotropic kernels' tendency within 1e-12 of its scale."""
+@pytest.mark.parametrize("L,nlat,nlon", [(85, 128, 256), (170, 256, 512)])
+def test_pe_properties_at_benchmark_size(L, nlat, nlon):
+    """Gauss 128 x 256 / L = 85 (the measured PE configuration) and
+    256 x 512 / L = 170, 8 levels: exact rest is bitwise zero and the
+    BVE-degenerate state reproduces the fused barotropic kernels' tendency
+    within 1e-12 of its scale."""
     import torch
     from aeolus_amd.physics import (BarotropicState, BarotropicVorticity,
                                     PrimitiveEquationsModel, SigmaGrid, isothermal_rest_state)
     from aeolus_amd.planet import Planet
     from oracle.models import planet_constants
-    print("case pe properties latlon L=85 K=8", flush=True)
+    print(f"case pe properties latlon L={L} K=8", flush=True)
     R, om = planet_constants(24.0)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=128, nlon=256, l_max=85,
+        pl = Planet.with_radius(R, om, grid_type="latlon", nlat=nlat, nlon=nlon, l_max=L,
                                 product_quadrature="fine")
     m = PrimitiveEquationsModel(pl, SigmaGrid.uniform(8))
-    st = isothermal_rest_state(85, 8, temperature=T0, surface_pressure=PS0, device="cuda")
+    st = isothermal_rest_state(L, 8, temperature=T0, surface_pressure=PS0, device="cuda")
     assert float(m.tendency(st.coeffs).abs().max()) == 0.0
-    z = torch.as_tensor(_potentials(85, 31, 85) * 1e-11, device="cuda")
+    z = torch.as_tensor(_potentials(L, 31, L) * 1e-11, device="cuda")
     y = st.coeffs.clone()
     y[:8] += z
     out = m.tendency(y)
