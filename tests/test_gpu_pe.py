@@ -589,3 +589,19 @@ def test_vector_curl_div_coarse_quadrature_matches_oracle():
     scale = max(np.abs(rc).max(), np.abs(rd).max())
     assert np.abs(_np(gc) - rc).max() < 1e-12 * scale
     assert np.abs(_np(gd) - rd).max() < 1e-12 * scale
+
+
+def test_vector_curl_div_batch_larger_than_plan_chunk():
+    """A 20-field batch (40 analyses, more than the extended plan's 16 fields
+    per launch sequence) equals field-by-field calls."""
+    import torch
+    pl, op = _planets("latlon")
+    N = pl.so.extended_product_transform().n_points
+    rng = np.random.default_rng(77)
+    fe = torch.as_tensor(rng.standard_normal((20, N)), device="cuda")
+    fn = torch.as_tensor(rng.standard_normal((20, N)), device="cuda")
+    bc, bd = pl.so.vector_curl_div_spectral(fe, fn)
+    for f in (0, 7, 15, 16, 19):
+        sc, sd = pl.so.vector_curl_div_spectral(fe[f], fn[f])
+        assert torch.allclose(bc[f], sc, rtol=0, atol=1e-14 * float(sc.abs().max()))
+        assert torch.allclose(bd[f], sd, rtol=0, atol=1e-14 * float(sd.abs().max()))
