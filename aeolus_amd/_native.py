@@ -51,6 +51,7 @@ _SIGNATURES = {
     "aeo_model_destroy": [_vp],
     "aeo_model_tendency": [_vp, _vp, _vp, _vp, _vp],
     "aeo_model_rk4": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp],
+    "aeo_model_enable_adjoint": [_vp],
     "aeo_model_checkpoint_elems": [_vp, _c_int, ctypes.POINTER(ctypes.c_longlong)],
     "aeo_model_rk4_record": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp, _vp],
     "aeo_model_rk4_adjoint": [_vp, _vp, _c_double, _c_int, _vp, _vp, _vp],

@@ -291,6 +291,7 @@ static int enable_split_points(aeo_plan_s* plan, int nslices) {
     AEO_REQUIRE(split_exact(nslices, K2) && split_exact(nslices, kPtsChunk), AEO_ERR_UNSUPPORTED,
                 "aeo_plan_enable_split: INT32 accumulation would not be exact at this size");
     if (plan->nslices == nslices) return AEO_OK;
+    ++plan->tables_gen;
     for (void* q : {(void*)plan->d_Ysl, (void*)plan->d_eYs, (void*)plan->d_Yasl,
                     (void*)plan->d_eYa})
         dev_free(q);
@@ -321,6 +322,7 @@ int aeo_plan_enable_split(aeo_plan_t plan, int nslices) {
                 AEO_ERR_UNSUPPORTED,
                 "aeo_plan_enable_split: INT32 accumulation would not be exact at this size");
     if (plan->nslices == nslices) return AEO_OK;
+    ++plan->tables_gen;
     for (void* q : {(void*)plan->d_Psl, (void*)plan->d_offs_ss, (void*)plan->d_esyn,
                     (void*)plan->d_Pasl, (void*)plan->d_offs_sa, (void*)plan->d_eana,
                     (void*)plan->d_offs_ea})
@@ -719,19 +721,40 @@ static int adjoint_supported(const aeo_model_s* m, const char* who) {
     return AEO_OK;
 }
 
-static int ensure_adjoint(aeo_model_s* m) {
-    if (m->d_aE) return AEO_OK;
-    const size_t plane = (size_t)m->plan->nh * (m->L + 1) * m->nmem;
-    const size_t st = state_elems(m);
-    int rc;
-    if ((rc = dev_alloc(&m->d_aE, plane * 5))) return rc;
-    if ((rc = dev_alloc(&m->d_aO, plane * 5))) return rc;
-    if ((rc = dev_alloc(&m->d_aS, plane * 4))) return rc;
-    if ((rc = dev_alloc(&m->d_aD, plane * 4))) return rc;
-    if ((rc = dev_alloc(&m->d_asum, st))) return rc;
-    if ((rc = dev_alloc(&m->d_abuf, 2 * st))) return rc;
-    if ((rc = dev_alloc(&m->d_kbuf, 2 * st))) return rc;
-    return AEO_OK;
+static bool adjoint_ready(const aeo_model_s* m) {
+    return m->d_aE && m->d_aO && m->d_aS && m->d_aD && m->d_asum && m->d_abuf && m->d_kbuf;
+}
+
+static void free_adjoint(aeo_model_s* m) {
+    for (double2** q : {&m->d_aE, &m->d_aO, &m->d_aS, &m->d_aD, &m->d_asum, &m->d_abuf,
+                        &m->d_kbuf}) {
+        dev_free(*q);
+        *q = nullptr;
+    }
+}
+
+// The adjoint execute calls allocate nothing: they need the workspaces this
+// set-up call allocates (all or none).
+static int require_adjoint(const aeo_model_s* m, const char* who) {
+    if (adjoint_ready(m)) return AEO_OK;
+    set_error(std::string(who) + ": call aeo_model_enable_adjoint on the model first");
+    return AEO_ERR_STATE;
+}
+
+int aeo_model_enable_adjoint(aeo_model_t model) {
+    AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_enable_adjoint: model is NULL");
+    int rc = adjoint_supported(model, "aeo_model_enable_adjoint");
+    if (rc) return rc;
+    if (adjoint_ready(model)) return AEO_OK;
+    free_adjoint(model);
+    const size_t plane = (size_t)model->plan->nh * (model->L + 1) * model->nmem;
+    const size_t st = state_elems(model);
+    if (!(rc = dev_alloc(&model->d_aE, plane * 5)) && !(rc = dev_alloc(&model->d_aO, plane * 5)) &&
+        !(rc = dev_alloc(&model->d_aS, plane * 4)) && !(rc = dev_alloc(&model->d_aD, plane * 4)) &&
+        !(rc = dev_alloc(&model->d_asum, st)) && !(rc = dev_alloc(&model->d_abuf, 2 * st)))
+        rc = dev_alloc(&model->d_kbuf, 2 * st);
+    if (rc) free_adjoint(model);
+    return rc;
 }
 
 int aeo_model_checkpoint_elems(aeo_model_t model, int nsteps, long long* out) {
@@ -790,6 +813,7 @@ int aeo_model_rk4_adjoint(aeo_model_t model, const double* checkpoints, double d
     AEO_REQUIRE(nsteps == 0 || checkpoints, AEO_ERR_INVALID,
                 "aeo_model_rk4_adjoint: checkpoints is NULL");
     int rc = adjoint_supported(model, "aeo_model_rk4_adjoint");
+    if (!rc) rc = require_adjoint(model, "aeo_model_rk4_adjoint");
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const size_t st = state_elems(model);
@@ -799,7 +823,6 @@ int aeo_model_rk4_adjoint(aeo_model_t model, const double* checkpoints, double d
                                          hipMemcpyDeviceToDevice, s));
         return AEO_OK;
     }
-    if ((rc = ensure_adjoint(model))) return rc;
     const double2* ck = reinterpret_cast<const double2*>(checkpoints);
     auto slot = [&](int step, int k) { return ck + ((size_t)step * 4 + k) * st; };
     double2* abuf[2] = {model->d_abuf, model->d_abuf + st};
@@ -829,8 +852,8 @@ int aeo_model_tendency_adjoint(aeo_model_t model, const double* y, const double*
     AEO_REQUIRE(model && y && kbar && ybar, AEO_ERR_INVALID,
                 "aeo_model_tendency_adjoint: NULL argument");
     int rc = adjoint_supported(model, "aeo_model_tendency_adjoint");
+    if (!rc) rc = require_adjoint(model, "aeo_model_tendency_adjoint");
     if (rc) return rc;
-    if ((rc = ensure_adjoint(model))) return rc;
     hipStream_t s = as_stream(stream);
     const size_t st = state_elems(model);
     double2* yi = model->d_abuf;

@@ -17,7 +17,8 @@ struct aeo_ensemble_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double dt = NAN;
-    std::vector<int> cfg;              // (mb, nslices) of every model at capture time
+    // (mb, nslices, plan table generation) of every model at capture time
+    std::vector<long long> cfg;
 };
 
 namespace aeo {
@@ -37,6 +38,19 @@ static int group_step(aeo_model_s* m, const double2* cur, double2* out, double d
         if (rc) return rc;
     }
     return AEO_OK;
+}
+
+// What a captured graph depends on beyond dt: the tiles and precision of
+// every model and the generation of its plan's split tables (rebuilding them
+// moves them, and the graph holds their addresses).
+static std::vector<long long> graph_key(const aeo_ensemble_s* e) {
+    std::vector<long long> k;
+    for (const aeo_model_s* m : e->models) {
+        k.push_back(m->mb);
+        k.push_back(m->nslices);
+        k.push_back(m->plan->tables_gen);
+    }
+    return k;
 }
 
 static void release_graph(aeo_ensemble_s* e) {
@@ -86,11 +100,7 @@ static int capture(aeo_ensemble_s* e, double dt) {
     e->graph = graph;
     AEO_CHECK_HIP(hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0));
     e->dt = dt;
-    e->cfg.clear();
-    for (aeo_model_s* m : e->models) {
-        e->cfg.push_back(m->mb);
-        e->cfg.push_back(m->nslices);
-    }
+    e->cfg = graph_key(e);
     return AEO_OK;
 }
 
@@ -163,12 +173,7 @@ int aeo_ensemble_rk4(aeo_ensemble_t ens, const double* const* y, double dt, int 
     const int pairs = nsteps / 2;
     if (pairs > 0) {
         if (use_graph) {
-            std::vector<int> cfg;
-            for (aeo_model_s* m : ens->models) {
-                cfg.push_back(m->mb);
-                cfg.push_back(m->nslices);
-            }
-            if (!ens->exec || ens->dt != dt || cfg != ens->cfg) {
+            if (!ens->exec || ens->dt != dt || graph_key(ens) != ens->cfg) {
                 // capture only records the launches; they execute in order on `s`
                 int rc = capture(ens, dt);
                 if (rc) return rc;

@@ -372,6 +372,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     unpack_sd(lds, n, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
 }
 
+// LDS rows of the split grid kernels: the larger SWE half transforms 9
+// grid rows, each adjoint half 3.
+constexpr int kSweLdsRows = 9;
+constexpr int kAdjLdsRows = 3;
+
 // Shallow water: 10 Legendre fields -> 12 grid rows -> 4 products -> S/D.
 // grid (nh, members); E/O columns (member, 10 fields), S/D columns (member, 4).
 template <bool P2>
@@ -401,11 +406,12 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     constexpr int rows0[9] = {0, 1, 2, 3, 4, 5, 6, 10, 11};
     constexpr int rows1[9] = {0, 1, 2, 3, 7, 8, 9, 11, 11};
     const int nr = half == 0 ? 9 : 8;
-    // LDS: rows [12][rs] | twiddles [n] | E, O columns [10][L+1] each
-    double2* sE = lds + 12 * rs + n;
+    // LDS: rows [9][rs] (the larger half's grid rows) | twiddles [n] | E, O
+    // columns [10][L+1] each
+    double2* sE = lds + kSweLdsRows * rs + n;
     double2* sO = sE + (L + 1) * 10;
-    const double2* twl = lds + 12 * rs;
-    grid_prologue<10>(g, E, O, eo, ms, lds + 12 * rs, sE, sO);
+    const double2* twl = lds + kSweLdsRows * rs;
+    grid_prologue<10>(g, E, O, eo, ms, lds + kSweLdsRows * rs, sE, sO);
     __syncthreads();
     for (int i = 0; i < nr; ++i) {
         // LDS row i <- grid row rows0[i] (half 0) or rows1[i] (half 1)
@@ -477,11 +483,12 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
     const size_t ms = (size_t)nmem * 5;
-    // LDS: rows [5][rs] | twiddles [n] | E, O columns [5][L+1] each
-    double2* sE = lds + 5 * rs + n;
+    // LDS: rows [3][rs] (each half transforms 3 grid rows) | twiddles [n] |
+    // E, O columns [5][L+1] each
+    double2* sE = lds + kAdjLdsRows * rs + n;
     double2* sO = sE + (L + 1) * 5;
-    const double2* twl = lds + 5 * rs;
-    grid_prologue<5>(g, E, O, eo, ms, lds + 5 * rs, sE, sO);
+    const double2* twl = lds + kAdjLdsRows * rs;
+    grid_prologue<5>(g, E, O, eo, ms, lds + kAdjLdsRows * rs, sE, sO);
     __syncthreads();
     // two workgroups per (row pair, member): half 0 forms cotangents 0, 1
     // from rows b_l, b_s, jt; half 1 cotangents 2, 3 from a_l, a_s, jt
@@ -782,7 +789,7 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     } else {
-        const size_t swe_lds = grid_lds(p, 12, 10);
+        const size_t swe_lds = grid_lds(p, kSweLdsRows, 10);
         int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);
         if (rc0) return rc0;
         pr.arm(kClsGrid);
@@ -851,7 +858,7 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     const size_t fs = (size_t)(L + 1) * (L + 1);
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
-    const size_t lds = grid_lds(p, 5, 5);
+    const size_t lds = grid_lds(p, kAdjLdsRows, 5);
     int rc0 = allow_lds((const void*)grid_adj_kernel_for(p), lds);
     if (rc0) return rc0;
     pr.arm(kClsGrid);

@@ -66,6 +66,9 @@ struct aeo_plan_s {
     double2* d_tw = nullptr;
     aeo::FftDev fft{};
     int nslices = 0;
+    // bumped whenever the split tables are rebuilt (their device addresses
+    // change): captured graphs that read them compare it before a replay
+    long long tables_gen = 0;
     signed char* d_Psl = nullptr;
     long long* d_offs_ss = nullptr;
     int* d_esyn = nullptr;

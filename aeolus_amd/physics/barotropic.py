@@ -121,6 +121,13 @@ class BarotropicVorticity:
         return y
 
     # -- adjoint (no reference counterpart) ----------------------------------
+    def enable_adjoint(self) -> None:
+        """Allocate the adjoint workspaces (once); the adjoint calls need them
+        and allocate nothing themselves."""
+        if not getattr(self, "_adjoint_ready", False):
+            nat.check(nat.lib().aeo_model_enable_adjoint(self._model), "enable_adjoint")
+            self._adjoint_ready = True
+
     def checkpoint_elems(self, nsteps: int) -> int:
         """complex128 entries of the stage checkpoints of an ``nsteps`` run."""
         n = ctypes.c_longlong()
@@ -132,6 +139,7 @@ class BarotropicVorticity:
                    out: torch.Tensor | None = None):
         """:meth:`rk4` (bit-identical result) that also keeps the input state of
         every RK4 stage; returns ``(y, checkpoints)`` for :meth:`rk4_adjoint`."""
+        self.enable_adjoint()
         z = self._coeffs(coeffs)
         y = torch.empty_like(z) if out is None else out
         ck = torch.empty(max(self.checkpoint_elems(nsteps), 1), dtype=torch.complex128,
@@ -150,6 +158,7 @@ class BarotropicVorticity:
         arrays, the torch.autograd convention for complex tensors)."""
         if checkpoints.numel() < self.checkpoint_elems(nsteps):
             raise ValueError("rk4_adjoint: checkpoint buffer is smaller than the recorded run")
+        self.enable_adjoint()
         g = self._coeffs(ybar_out)
         res = torch.empty_like(g) if out is None else out
         nat.check(nat.lib().aeo_model_rk4_adjoint(self._model, checkpoints.data_ptr(), float(dt),
@@ -159,6 +168,7 @@ class BarotropicVorticity:
 
     def tendency_adjoint(self, coeffs, kbar) -> torch.Tensor:
         """``T'(coeffs)^T kbar`` for one tendency evaluation."""
+        self.enable_adjoint()
         z = self._coeffs(coeffs)
         k = self._coeffs(kbar)
         out = torch.empty_like(z)

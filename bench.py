@@ -412,11 +412,7 @@ def cpu_baseline(seconds: float):
     """Oracle restatement (NumPy FFT + stable Legendre) of the same L = 170 BVE step."""
     from oracle.models import Planet as OPlanet, bve_tendency, rk4_step
     from aeolus_amd.run.initial_conditions import spectrum_l15
-    try:
-        from threadpoolctl import threadpool_info
-        blas = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
-    except Exception:
-        blas = 1
+    blas = _blas_threads()
     op = OPlanet(l_max=170, radius=R_EARTH, omega=OMEGA, grid_type="latlon", nlat=256,
                  nlon=512)
     y = spectrum_l15(170, np.random.default_rng(2026), rms=5e-5)
@@ -438,6 +434,96 @@ def cpu_baseline(seconds: float):
                       f"FFT + stable-Legendre restatement, FP64), after 1 warm-up step"}
 
 
+def _blas_threads() -> int:
+    try:
+        from threadpoolctl import threadpool_info
+        return int(max((i.get("num_threads", 1) for i in threadpool_info()), default=1))
+    except Exception:
+        return 1
+
+
+def reference_algorithm_cpu(c3_steps: int = 2):
+    """The reference algorithm itself on the host cores: dense basis Y built
+    with the reference kernel's formula (cu/sh_matrix.cu:7-76), analysis
+    Y^H (w f) and synthesis 2 Re(Y a) - Re(Y a_0) per field
+    (num/fast_geodesic_sh.py:68-107), called the way the tendencies call them
+    (phy/barotropic.py:120-167, phy/shallow_water.py:347-418), RK4 of
+    run/bve/runner.py:211-216 / run/engine.py:406-431 -- the oracle's NumPy
+    restatement with basis="reference", on the same seeded inputs as the
+    device runs:
+      C1  BVE Gauss 12x24 / 13x25 product, L = 8, 144 x 600 s (all steps);
+      C2  BVE geodesic res 4 / res 5 product, L = 21, 96 x 900 s (all steps);
+      C3  SWE Gauss 128x256, L = 85, 2 km mountain, c3_steps x 300 s (dense Y
+          is 1.96 GB per grid; the mountain's analysis and synthesis run first
+          and warm both grids' Y).
+    C1 and C2 are timed after one warm-up step."""
+    from aeolus_amd.run.initial_conditions import gaussian_vortex_pair, spectrum_l15, swe_spectra
+    from oracle.models import (Planet as OPlanet, SWEParams, bve_tendency, mountain, rk4_step,
+                               rk4_step_array, swe_tendency)
+    res = {"algorithm": "dense Y (cu/sh_matrix.cu formula), zgemv-shaped NumPy transforms",
+           "blas_threads": _blas_threads(), "host_cores_available": len(os.sched_getaffinity(0))}
+
+    def run_bve(name, p, z, dt, n):
+        tend = lambda y: bve_tendency(p, y)
+        z = rk4_step(tend, z, dt)                                   # warm-up step
+        t0 = time.perf_counter()
+        for _ in range(n):
+            z = rk4_step(tend, z, dt)
+        el = time.perf_counter() - t0
+        res[name] = {"steps": n, "seconds": el, "steps_per_s": n / el}
+
+    p = OPlanet(l_max=8, radius=R_EARTH, omega=OMEGA, grid_type="latlon", nlat=12, nlon=24,
+                dense=True, basis="reference")
+    z = p.sh.transform(gaussian_vortex_pair(p.grid.point_latitudes, p.grid.point_longitudes,
+                                            R_EARTH, seed=1))
+    run_bve("c1_bve_gauss12x24_L8", p, z, 600.0, 144)
+    p = OPlanet(l_max=21, radius=R_EARTH, omega=OMEGA, grid_type="geodesic", resolution=4,
+                basis="reference")
+    run_bve("c2_bve_geodesic_res4_L21", p, spectrum_l15(21, np.random.default_rng(21), rms=5e-5),
+            900.0, 96)
+    t0 = time.perf_counter()
+    p = OPlanet(l_max=85, radius=R_EARTH, omega=OMEGA, grid_type="latlon", nlat=128, nlon=256,
+                dense=True, basis="reference")
+    build = time.perf_counter() - t0
+    rng = np.random.default_rng(85)
+    lat0 = math.degrees(math.asin(rng.uniform(-1, 1)))
+    lon0 = math.degrees(rng.uniform(0, 2 * math.pi))
+    elev = mountain(p, height_m=2000.0, lat_deg=lat0, lon_deg=lon0,
+                    width_deg=math.degrees(1e6 / R_EARTH))
+    prm = SWEParams(p, gravity=9.80616, mean_depth=1e4, elevation_lm=elev)
+    y = swe_spectra(85, R_EARTH, 9.80616, seed=85)
+    t0 = time.perf_counter()
+    for _ in range(c3_steps):
+        y = rk4_step_array(lambda s: swe_tendency(prm, s), y, DT)
+    el = time.perf_counter() - t0
+    res["c3_swe_gauss128x256_L85_mountain"] = {"steps": c3_steps, "seconds": el,
+                                               "steps_per_s": c3_steps / el,
+                                               "basis_build_s": build}
+    return res
+
+
+def c1_device(steps: int):
+    """C1 on the device: BVE Gauss 12x24 / 13x25 product, L = 8, FP64, the
+    two-vortex start of the reference algorithm's CPU run; ms per RK4 step."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.planet import Planet
+    from aeolus_amd.run.initial_conditions import gaussian_vortex_pair
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pl = Planet.with_radius(R_EARTH, OMEGA, grid_type="latlon", nlat=12, nlon=24, l_max=8,
+                                product_quadrature="fine")
+    model = BarotropicVorticity(pl)
+    lat = pl.grid.point_latitudes
+    lon = pl.grid.point_longitudes
+    lat = lat.cpu().numpy() if hasattr(lat, "cpu") else np.asarray(lat)
+    lon = lon.cpu().numpy() if hasattr(lon, "cpu") else np.asarray(lon)
+    z0 = pl.sh.transform(torch.as_tensor(gaussian_vortex_pair(lat, lon, R_EARTH, seed=1),
+                                         device="cuda"))
+    out = torch.empty_like(z0)
+    return _timed(lambda k: model.rk4(z0, 600.0, k, out=out), steps, 16)
+
+
 def _pe_oracle_ms(seconds: float) -> float:
     """Oracle primitive-equation tendency (NumPy, dense stable-recurrence basis)
     on the reference's default PE case: geodesic res 3 (res 4 product), L = 10,
@@ -455,20 +541,42 @@ def _pe_oracle_ms(seconds: float) -> float:
     return (time.perf_counter() - t0) * 1000.0 / n
 
 
+def kernel_sources_hash() -> str:
+    """Short SHA-256 of every HIP source of the library (csrc/*.hip, *.hpp):
+    a traffic profile records it, and a profile taken before the kernels
+    changed is not used."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for path in sorted(glob.glob(os.path.join(ROOT, "aeolus_amd", "csrc", "*.*p*"))):
+        if path.endswith((".hip", ".hpp")):
+            h.update(os.path.basename(path).encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _measured_traffic(kernel: str, precision: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of the same workload (profiles/*traffic*.md, counters
-    scaled by a plain copy of known size), or None if none matches."""
+    """(bytes, status): HBM bytes per launch of `kernel` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload
+    (profiles/*traffic*.md, counters scaled by a plain copy of known size),
+    provided the summary was taken from the current kernel sources; else
+    (None, why)."""
     import glob
     import re
+    want = kernel_sources_hash()
+    stale = False
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
         with open(path) as f:
             for line in f:
-                m = re.match(r"traffic_bytes_per_launch: (\d+) kernel: (.+?) precision: (\S+)",
-                             line.strip())
+                m = re.match(r"traffic_bytes_per_launch: (\d+) kernel: (.+?) precision: (\S+)"
+                             r"(?: workload: \S+)?(?: sources: (\S+))?$", line.strip())
                 if m and kernel.startswith(m.group(2)) and m.group(3) == precision:
-                    return int(m.group(1))
-    return None
+                    if m.group(4) == want:
+                        return int(m.group(1)), "current"
+                    stale = True
+    return None, ("stale: the traffic profile predates the current kernel sources" if stale
+                  else "no traffic profile for this kernel and precision")
 
 
 def _mode_name(ns):
@@ -496,6 +604,7 @@ def main():
     nbytes = _stage_bytes(dom, 170, 256, best)
     achieved = nbytes / (h["dom_ms"] * 1e-3) / 1e9
     flops = _stage_flops(dom, 170, 256)
+    traffic = _measured_traffic(KCLASS[dom], _mode_name(best))
     secondary = {
         "bve170_modes": {_mode_name(k): v for k, v in h["modes"].items()},
         "bve170_ms_per_step_kernel_timed_pass": h["ms_instr"],
@@ -517,6 +626,7 @@ def main():
         secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
         secondary["geodesic_res4_L21_modes"] = geodesic_modes(args.steps, args.warmup)
         secondary["primitive_equations_K8"] = pe_modes(min(args.steps, 100))
+        secondary["c1_bve_gauss12x24_L8_fp64_ms_per_step"] = c1_device(144)
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
         "value": 1000.0 / h["ms_step"],
@@ -538,11 +648,23 @@ def main():
         "secondary": secondary,
         "roofline": {"bound": "hbm", "kernel": KCLASS[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _measured_traffic(KCLASS[dom], _mode_name(best)),
+                     "traffic": traffic[0], "traffic_profile": traffic[1],
+                     "kernel_sources": kernel_sources_hash(),
                      "bytes_per_launch": nbytes, "launch_ms": h["dom_ms"],
                      "launches": h["dom_launches"]},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
     }
+    if not args.no_cpu and not args.headline_only:
+        ref = reference_algorithm_cpu()
+        device_ms = {
+            "c1_bve_gauss12x24_L8": secondary["c1_bve_gauss12x24_L8_fp64_ms_per_step"],
+            "c2_bve_geodesic_res4_L21": secondary["geodesic_res4_L21_modes"]["fp64"]["ms_per_step"],
+            "c3_swe_gauss128x256_L85_mountain": secondary["swe85_mountain_modes"]["fp64"]["ms_per_step"],
+        }
+        for name, ms in device_ms.items():
+            ref[name]["device_fp64_steps_per_s"] = 1000.0 / ms
+            ref[name]["device_over_cpu"] = (1000.0 / ms) / ref[name]["steps_per_s"]
+        line["cpu_baseline"]["reference_algorithm"] = ref
     print(json.dumps(line))
 
 

@@ -198,6 +198,10 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
  * complex128 arrays, <x, y> = sum Re(conj(x) y) (the convention of
  * torch.autograd for complex tensors).
  *
+ * aeo_model_enable_adjoint: set-up call that allocates the model's adjoint
+ * workspaces (all of them or, on failure, none); aeo_model_rk4_adjoint and
+ * aeo_model_tendency_adjoint allocate nothing and return AEO_ERR_STATE
+ * until it has succeeded.  Calling it again is a no-op.
  * aeo_model_checkpoint_elems: complex128 entries a checkpoint buffer for
  * nsteps steps needs (nsteps * 4 * members * (L+1)^2).
  * aeo_model_rk4_record: aeo_model_rk4 (bit-identical y_out) that also writes
@@ -208,6 +212,7 @@ int aeo_model_rk4(aeo_model_t model, const double* y, double dt, int nsteps,
  * ybar may equal ybar_out.
  * aeo_model_tendency_adjoint: ybar = T'(y)^T kbar for one tendency call
  * (forcing drops out). */
+int aeo_model_enable_adjoint(aeo_model_t model);
 int aeo_model_checkpoint_elems(aeo_model_t model, int nsteps, long long* out);
 int aeo_model_rk4_record(aeo_model_t model, const double* y, double dt, int nsteps,
                          const double* forcing, double* y_out, double* checkpoints,

@@ -238,8 +238,14 @@ def swe_tendency(p: SWEParams, y: np.ndarray) -> np.ndarray:
 
 
 def mountain(planet: Planet, *, height_m: float, lat_deg: float, lon_deg: float,
-             width_deg: float) -> np.ndarray:
-    """Band-limited Gaussian mountain (``phy/topography.py:207-290``)."""
+             width_deg: float, l_cut: int | None = None,
+             return_residual: bool = False):
+    """Band-limited Gaussian mountain (``phy/topography.py:207-290``).
+
+    ``l_cut`` zeroes every coefficient of degree or order above it before the
+    0.2 projection-residual gate (``:263-266``); with ``return_residual`` the
+    relative residual is returned beside the coefficients.
+    """
     lat = planet.grid.point_latitudes
     lon = planet.grid.point_longitudes
     lat0, lon0, sig = map(math.radians, (lat_deg, lon_deg, width_deg))
@@ -247,9 +253,12 @@ def mountain(planet: Planet, *, height_m: float, lat_deg: float, lon_deg: float,
     d = np.arccos(np.clip(cosd, -1.0, 1.0))
     analytic = height_m * np.exp(-(d / sig) ** 2)
     coeffs = planet.sh.transform(analytic)
+    if l_cut is not None:
+        coeffs[int(l_cut) + 1:, :] = 0.0
+        coeffs[:, int(l_cut) + 1:] = 0.0
     synth = planet.sh.inv_transform(coeffs)
     w = planet.sh.weights
     rel = math.sqrt(np.sum(w * (synth - analytic) ** 2)) / math.sqrt(np.sum(w * analytic ** 2))
     if not rel <= 0.2:
         raise ValueError(f"mountain projection residual {rel:.3f} exceeds 0.2")
-    return coeffs
+    return (coeffs, rel) if return_residual else coeffs

@@ -68,6 +68,8 @@ def test_invalid_arguments_are_rejected_on_the_host():
     assert lib.aeo_swe_create(None, 1.0, 0.0, -1.0, 1.0, 0.0, None,
                               ctypes.byref(out)) == nat.AEO_ERR_INVALID
     assert lib.aeo_model_profile(None, 0, 1) == nat.AEO_ERR_INVALID
+    assert lib.aeo_model_enable_adjoint(None) == nat.AEO_ERR_INVALID
+    assert lib.aeo_model_rk4_adjoint(None, 16, 1.0, 1, 16, 16, None) == nat.AEO_ERR_INVALID
 
 
 class _FakeSH:

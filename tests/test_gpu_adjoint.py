@@ -125,7 +125,7 @@ def _tl_step(model, y, d, dt):
             d + (dt / 6.0) * (d1 + 2 * d2 + 2 * d3 + d4))
 
 
-@pytest.mark.parametrize("L,nlat,nlon", [(42, 64, 128), (85, 128, 256)])
+@pytest.mark.parametrize("L,nlat,nlon", [(42, 64, 128), (85, 128, 256), (170, 256, 512)])
 def test_dot_product_tendency_and_step(L, nlat, nlon):
     import torch
     from aeolus_amd.physics import BarotropicState, BarotropicVorticity
@@ -151,7 +151,10 @@ def test_dot_product_tendency_and_step(L, nlat, nlon):
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
 
 
-def _fd_check(model, L, dt, nsteps, ndir=8):
+def _fd_check(model, L, dt, nsteps, ndir=8, fd_precision=None):
+    """Worst relative mismatch between the autograd gradient (at the model's
+    precision) and central differences of the forward map (at
+    ``fd_precision`` when given, else the same precision)."""
     import torch
     from aeolus_amd.run.adjoint import rollout, sphere_misfit
     dev = "cuda"
@@ -161,6 +164,8 @@ def _fd_check(model, L, dt, nsteps, ndir=8):
     J = sphere_misfit(rollout(model, z, dt, nsteps), target)
     J.backward()
     g = z.grad.detach()
+    if fd_precision is not None:
+        model.set_precision(fd_precision)
     eps = 1e-7 * float(z0.abs().norm())
     worst = 0.0
     for k in range(ndir):
@@ -186,17 +191,20 @@ def test_autograd_gradient_matches_finite_differences(nslices):
     assert worst < 1e-6
 
 
-def test_c5_adjoint_gradient_l170_split():
-    """L = 170 on Gauss 256x512, split-precision transforms (7 slices) with the
-    FP64 state, 48 RK4 steps of 300 s, 8 seeded directions.  (With 6 slices the
-    digit rounding of the forward map puts the central-difference quotient
-    itself at ~1e-6 from the derivative for eps = 1e-7.)"""
+@pytest.mark.parametrize("nslices,fd_precision", [(0, None), (7, None), (6, 0)])
+def test_c5_adjoint_gradient_l170(nslices, fd_precision):
+    """L = 170 on Gauss 256x512, 48 RK4 steps of 300 s, 8 seeded directions:
+    the FP64 gradient and the split-precision gradients (FP64 state).  With 6
+    slices the digit rounding of the forward map puts the central-difference
+    quotient of that map itself ~1e-6 from its derivative at eps = 1e-7, so
+    the 6-slice gradient is checked against central differences of the FP64
+    forward map (the model it approximates)."""
     from aeolus_amd.physics import BarotropicVorticity
-    print("case autograd fd L=170 nslices=7 48 steps", flush=True)
+    print(f"case autograd fd L=170 nslices={nslices} fd={fd_precision} 48 steps", flush=True)
     pl, _ = _planets(170, 256, 512, oracle=False)
     model = BarotropicVorticity(pl)
-    model.set_precision(7)
-    worst = _fd_check(model, 170, 300.0, 48)
+    model.set_precision(nslices)
+    worst = _fd_check(model, 170, 300.0, 48, fd_precision=fd_precision)
     print(f"worst fd mismatch {worst:.3e}", flush=True)
     assert worst < 1e-6
 
@@ -216,3 +224,29 @@ def test_batch_adjoint_matches_single_members():
     for k in range(3):
         _, ck1 = single.rk4_record(z[k], dt, ns)
         assert torch.equal(g[k], single.rk4_adjoint(ck1, dt, ns, yb[k]))
+
+
+def test_adjoint_calls_need_the_setup_call():
+    """The adjoint execute calls allocate nothing: before
+    aeo_model_enable_adjoint they return AEO_ERR_STATE; after it they run."""
+    import torch
+    from aeolus_amd import _native as nat
+    from aeolus_amd.physics import BarotropicVorticity
+    print("case adjoint setup state", flush=True)
+    L = 21
+    pl, _ = _planets(L, 32, 64, oracle=False)
+    model = BarotropicVorticity(pl)
+    z = torch.as_tensor(_field(L, 1), device="cuda")
+    kb = torch.as_tensor(_field(L, 2), device="cuda")
+    out = torch.empty_like(z)
+    lib = nat.lib()
+    rc = lib.aeo_model_tendency_adjoint(model._model, z.data_ptr(), kb.data_ptr(),
+                                        out.data_ptr(), nat.stream())
+    assert rc == nat.AEO_ERR_STATE and b"enable_adjoint" in lib.aeo_last_error()
+    assert lib.aeo_model_enable_adjoint(model._model) == nat.AEO_OK
+    assert lib.aeo_model_enable_adjoint(model._model) == nat.AEO_OK      # idempotent
+    rc = lib.aeo_model_tendency_adjoint(model._model, z.data_ptr(), kb.data_ptr(),
+                                        out.data_ptr(), nat.stream())
+    assert rc == nat.AEO_OK
+    torch.cuda.synchronize()
+    assert torch.isfinite(out.real).all()

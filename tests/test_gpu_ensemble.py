@@ -49,3 +49,26 @@ def test_ensemble_graph_matches_eager_and_single(precision, tmp_path, monkeypatc
     # the cached choice is reused
     again = ens.autotune(states, 300.0, steps=2)
     assert all(t.get("cached") for t in again)
+
+
+def test_graph_recaptured_after_shared_plan_tables_rebuilt():
+    """Another model on the same planet switches the shared plan's digit tables
+    6 -> 7 -> 6 slices (rebuilt at new addresses); the ensemble's next graph
+    replay must use the rebuilt tables: graph == eager, bitwise."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print("case ensemble graph after plan table rebuild", flush=True)
+    pl = _planet(42, 64, 128)
+    ens = Ensemble([(pl, 3)], precision=6)
+    y0 = torch.as_tensor(np.stack([spectrum_l15(42, np.random.default_rng(500 + k), rms=5e-5)
+                                   for k in range(3)]), device="cuda")
+    first = ens.rk4([y0], 300.0, 4, use_graph=True)[0]
+    other = BarotropicVorticity(pl)
+    other.set_precision(7)
+    other.set_precision(6)
+    g = ens.rk4([y0], 300.0, 4, use_graph=True)[0]
+    e = ens.rk4([y0], 300.0, 4, use_graph=False)[0]
+    assert torch.equal(g, e)
+    assert torch.equal(g, first)
