@@ -3,9 +3,10 @@
 analysed once with the backend transform, optional ``l_cut``, 0.2 relative
 projection-residual gate).
 
-Bars: coefficients within 1e-13 relative L2 of the oracle, the projection
-residual within 1e-10 relative; an unrepresentable mountain is rejected by
-both.
+Bars: coefficients within 1e-13 relative L2 of the oracle (1e-12 on the
+geodesic grid), the projection residual within 1e-10 relative (plus 1e-13
+absolute, for residuals at round-off); an unrepresentable mountain is
+rejected by both.
 """
 import math
 import os
@@ -69,8 +70,10 @@ def test_mountain_c3_matches_oracle(l_cut):
     assert np.all(np.triu(got, 1) == 0)          # padding m > l written as 0
     phi = topo.surface_geopotential_lm(GRAVITY).cpu().numpy()
     assert rel_l2(_tri(phi), _tri(GRAVITY * ref)) < 1e-13
+    # without l_cut the 1000 km mountain is resolved at L = 85 under exact Gauss
+    # quadrature and both residuals are round-off (~2e-14): compare absolutely
     res = topo.parameters["projection_residual"]
-    assert abs(res - rel_ref) <= 1e-10 * rel_ref
+    assert abs(res - rel_ref) <= 1e-10 * rel_ref + 1e-13
     assert topo.parameters.get("l_cut") == l_cut
 
 
