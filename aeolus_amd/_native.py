@@ -66,6 +66,7 @@ _SIGNATURES = {
     "aeo_model_profile_read": [_vp, _dp, ctypes.POINTER(_c_int)],
     "aeo_vector_curl_div_work": [_vp, _c_int, ctypes.POINTER(ctypes.c_longlong)],
     "aeo_vector_curl_div": [_vp, _c_double, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp],
+    "aeo_vector_curl_div_noalloc": [_vp, _c_double, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp],
     "aeo_pe_create": [_vp, _vp, _c_int, _dp, _c_double, _c_double, _c_double, _c_double, _vp,
                       ctypes.POINTER(_vp)],
     "aeo_pe_destroy": [_vp],

@@ -24,6 +24,7 @@
 #include "stage.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace aeo {
@@ -300,6 +301,69 @@ static const double* plan_coslat(const aeo_plan_s* p, int* row_len) {
 
 static unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// Weak-form vector curl/div of nfields tangent fields with the caller's
+// workspace `work` (aeo_vector_curl_div_work doubles): the half-metric
+// components on the extended plan's points, their degree-(L+1) analyses, then
+// curl/div.  Every entry of `work` is written before it is read.
+static int vector_curl_div_run(aeo_plan_s* ext, double radius, const double* f_east,
+                               const double* f_north, int nfields, int truncate,
+                               double* curl_out, double* div_out, double* work, hipStream_t s) {
+    AEO_REQUIRE(ext && f_east && f_north && curl_out && div_out && work, AEO_ERR_INVALID,
+                "aeo_vector_curl_div: NULL argument");
+    AEO_REQUIRE(nfields >= 1 && ext->L >= 1, AEO_ERR_INVALID,
+                "aeo_vector_curl_div: needs nfields >= 1 and an extended plan of degree >= 1");
+    AEO_REQUIRE(std::isfinite(radius) && radius > 0.0, AEO_ERR_INVALID,
+                "aeo_vector_curl_div: radius must be finite and > 0");
+    const int L = ext->L - 1;
+    const size_t N = plan_points(ext);
+    double* grid = work;
+    double2* spx = reinterpret_cast<double2*>(work + 2 * N * nfields);
+    int row_len = 0;
+    const double* cl = plan_coslat(ext, &row_len);
+    hipLaunchKernelGGL(halfmetric_kernel, dim3(blocks_for(2 * N * nfields)), dim3(256), 0, s,
+                       (int)N, row_len, cl, nfields, f_east, f_north, grid);
+    AEO_LAUNCH_CHECK();
+    int rc = aeo_analysis(ext, grid, 2 * nfields, reinterpret_cast<double*>(spx), s);
+    if (rc) return rc;
+    const size_t fs = (size_t)(L + 1) * (L + 1);
+    hipLaunchKernelGGL(curl_div_kernel, dim3(blocks_for(fs * nfields)), dim3(256), 0, s, L,
+                       radius, nfields, truncate ? 1 : 0, spx,
+                       reinterpret_cast<double2*>(curl_out), reinterpret_cast<double2*>(div_out));
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+// Workspace of aeo_vector_curl_div_noalloc: a fixed device array, one copy
+// per GPU, resident from the library's load until the process ends (no
+// allocation at call time).  In host code `scratch` names a host-side stand-in,
+// not device memory: the kernels get the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on the entry point's first call there.
+constexpr size_t kNoAllocScratchDoubles = (size_t)2 << 20;   // 16 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kScratchMaxDevices];   // device address per device id
+
+static int scratch_on_current_device(double** out) {
+    int dev = -1;
+    AEO_CHECK_HIP(hipGetDevice(&dev));
+    AEO_REQUIRE(dev >= 0 && dev < kScratchMaxDevices, AEO_ERR_UNSUPPORTED,
+                "aeo_vector_curl_div_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        AEO_REQUIRE(e == hipSuccess && p != nullptr, AEO_ERR_HIP,
+                    std::string("aeo_vector_curl_div_noalloc: hipGetSymbolAddress(scratch): ") +
+                        hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return AEO_OK;
+}
+
 }  // namespace aeo
 
 using namespace aeo;
@@ -480,30 +544,26 @@ int aeo_vector_curl_div_work(aeo_plan_t ext, int nfields, long long* n_doubles) 
 int aeo_vector_curl_div(aeo_plan_t ext, double radius, const double* f_east, const double* f_north,
                         int nfields, int truncate, double* curl_out, double* div_out,
                         double* work, void* stream) {
-    AEO_REQUIRE(ext && f_east && f_north && curl_out && div_out && work, AEO_ERR_INVALID,
-                "aeo_vector_curl_div: NULL argument");
-    AEO_REQUIRE(nfields >= 1 && ext->L >= 1, AEO_ERR_INVALID,
-                "aeo_vector_curl_div: needs nfields >= 1 and an extended plan of degree >= 1");
-    AEO_REQUIRE(std::isfinite(radius) && radius > 0.0, AEO_ERR_INVALID,
-                "aeo_vector_curl_div: radius must be finite and > 0");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int L = ext->L - 1;
-    const size_t N = plan_points(ext);
-    double* grid = work;
-    double2* spx = reinterpret_cast<double2*>(work + 2 * N * nfields);
-    int row_len = 0;
-    const double* cl = plan_coslat(ext, &row_len);
-    hipLaunchKernelGGL(halfmetric_kernel, dim3(blocks_for(2 * N * nfields)), dim3(256), 0, s,
-                       (int)N, row_len, cl, nfields, f_east, f_north, grid);
-    AEO_LAUNCH_CHECK();
-    int rc = aeo_analysis(ext, grid, 2 * nfields, reinterpret_cast<double*>(spx), stream);
+    AEO_REQUIRE(work, AEO_ERR_INVALID, "aeo_vector_curl_div: NULL argument");
+    return vector_curl_div_run(ext, radius, f_east, f_north, nfields, truncate, curl_out, div_out,
+                               work, reinterpret_cast<hipStream_t>(stream));
+}
+
+int aeo_vector_curl_div_noalloc(aeo_plan_t ext, double radius, const double* f_east,
+                                const double* f_north, int nfields, int truncate,
+                                double* curl_out, double* div_out, void* stream) {
+    long long need = 0;
+    int rc = aeo_vector_curl_div_work(ext, nfields, &need);
     if (rc) return rc;
-    const size_t fs = (size_t)(L + 1) * (L + 1);
-    hipLaunchKernelGGL(curl_div_kernel, dim3(blocks_for(fs * nfields)), dim3(256), 0, s, L,
-                       radius, nfields, truncate ? 1 : 0, spx,
-                       reinterpret_cast<double2*>(curl_out), reinterpret_cast<double2*>(div_out));
-    AEO_LAUNCH_CHECK();
-    return AEO_OK;
+    AEO_REQUIRE((unsigned long long)need <= kNoAllocScratchDoubles, AEO_ERR_UNSUPPORTED,
+                "aeo_vector_curl_div_noalloc: needs " + std::to_string(need) +
+                    " doubles of scratch, more than its fixed " +
+                    std::to_string(kNoAllocScratchDoubles) +
+                    " (16 MiB); use aeo_vector_curl_div with a caller workspace");
+    double* work = nullptr;
+    if ((rc = scratch_on_current_device(&work))) return rc;
+    return vector_curl_div_run(ext, radius, f_east, f_north, nfields, truncate, curl_out, div_out,
+                               work, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -295,6 +295,21 @@ int aeo_vector_curl_div(aeo_plan_t ext, double radius, const double* f_east,
                         const double* f_north, int nfields, int truncate, double* curl_out,
                         double* div_out, double* work, void* stream);
 
+/* aeo_vector_curl_div without the `work` argument, for callers that cannot
+ * allocate device memory (code under stream/graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a fixed 16 MiB device array inside the library, one copy per
+ * GPU, held from the library's load until the process ends; the call
+ * allocates nothing and returns AEO_ERR_UNSUPPORTED when
+ * aeo_vector_curl_div_work(ext, nfields) exceeds 2^21 doubles (for example
+ * more than 21 fields on a 128 x 256 product grid at L = 85).  Every call on
+ * one GPU shares that array (and the plan's own workspace): calls that
+ * overlap on one GPU, on any streams, race.  Results equal
+ * aeo_vector_curl_div's bitwise. */
+int aeo_vector_curl_div_noalloc(aeo_plan_t ext, double radius, const double* f_east,
+                                const double* f_north, int nfields, int truncate,
+                                double* curl_out, double* div_out, void* stream);
+
 /* Dry hydrostatic primitive-equation model on a product plan `prod` (degree
  * L) and its degree-(L+1) companion `ext` on the same points.
  *   interfaces: host, nlev + 1 sigma interfaces, exactly 0 first and 1 last,
