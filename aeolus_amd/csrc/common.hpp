@@ -69,6 +69,11 @@ __host__ __device__ inline double2 cmul(double2 a, double2 b) {
 }
 __host__ __device__ inline double2 cscale(double2 a, double s) { return c2(a.x * s, a.y * s); }
 __host__ __device__ inline double2 cconj(double2 a) { return c2(a.x, -a.y); }
+// a * s + c with one rounding per component (written out, so every kernel
+// that forms it rounds alike whatever the compiler would contract)
+__host__ __device__ inline double2 cfma(double2 a, double s, double2 c) {
+    return c2(fma(a.x, s, c.x), fma(a.y, s, c.y));
+}
 
 inline int n_basis(int L) { return (L + 1) * (L + 2) / 2; }
 

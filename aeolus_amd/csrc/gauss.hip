@@ -15,8 +15,8 @@
 // S = w (F_north + F_south) and D = w (F_north - F_south).  One complex FFT
 // of length nlon transforms a north/south row pair at once
 // (z = f_north + i f_south).
-#include "legendre.hpp"
 #include "split.hpp"
+#include "stage_launch.hpp"
 
 namespace aeo {
 
@@ -342,6 +342,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
     const size_t ms = (size_t)nmem * 4;
+    // row constants are loaded first: no later phase waits on global memory
+    const double cN = clamp_coslat(g.coslat[jN]);
+    const double cS = clamp_coslat(g.coslat[jS]);
+    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     // LDS: rows [4][rs] | twiddles [n] | E, O columns [4][L+1] each
     double2* sE = lds + 4 * rs + n;
     double2* sO = sE + (L + 1) * 4;
@@ -355,8 +359,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 4, g.fft, twl);
-    const double cN = clamp_coslat(g.coslat[jN]);
-    const double cS = clamp_coslat(g.coslat[jS]);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
@@ -367,7 +369,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<-1, P2>(lds, 1, g.fft, twl);
-    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = (size_t)jp * (L + 1) * nmem + b;
     unpack_sd(lds, n, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
 }
@@ -397,6 +398,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 10;
     const size_t ms = (size_t)nmem * 10;
+    const double cN = clamp_coslat(g.coslat[jN]);
+    const double cS = clamp_coslat(g.coslat[jS]);
+    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     // grid row r <- (Legendre field, spectral factor)
     constexpr int src[12] = {0, 1, 2, 3, 4, 5, 4, 6, 7, 6, 8, 9};
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
@@ -421,8 +425,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, nr, g.fft, twl);
-    const double cN = clamp_coslat(g.coslat[jN]);
-    const double cS = clamp_coslat(g.coslat[jS]);
     const double icN = 1.0 / cN, icS = 1.0 / cS;
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
@@ -454,7 +456,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<-1, P2>(lds, 2, g.fft, twl);
-    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
     for (int r = 0; r < 2; ++r)
         unpack_sd(lds + r * rs, n, mmax, w, S + so + 2 * half + r, D + so + 2 * half + r,
@@ -483,6 +484,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 5;
     const size_t ms = (size_t)nmem * 5;
+    const double cN = clamp_coslat(g.coslat[jN]);
+    const double cS = clamp_coslat(g.coslat[jS]);
+    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     // LDS: rows [3][rs] (each half transforms 3 grid rows) | twiddles [n] |
     // E, O columns [5][L+1] each
     double2* sE = lds + kAdjLdsRows * rs + n;
@@ -501,8 +505,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<+1, P2>(lds, 3, g.fft, twl);
-    const double cN = clamp_coslat(g.coslat[jN]);
-    const double cS = clamp_coslat(g.coslat[jS]);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < n; k += kGridThreads) {
         const int e = lp(k);
@@ -519,7 +521,6 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
     }
     __syncthreads();
     fft_rows<-1, P2>(lds, 2, g.fft, twl);
-    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
     const size_t so = ((size_t)jp * (L + 1) * nmem + b) * 4;
     for (int r = 0; r < 2; ++r)
         unpack_sd(lds + r * rs, n, L, w, S + so + 2 * half + r, D + so + 2 * half + r,
@@ -541,59 +542,6 @@ AEO_GRID_PICK(grid_adj_kernel)
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static ColK col_consts(const aeo_plan_s* p, double R, const double* lap, double f10) {
-    return ColK{p->L, R, p->d_CpT, p->d_CmT, lap, f10};
-}
-
-template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
-static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& ep,
-                           hipStream_t s) {
-    const size_t lds = StageLds<KS, Epi, MB, NS>::bytes(p->L, p->nh, ANA, SYN);
-    if (lds > 163840) armed_events() = ArmedEvents{};
-    AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
-                "Legendre stage kernel: LDS demand exceeds 160 KiB (reduce members per block)");
-    auto kern = leg_stage_kernel<KS, Epi, MB, ANA, SYN, NS>;
-    if (lds > 65536) {
-        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
-    dim3 grid(p->L + 1, 1, (a.nmem + MB - 1) / MB);
-    launch_k(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
-    AEO_LAUNCH_CHECK();
-    return AEO_OK;
-}
-
-template <class KS, class Epi, bool ANA, bool SYN, int NS>
-static int launch_stage_ns(const aeo_plan_s* p, int mb, const StageArgs& a, const Epi& ep,
-                           hipStream_t s) {
-    switch (mb) {
-        case 1: return launch_stage_mb<KS, Epi, 1, ANA, SYN, NS>(p, a, ep, s);
-        case 2: return launch_stage_mb<KS, Epi, 2, ANA, SYN, NS>(p, a, ep, s);
-        case 4: return launch_stage_mb<KS, Epi, 4, ANA, SYN, NS>(p, a, ep, s);
-        default: return launch_stage_mb<KS, Epi, 8, ANA, SYN, NS>(p, a, ep, s);
-    }
-}
-
-// ns = 0: FP64 contractions; ns in {4, 5, 6, 7}: split digits (plan tables must match).
-template <class KS, class Epi, bool ANA, bool SYN>
-static int launch_stage(const aeo_plan_s* p, int mb, const StageArgs& a, const Epi& ep,
-                        hipStream_t s, int ns = 0) {
-    if (ns != 0) {
-        AEO_REQUIRE(p->nslices == ns, AEO_ERR_STATE,
-                    "split-precision stage: the plan's digit tables have a different slice count");
-    }
-    switch (ns) {
-        case 0: return launch_stage_ns<KS, Epi, ANA, SYN, 0>(p, mb, a, ep, s);
-        case 4: return launch_stage_ns<KS, Epi, ANA, SYN, 4>(p, mb, a, ep, s);
-        case 5: return launch_stage_ns<KS, Epi, ANA, SYN, 5>(p, mb, a, ep, s);
-        case 6: return launch_stage_ns<KS, Epi, ANA, SYN, 6>(p, mb, a, ep, s);
-        case 7: return launch_stage_ns<KS, Epi, ANA, SYN, 7>(p, mb, a, ep, s);
-        default:
-            set_error("split-precision stage: nslices must be 4, 5, 6 or 7");
-            return AEO_ERR_UNSUPPORTED;
-    }
-}
-
 int gauss_build_tables(aeo_plan_s* p, hipStream_t s) {
     dim3 grid(p->L + 1, (p->nh + 63) / 64);
     hipLaunchKernelGGL(legendre_table_kernel, grid, dim3(64), 0, s, p->gdev(), p->d_Ps, p->d_Pa);
@@ -756,6 +704,8 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     sa.O = md->d_O;
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     md->prof.arm(kClsSynthesis);
+    if (bve1_supported(md))
+        return bve1_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true, s);
     int rc = (md->kind == kBVE)
                  ? launch_stage<KindBve, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
                                                                  md->nslices)
@@ -814,7 +764,9 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     Rk4B rk{stage, stage == 2 ? dt : 0.5 * dt, dt / 6.0, y0, md->d_ksum, ynext, kout};
     int rc;
     pr.arm(kClsAnalysis);
-    if (md->kind == kBVE) {
+    if (md->kind == kBVE && bve1_supported(md)) {
+        rc = bve1_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next, s);
+    } else if (md->kind == kBVE) {
         EpiBveS ep{L, fs, forcing, md->d_nulap, rk};
         rc = stage_ana<KindBve>(md, sa, ep, syn_next, s);
     } else {

@@ -163,8 +163,8 @@ __device__ inline void derive_columns(const double2* st, int m, const ColK& k, d
             const int l = m + i;
             const double2* s = cols + src * K;
             double2 v = c2(0.0, 0.0);
-            if (i >= 1) v = cadd(v, cscale(s[i - 1], k.cpc[l - 1]));
-            if (l <= k.L - 1) v = cadd(v, cscale(s[i + 1], k.cmc[l + 1]));
+            if (i >= 1) v = cfma(s[i - 1], k.cpc[l - 1], v);
+            if (l <= k.L - 1) v = cfma(s[i + 1], k.cmc[l + 1], v);
             cols[t] = v;
         }
     }
@@ -177,24 +177,33 @@ __device__ inline void derive_columns(const double2* st, int m, const ColK& k, d
 // (read the stage input, overwrite with the next stage state), writer =
 // whether this workgroup writes the global outputs.
 // ---------------------------------------------------------------------------
+// Global inputs of an epilogue entry (y0, ksum, forcing, topography, ...) are
+// read by prefetch(b, l, m, pf) into NPF double2 slots; the single-member
+// stage kernel issues these loads with its prologue (they do not depend on
+// the launch's own results) and hands them to apply(); other launches pass
+// nullptr and apply() reads them itself.
 struct EpiNoneS {
-    static constexpr int NA = 1, NSTATE = 1;
-    __device__ void apply(int, int, int, const double2*, double2*, bool) const {}
+    static constexpr int NA = 1, NSTATE = 1, NPF = 0;
+    __device__ void prefetch(int, int, int, double2*) const {}
+    __device__ void apply(int, int, int, const double2*, double2*, bool, const double2*) const {}
 };
 
 // Store the analysed field (generic analysis: one member per field; Jacobian).
 struct EpiStore1S {
-    static constexpr int NA = 1, NSTATE = 1;
+    static constexpr int NA = 1, NSTATE = 1, NPF = 0;
     double2* out;
     int L;
     size_t member_stride;
-    __device__ void apply(int b, int l, int m, const double2* acc, double2*, bool writer) const {
+    __device__ void prefetch(int, int, int, double2*) const {}
+    __device__ void apply(int b, int l, int m, const double2* acc, double2*, bool writer,
+                          const double2*) const {
         if (writer) out[b * member_stride + l * (L + 1) + m] = acc[0];
     }
 };
 
 // RK4 stage bookkeeping on one entry (see Rk4 in stage.hpp); returns the
-// next stage state (or k for the tendency-only mode).
+// next stage state (or k for the tendency-only mode).  Inputs y0[idx] and
+// ksum[idx] come from pre() (2 slots).
 struct Rk4B {
     int stage;      // -1: tendency only
     double cst;
@@ -203,53 +212,73 @@ struct Rk4B {
     double2* ksum;
     double2* ynext;
     double2* kout;
-    __device__ double2 apply(size_t idx, double2 k, bool writer) const {
+    __device__ void pre(size_t idx, double2* pf) const {
+        pf[0] = (stage >= 0) ? y0[idx] : c2(0.0, 0.0);
+        pf[1] = (stage >= 1) ? ksum[idx] : c2(0.0, 0.0);
+    }
+    __device__ double2 apply(size_t idx, double2 k, bool writer, const double2* pf) const {
         if (stage < 0) {
             if (writer) kout[idx] = k;
             return k;
         }
-        const double2 y = y0[idx];
+        const double2 y = pf[0];
         double2 nx;
         if (stage == 0) {
             if (writer) ksum[idx] = k;
-            nx = cadd(y, cscale(k, cst));
+            nx = cfma(k, cst, y);
         } else if (stage < 3) {
-            // only the writer tile touches ksum here: the other latitude tiles
-            // of the column must not read an entry the writer is updating
-            if (writer) ksum[idx] = cadd(ksum[idx], cscale(k, 2.0));
-            nx = cadd(y, cscale(k, cst));
+            if (writer) ksum[idx] = cfma(k, 2.0, pf[1]);
+            nx = cfma(k, cst, y);
         } else {
-            const double2 s = cadd(ksum[idx], k);
-            nx = cadd(y, cscale(s, dt6));
+            const double2 s = cadd(pf[1], k);
+            nx = cfma(s, dt6, y);
         }
         if (writer) ynext[idx] = nx;
         return nx;
     }
 };
 
-// BVE: k = -J + nu lap' zeta + forcing, row 0 pinned.
+// BVE: k = -J + nu lap' zeta + forcing, row 0 pinned.  Slots: y0, ksum,
+// forcing, (nu lap'_l, 0).
 struct EpiBveS {
-    static constexpr int NA = 1, NSTATE = 1;
+    static constexpr int NA = 1, NSTATE = 1, NPF = 4;
     int L;
     size_t member_stride;
     const double2* forcing;  // per member, or nullptr
     const double* nulap;
     Rk4B rk;
-    __device__ void apply(int b, int l, int m, const double2* acc, double2* st,
-                          bool writer) const {
-        const size_t idx = b * member_stride + (size_t)m * (L + 1) + l;   // internal [m][l]
+    __device__ size_t index(int b, int l, int m) const {
+        return b * member_stride + (size_t)m * (L + 1) + l;   // internal [m][l]
+    }
+    __device__ void prefetch(int b, int l, int m, double2* pf) const {
+        const size_t idx = index(b, l, m);
+        rk.pre(idx, pf);
+        pf[2] = forcing ? forcing[idx] : c2(0.0, 0.0);
+        pf[3] = c2(nulap[l], 0.0);
+    }
+    __device__ void apply(int b, int l, int m, const double2* acc, double2* st, bool writer,
+                          const double2* pin) const {
+        double2 pf[NPF];
+        if (pin) {
+#pragma unroll
+            for (int i = 0; i < NPF; ++i) pf[i] = pin[i];
+        } else {
+            prefetch(b, l, m, pf);
+        }
+        const size_t idx = index(b, l, m);
         double2 k = c2(0.0, 0.0);
         if (l > 0) {
-            k = cadd(c2(-acc[0].x, -acc[0].y), cscale(st[l], nulap[l]));
-            if (forcing) k = cadd(k, forcing[idx]);
+            k = cfma(st[l], pf[3].x, c2(-acc[0].x, -acc[0].y));
+            if (forcing) k = cadd(k, pf[2]);
         }
-        st[l] = rk.apply(idx, k, writer);
+        st[l] = rk.apply(idx, k, writer, pf);
     }
 };
 
-// SWE assembly (phy/shallow_water.py:389-418).
+// SWE assembly (phy/shallow_water.py:389-418).  Slots: (y0, ksum) per field,
+// lap phi_s, nu4 lap^2 phi_s, (lap_l, nu4 lap_l^2).
 struct EpiSweS {
-    static constexpr int NA = 4, NSTATE = 3;
+    static constexpr int NA = 4, NSTATE = 3, NPF = 9;
     int L;
     size_t member_stride;
     const double* lap;
@@ -258,8 +287,28 @@ struct EpiSweS {
     const double2* nu4_lap2_phi_s;
     double phi0;
     Rk4B rk;
-    __device__ void apply(int b, int l, int m, const double2* acc, double2* st,
-                          bool writer) const {
+    __device__ void prefetch(int b, int l, int m, double2* pf) const {
+        const int n = L + 1;
+        const size_t fs = (size_t)n * n;
+        const size_t e = (size_t)m * n + l;   // internal [m][l]
+        const size_t base = b * member_stride;
+        rk.pre(base + e, pf);
+        rk.pre(base + fs + e, pf + 2);
+        rk.pre(base + 2 * fs + e, pf + 4);
+        const size_t eu = (size_t)l * n + m;  // topography terms keep the boundary layout
+        pf[6] = lap_phi_s ? lap_phi_s[eu] : c2(0.0, 0.0);
+        pf[7] = nu4_lap2_phi_s ? nu4_lap2_phi_s[eu] : c2(0.0, 0.0);
+        pf[8] = c2(lap[l], nu4lap2 ? nu4lap2[l] : 0.0);
+    }
+    __device__ void apply(int b, int l, int m, const double2* acc, double2* st, bool writer,
+                          const double2* pin) const {
+        double2 pf[NPF];
+        if (pin) {
+#pragma unroll
+            for (int i = 0; i < NPF; ++i) pf[i] = pin[i];
+        } else {
+            prefetch(b, l, m, pf);
+        }
         const int n = L + 1;
         const size_t fs = (size_t)n * n;
         const size_t e = (size_t)m * n + l;   // internal [m][l]
@@ -269,20 +318,19 @@ struct EpiSweS {
             const double2 zeta = st[l], delta = st[n + l], phi = st[2 * n + l];
             kz = acc[0];
             kp = csub(acc[3], cscale(delta, phi0));
-            kd = csub(acc[1], cscale(cadd(acc[2], phi), lap[l]));
-            const size_t eu = (size_t)l * n + m;  // topography terms keep the boundary layout
-            if (lap_phi_s) kd = csub(kd, lap_phi_s[eu]);
+            kd = csub(acc[1], cscale(cadd(acc[2], phi), pf[8].x));
+            if (lap_phi_s) kd = csub(kd, pf[6]);
             if (nu4lap2) {
-                const double q = nu4lap2[l];
+                const double q = pf[8].y;
                 kz = csub(kz, cscale(zeta, q));
                 kd = csub(kd, cscale(delta, q));
                 kp = csub(kp, cscale(phi, q));
-                if (nu4_lap2_phi_s) kp = csub(kp, nu4_lap2_phi_s[eu]);
+                if (nu4_lap2_phi_s) kp = csub(kp, pf[7]);
             }
         }
-        st[l] = rk.apply(base + e, kz, writer);
-        st[n + l] = rk.apply(base + fs + e, kd, writer);
-        st[2 * n + l] = rk.apply(base + 2 * fs + e, kp, writer);
+        st[l] = rk.apply(base + e, kz, writer, pf);
+        st[n + l] = rk.apply(base + fs + e, kd, writer, pf + 2);
+        st[2 * n + l] = rk.apply(base + 2 * fs + e, kp, writer, pf + 4);
     }
 };
 
@@ -294,8 +342,8 @@ struct EpiSweS {
 //   r=1: asum += yb_3,      kb_2 = dt/3 a + dt/2 yb_3
 //   r=2: asum += yb_2,      kb_1 = dt/6 a + dt/2 yb_2
 //   r=3: anext = asum + yb_1 (cotangent of the step input), kb_4' = dt/6 anext
-// Only the writer tile touches asum in r = 0..2; r = 3 reads it in every tile
-// and writes nothing but anext.  Returns the next stage's kb.
+// Inputs a[idx] and asum[idx] come from pre() (2 slots).  Returns the next
+// stage's kb.
 struct Rk4Adj {
     int stage;          // 0..3; -1: tendency adjoint only (kout = yb)
     double dt;
@@ -304,24 +352,28 @@ struct Rk4Adj {
     double2* anext;
     double2* kout;
     double2* knext;     // next kb (global copy for the next launch's epilogue)
-    __device__ double2 apply(size_t idx, double2 yb, bool writer) const {
+    __device__ void pre(size_t idx, double2* pf) const {
+        pf[0] = (stage >= 0) ? a[idx] : c2(0.0, 0.0);
+        pf[1] = (stage >= 1) ? asum[idx] : c2(0.0, 0.0);
+    }
+    __device__ double2 apply(size_t idx, double2 yb, bool writer, const double2* pf) const {
         if (stage < 0) {
             if (writer) kout[idx] = yb;
             return c2(0.0, 0.0);
         }
-        const double2 av = a[idx];
+        const double2 av = pf[0];
         double2 nx;
         if (stage == 0) {
             if (writer) asum[idx] = cadd(av, yb);
             nx = cadd(cscale(av, dt / 3.0), cscale(yb, dt));
         } else if (stage == 1) {
-            if (writer) asum[idx] = cadd(asum[idx], yb);
+            if (writer) asum[idx] = cadd(pf[1], yb);
             nx = cadd(cscale(av, dt / 3.0), cscale(yb, 0.5 * dt));
         } else if (stage == 2) {
-            if (writer) asum[idx] = cadd(asum[idx], yb);
+            if (writer) asum[idx] = cadd(pf[1], yb);
             nx = cadd(cscale(av, dt / 6.0), cscale(yb, 0.5 * dt));
         } else {
-            const double2 tot = cadd(asum[idx], yb);
+            const double2 tot = cadd(pf[1], yb);
             if (writer) anext[idx] = tot;
             nx = cscale(tot, dt / 6.0);
         }
@@ -339,13 +391,14 @@ struct Rk4Adj {
 //   zb   = nu lap' kb' + psib / lap (l >= 1) + etab,     kb' = kb with row 0 = 0,
 // with (S^T x)_l = C+_l x_{l+1} + C-_l x_{l-1} (l-1 >= m).  State column 0
 // receives the forward state of the next adjoint stage, column 1 its kb.
+// Slots: a, asum, the next forward state, (nu lap'_l, lap_l).
 template <class T, class = void>
 struct epi_column { static constexpr bool value = false; };
 template <class T>
 struct epi_column<T, decltype((void)T::COLUMN)> { static constexpr bool value = T::COLUMN; };
 
 struct EpiBveAdjS {
-    static constexpr int NA = 4, NSTATE = 2;
+    static constexpr int NA = 4, NSTATE = 2, NPF = 4;
     static constexpr bool COLUMN = true;
     int L;
     double R;
@@ -356,35 +409,54 @@ struct EpiBveAdjS {
     const double* Cm;          // column-major [m][l]
     const double2* fwd_next;   // forward stage state of the next adjoint stage, or nullptr
     Rk4Adj rk;
-    __device__ double2 st_term(const double2* x, int l, int m) const {
-        const int n = L + 1;
+    __device__ size_t index(int b, int l, int m) const {
+        return b * member_stride + (size_t)m * (L + 1) + l;   // internal [m][l]
+    }
+    __device__ void prefetch(int b, int l, int m, double2* pf) const {
+        const size_t idx = index(b, l, m);
+        rk.pre(idx, pf);
+        pf[2] = fwd_next ? fwd_next[idx] : c2(0.0, 0.0);
+        pf[3] = c2(nulap[l], lap[l]);
+    }
+    // (S^T x)_l with the column-m coefficients cpc / cmc ([l], LDS or global)
+    __device__ double2 st_term(const double2* x, int l, int m, const double* cpc,
+                               const double* cmc) const {
         double2 v = c2(0.0, 0.0);
         if (l >= m) {
-            if (l + 1 <= L) v = cadd(v, cscale(x[l + 1], Cp[m * n + l]));
-            if (l - 1 >= m) v = cadd(v, cscale(x[l - 1], Cm[m * n + l]));
+            if (l + 1 <= L) v = cadd(v, cscale(x[l + 1], cpc[l]));
+            if (l - 1 >= m) v = cadd(v, cscale(x[l - 1], cmc[l]));
         }
         return v;
     }
-    // acc: the member's analysed columns [NA][n]; st: its state columns [2][n]
+    // acc: the member's analysed columns [NA][n]; st: its state columns [2][n];
+    // cpc / cmc: C+ / C- of column m
     __device__ void apply_col(int b, int l, int m, const double2* acc, double2* st,
-                              bool writer) const {
+                              bool writer, const double2* pin, const double* cpc,
+                              const double* cmc) const {
+        double2 pf[NPF];
+        if (pin) {
+#pragma unroll
+            for (int i = 0; i < NPF; ++i) pf[i] = pin[i];
+        } else {
+            prefetch(b, l, m, pf);
+        }
         const int n = L + 1;
-        const size_t idx = b * member_stride + (size_t)m * n + l;   // internal [m][l]
+        const size_t idx = index(b, l, m);
         const double q = (double)m / R, invR = 1.0 / R;
         const double cm = (m == 0) ? 1.0 : 2.0;
         const double2 al = acc[l], bl = acc[2 * n + l];
-        const double2 sa = st_term(acc + n, l, m), sb = st_term(acc + 3 * n, l, m);
+        const double2 sa = st_term(acc + n, l, m, cpc, cmc), sb = st_term(acc + 3 * n, l, m, cpc, cmc);
         // -(i q) x = (q x.y, -q x.x)
         const double2 psib = cscale(c2(q * al.y + invR * sa.x, -q * al.x + invR * sa.y), cm);
         const double2 etab = cscale(c2(q * bl.y + invR * sb.x, -q * bl.x + invR * sb.y), cm);
         double2 zb = etab;
         if (l > 0) {
-            const double d = lap[l];
+            const double d = pf[3].y;
             zb = cadd(zb, c2(psib.x / d, psib.y / d));
-            zb = cadd(zb, cscale(st[n + l], nulap[l]));
+            zb = cadd(zb, cscale(st[n + l], pf[3].x));
         }
-        const double2 kn = rk.apply(idx, zb, writer);
-        st[l] = fwd_next ? fwd_next[idx] : c2(0.0, 0.0);
+        const double2 kn = rk.apply(idx, zb, writer, pf);
+        st[l] = fwd_next ? pf[2] : c2(0.0, 0.0);
         st[n + l] = kn;
     }
 };
@@ -412,6 +484,13 @@ struct StageArgs {
 // doubles, rounded up) and the recurrence pairs (a_lm, b_lm) (n double2)
 __host__ __device__ inline int coef_slots(int n) { return (3 * n + 1) / 2 + n; }
 
+// double2 slots per degree of the epilogue prefetch: single-member launches
+// with an analysis load the epilogue's global inputs with the prologue.
+template <class Epi, int MB>
+__host__ __device__ constexpr int epi_pre_slots(bool ana) {
+    return (MB == 1 && ana) ? Epi::NPF : 0;
+}
+
 template <class KS, class Epi, int MB, int NS>
 struct StageLds {
     static constexpr int NST = (KS::NSTATE > Epi::NSTATE) ? KS::NSTATE : Epi::NSTATE;
@@ -419,7 +498,9 @@ struct StageLds {
     static constexpr int NCS = MB * KS::NF * 2, NTS = (NCS + 15) / 16;
     static size_t bytes(int L, int nh, bool ana, bool syn) {
         const int n = L + 1;
-        size_t s = sizeof(double2) * ((size_t)MB * NST * n + coef_slots(n));  // state + C+/C-/lap
+        // state + C+/C-/lap + the single-member epilogue prefetch
+        size_t s = sizeof(double2) * ((size_t)MB * NST * n + coef_slots(n) +
+                                      (size_t)epi_pre_slots<Epi, MB>(ana) * n);
         size_t a = 0, c = 0;
         if (ana) {
             a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) +
@@ -480,7 +561,10 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     double* cmc = cpc + n;                                                 // C- column m
     double* lapc = cmc + n;                                                // lap
     double2* rcc = lds + (size_t)MB * NST * n + (3 * n + 1) / 2;           // (a, b) of m
-    double2* work = lds + (size_t)MB * NST * n + coef_slots(n);
+    constexpr int kNPF = epi_pre_slots<Epi, MB>(ANA);
+    constexpr bool kEpiPre = kNPF > 0;
+    double2* epf = lds + (size_t)MB * NST * n + coef_slots(n);              // [n][kNPF]
+    double2* work = epf + (size_t)kNPF * n;
     ColK kc = a.k;
     kc.cpc = cpc;
     kc.cmc = cmc;
@@ -493,7 +577,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     //    into LDS.
     double pa[kPC][kPre];
     if constexpr (ANA && NS == 0) {
-        const double* Pa = g.Pa + g.offs[m];
+        const double* Pa = g.Pa + g.block_offset(m);
 #pragma unroll
         for (int c = 0; c < kPC; ++c)
 #pragma unroll
@@ -591,7 +675,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         const int Kb = ((K + 3) / 4 + 1) & ~1;
         const int i0 = kb * Kb;
         if (jp < nh && i0 < K) {
-            const double* P = g.Ps + g.offs[m] + jp;
+            const double* P = g.Ps + g.block_offset(m) + jp;
             sx = g.xn[jp];
             sp0 = P[(size_t)i0 * nh];
             if (i0 + 1 < K) sp1 = P[(size_t)(i0 + 1) * nh];
@@ -610,6 +694,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
         for (int t0 = 0; t0 < tot; t0 += 2 * kStageThreads) {
             double2 vs[2], vx[2][2], vr[2];
             double vc[2][3];
+            double2 vp[2][kNPF > 0 ? kNPF : 1];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int t = t0 + threadIdx.x + u * kStageThreads;
@@ -647,6 +732,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 vc[u][2] = a.k.lap ? a.k.lap[tc] : 0.0;
                 vr[u] = c2(0.0, 0.0);
                 if constexpr (SYN && NS == 0) vr[u] = g.rec[(size_t)m * n + tc];   // FP64 recurrence
+                if constexpr (kEpiPre) ep.prefetch(b0, tc, m, vp[u]);                // epilogue inputs
             }
             if (t0 == 0) {
                 if constexpr (ANA && NS > 0) prefetch_ana();
@@ -672,6 +758,10 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     cmc[tc] = vc[u][1];
                     lapc[tc] = vc[u][2];
                     if constexpr (SYN && NS == 0) rcc[tc] = vr[u];
+                    if constexpr (kEpiPre) {
+#pragma unroll
+                        for (int i = 0; i < kNPF; ++i) epf[(size_t)tc * kNPF + i] = vp[u][i];
+                    }
                 }
             }
         }
@@ -843,7 +933,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             // one chunk of 64 degrees per pass (lanes = degrees); the first kPC
             // chunks use the rows prefetched in the prologue
             constexpr int kCH = 1;
-            const double* Pa = g.Pa + g.offs[m];
+            const double* Pa = g.Pa + g.block_offset(m);
             for (int l0 = m, c = 0; l0 <= top; l0 += 64 * kCH, ++c) {
                 double2 acc[kCH][MB][NA];
 #pragma unroll
@@ -921,14 +1011,15 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             const int bb = t / n, l = t - bb * n;
             const int b = b0 + bb;
             if (b >= a.nmem) continue;
+            const double2* pin = kEpiPre ? epf + (size_t)t * kNPF : nullptr;
             if constexpr (epi_column<Epi>::value) {
                 ep.apply_col(b, l, m, accl + (size_t)bb * NA * n, st + (size_t)bb * NST * n,
-                             writer);
+                             writer, pin, cpc, cmc);
             } else {
                 double2 acc[NA];
 #pragma unroll
                 for (int f = 0; f < NA; ++f) acc[f] = accl[(bb * NA + f) * n + l];
-                ep.apply(b, l, m, acc, st + (size_t)bb * NST * n, writer);
+                ep.apply(b, l, m, acc, st + (size_t)bb * NST * n, writer, pin);
             }
         }
         __syncthreads();
@@ -1142,7 +1233,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
 #pragma unroll
                 for (int f = 0; f < NF; ++f) accE[bb][f] = accO[bb][f] = c2(0.0, 0.0);
             if (ok && i0 < i1) {
-                const double* P = g.Ps + g.offs[m] + jp;
+                const double* P = g.Ps + g.block_offset(m) + jp;
                 const bool first = (jbase == 0);        // seeds issued in the prologue
                 const double x = first ? sx : g.xn[jp];
                 double p2 = first ? sp0 : P[(size_t)i0 * nh];

@@ -14,6 +14,11 @@ struct GaussDev {
     const double* Ps;         // synthesis table: block m, [l-m][jp]   (jp fastest)
     const double* Pa;         // analysis table:  block m, [jp][l-m]   (l fastest)
     const long long* offs;    // element offset of block m (L+2 entries)
+    // the same offset in closed form, nh sum_{m' < m} (L - m' + 1): kernels
+    // use it instead of a dependent load of offs[m]
+    __host__ __device__ long long block_offset(int m) const {
+        return (long long)nh * ((long long)m * (L + 1) - (long long)m * (m - 1) / 2);
+    }
     const double* w_rows;     // nlat solid-angle weights per point of a row
     const double* coslat;     // nlat cos(latitude), unclamped
     const double* xn;         // nh northern nodes x_j = cos(colatitude)
@@ -226,6 +231,15 @@ int gauss_model_prime(aeo_model_s* m, const double2* y, hipStream_t s);
 int gauss_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
                       double2* ynext, bool syn_next, hipStream_t s);
+
+// Single-member FP64 BVE stage kernel (bve1.hip): the same results as the
+// generic stage kernel for models with one member and FP64 contractions.
+// ana: analysis + tendency + RK4 epilogue of stage `stage` (-1: tendency into
+// kout); syn: synthesis of the resulting (ana) or given (!ana) state into E/O.
+bool bve1_supported(const aeo_model_s* md);
+int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
+                double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
+                hipStream_t s);
 
 // Adjoint of the BVE stage (Gauss plans).  prime: synthesis of the forward
 // fields of yfwd and of the tendency cotangent kb into the adjoint E/O

@@ -442,7 +442,7 @@ def _blas_threads() -> int:
         return 1
 
 
-def reference_algorithm_cpu(c3_steps: int = 2):
+def reference_algorithm_cpu(c3_steps: int = 2, progress=lambda msg: None):
     """The reference algorithm itself on the host cores: dense basis Y built
     with the reference kernel's formula (cu/sh_matrix.cu:7-76), analysis
     Y^H (w f) and synthesis 2 Re(Y a) - Re(Y a_0) per field
@@ -477,10 +477,12 @@ def reference_algorithm_cpu(c3_steps: int = 2):
     z = p.sh.transform(gaussian_vortex_pair(p.grid.point_latitudes, p.grid.point_longitudes,
                                             R_EARTH, seed=1))
     run_bve("c1_bve_gauss12x24_L8", p, z, 600.0, 144)
+    progress("CPU reference algorithm C1 done")
     p = OPlanet(l_max=21, radius=R_EARTH, omega=OMEGA, grid_type="geodesic", resolution=4,
                 basis="reference")
     run_bve("c2_bve_geodesic_res4_L21", p, spectrum_l15(21, np.random.default_rng(21), rms=5e-5),
             900.0, 96)
+    progress("CPU reference algorithm C2 done")
     t0 = time.perf_counter()
     p = OPlanet(l_max=85, radius=R_EARTH, omega=OMEGA, grid_type="latlon", nlat=128, nlon=256,
                 dense=True, basis="reference")
@@ -492,9 +494,11 @@ def reference_algorithm_cpu(c3_steps: int = 2):
                     width_deg=math.degrees(1e6 / R_EARTH))
     prm = SWEParams(p, gravity=9.80616, mean_depth=1e4, elevation_lm=elev)
     y = swe_spectra(85, R_EARTH, 9.80616, seed=85)
+    progress("CPU reference algorithm C3 basis built")
     t0 = time.perf_counter()
-    for _ in range(c3_steps):
+    for k in range(c3_steps):
         y = rk4_step_array(lambda s: swe_tendency(prm, s), y, DT)
+        progress(f"CPU reference algorithm C3 step {k + 1}")
     el = time.perf_counter() - t0
     res["c3_swe_gauss128x256_L85_mountain"] = {"steps": c3_steps, "seconds": el,
                                                "steps_per_s": c3_steps / el,
@@ -583,6 +587,11 @@ def _mode_name(ns):
     return "fp64" if ns == 0 else f"split{ns}"
 
 
+def _progress(msg: str) -> None:
+    """One progress line per finished part on stderr (stdout keeps the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = _args()
     if args.copy_calibration:
@@ -600,6 +609,7 @@ def main():
                                    "run with --gpus 1"}))
         sys.exit(2)
     h = headline(args)
+    _progress("headline L=170 done")
     best, dom = h["best"], h["dom"]
     nbytes = _stage_bytes(dom, 170, 256, best)
     achieved = nbytes / (h["dom_ms"] * 1e-3) / 1e9
@@ -613,6 +623,7 @@ def main():
     }
     if not args.headline_only:
         _, _, m85, b85 = bve_modes(85, 128, 256, args.steps, args.warmup, 85)
+        _progress("L=85 modes done")
         fp = h["modes"][0]["ms_per_step"]
         secondary["bve170_split_speedup_vs_fp64"] = {
             _mode_name(k): fp / v["ms_per_step"] for k, v in h["modes"].items() if k}
@@ -623,10 +634,12 @@ def main():
         ns_split = best if best else 6
         secondary["ensemble_256_members"] = {_mode_name(k): v for k, v in
                                              ensemble_modes(48, ns_split).items()}
+        _progress("SWE and ensemble done")
         secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
         secondary["geodesic_res4_L21_modes"] = geodesic_modes(args.steps, args.warmup)
         secondary["primitive_equations_K8"] = pe_modes(min(args.steps, 100))
         secondary["c1_bve_gauss12x24_L8_fp64_ms_per_step"] = c1_device(144)
+        _progress("device secondaries done")
     line = {
         "metric": "RK4 steps/s at lmax 85/170 vs plain FP64 at rel-err<=1e-11; adjoint:fwd cost",
         "value": 1000.0 / h["ms_step"],
@@ -655,7 +668,8 @@ def main():
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
     }
     if not args.no_cpu and not args.headline_only:
-        ref = reference_algorithm_cpu()
+        _progress("CPU port leg done")
+        ref = reference_algorithm_cpu(progress=_progress)
         device_ms = {
             "c1_bve_gauss12x24_L8": secondary["c1_bve_gauss12x24_L8_fp64_ms_per_step"],
             "c2_bve_geodesic_res4_L21": secondary["geodesic_res4_L21_modes"]["fp64"]["ms_per_step"],
