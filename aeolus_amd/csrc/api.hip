@@ -185,14 +185,15 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
         tw[t] = make_double2(std::cos(ang), -std::sin(ang));
     }
     std::vector<double> xn(x_nodes, x_nodes + nh);
-    // coefficients of Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}), m-major
+    // Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}), stored m-major as
+    // (a_lm, a_lm b_lm) for the one-fma step of rec_step
     std::vector<double2> rec((size_t)(L + 1) * (L + 1), make_double2(0.0, 0.0));
     for (int m = 0; m <= L; ++m)
         for (int l = m + 2; l <= L; ++l) {
             const double ll = (double)l * l, mm = (double)m * m, l1 = l - 1.0;
-            rec[(size_t)m * (L + 1) + l] =
-                make_double2(std::sqrt((4.0 * ll - 1.0) / (ll - mm)),
-                             std::sqrt((l1 * l1 - mm) / (4.0 * l1 * l1 - 1.0)));
+            const double a = std::sqrt((4.0 * ll - 1.0) / (ll - mm));
+            const double b = std::sqrt((l1 * l1 - mm) / (4.0 * l1 * l1 - 1.0));
+            rec[(size_t)m * (L + 1) + l] = make_double2(a, a * b);
         }
     int rc = AEO_OK;
     const size_t plane = (size_t)nh * (L + 1);

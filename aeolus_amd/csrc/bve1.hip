@@ -30,6 +30,16 @@ constexpr int kB1Waves = kB1Threads / 64;
 constexpr int kB1Pre = 16;    // analysis-table rows prefetched per lane and chunk
 constexpr int kB1Chunks = 2;  // prefetched 64-degree chunks
 
+// Raw buffer over `count` doubles (the gfx9 raw-buffer descriptor word 3).
+__device__ inline __amdgpu_buffer_rsrc_t table_rsrc(const double* p, size_t count) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0,
+                                             (int)(count * sizeof(double)), 0x00020000);
+}
+constexpr unsigned kOutOfRange = 0x80000000u;   // byte offset past every table block
+__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 struct Bve1Args {
     int L, nh, top;           // top: analysed degrees l <= top (2L/3)
     int msplit;               // columns m < msplit run as two latitude halves
@@ -96,12 +106,18 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     double pa[kB1Chunks][kB1Pre];
     if (active) {
         const double* P = a.Pa + boff;
+        // buffer loads over block m: lanes past the band get an offset past the
+        // buffer's range, which reads 0 without a memory request (no branch per
+        // load; clamping them onto one address instead serialises the wave)
+        const __amdgpu_buffer_rsrc_t rs = table_rsrc(P, (size_t)K * nh);
 #pragma unroll
         for (int c = 0; c < kB1Chunks; ++c)
 #pragma unroll
             for (int u = 0; u < kB1Pre; ++u) {
                 const int l = m + 64 * c + lane, jp = wave + kB1Waves * u;
-                pa[c][u] = (l <= top && jp < nh) ? P[jp * K + (l - m)] : 0.0;
+                const unsigned off = (l <= top && jp < nh) ? (unsigned)(jp * K + (l - m)) * 8u
+                                                           : kOutOfRange;
+                pa[c][u] = load_f64(rs, off);
             }
     }
     // synthesis seeds of this lane's first latitude pass
@@ -188,13 +204,20 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                             pv[u] = (jp < nh && l <= top) ? P[jp * K + (l - m)] : 0.0;
                         }
                     }
+                    // rows past nh have pv = 0 and read row nh - 1 (branch-free);
+                    // four LDS reads in flight per group
 #pragma unroll
-                    for (int u = 0; u < kB1Pre; ++u) {
-                        const int jp = j0 + kB1Waves * u;
-                        if (jp < nh) {
-                            const double2 x = X[par * nh + jp];
-                            acc.x = fma(pv[u], x.x, acc.x);
-                            acc.y = fma(pv[u], x.y, acc.y);
+                    for (int u0 = 0; u0 < kB1Pre; u0 += 4) {
+                        double2 x[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int jq = j0 + kB1Waves * (u0 + u);
+                            x[u] = X[par * nh + ((jq < nh) ? jq : nh - 1)];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            acc.x = fma(pv[u0 + u], x[u].x, acc.x);
+                            acc.y = fma(pv[u0 + u], x[u].y, acc.y);
                         }
                     }
                 }
@@ -306,19 +329,18 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                 accE[f] = c2(pe_mul(p2, ce.x), pe_mul(p2, ce.y));
                 accO[f] = c2(pe_mul(p1, co.x), pe_mul(p1, co.y));
             }
-            for (int i = i0 + 2; i < i1; i += 4) {
+            int i = i0 + 2;
+            for (; i + 4 <= i1; i += 4) {
                 double2 ab[4], cv[4][4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int ii = (i + u < i1) ? i + u : i1 - 1;
-                    ab[u] = rcc[m + ii];
+                    ab[u] = rcc[m + i + u];
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) cv[u][f] = cols[f * K + ii];
+                    for (int f = 0; f < 4; ++f) cv[u][f] = cols[f * K + i + u];
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    double p = rec_step(ab[u], x, p1, p2);
-                    p = (i + u < i1) ? p : 0.0;
+                    const double p = rec_step(ab[u], x, p1, p2);
                     p2 = p1;
                     p1 = p;
 #pragma unroll
@@ -327,6 +349,19 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                         acc.x = fma(p, cv[u][f].x, acc.x);
                         acc.y = fma(p, cv[u][f].y, acc.y);
                     }
+                }
+            }
+            for (; i < i1; ++i) {       // the block's last degrees (fewer than a group)
+                const double p = rec_step(rcc[m + i], x, p1, p2);
+                p2 = p1;
+                p1 = p;
+                const bool odd = (i - i0) & 1;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const double2 c = cols[f * K + i];
+                    double2& acc = odd ? accO[f] : accE[f];
+                    acc.x = fma(p, c.x, acc.x);
+                    acc.y = fma(p, c.y, acc.y);
                 }
             }
         }

@@ -34,7 +34,7 @@ __host__ __device__ inline int row_cap(int n) {
 //   Pbar_mm = -sqrt((2m+1)/(2m)) s Pbar_{m-1,m-1}      (Condon-Shortley phase)
 //   Pbar_{m+1,m} = sqrt(2m+3) x Pbar_mm
 //   Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m})
-// with a_lm, b_lm from the plan's recurrence table; rec_step is the exact
+// with (a_lm, a_lm b_lm) from the plan's recurrence table; rec_step is the exact
 // expression the synthesis kernels use to continue the recurrence from two
 // table rows, so both produce the same values.
 // ---------------------------------------------------------------------------
@@ -527,6 +527,145 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
                   (size_t)nmem * 4, threadIdx.x, kGridThreads);
 }
 
+// ---------------------------------------------------------------------------
+// Jacobian grid stage for power-of-two rows n = 2^LGN <= 512 with every index
+// a compile-time constant: the same arithmetic as grid_jac_kernel (the same
+// packed spectra, twiddles, butterflies, products and unpacking), with the
+// spectrum packing of the four inverse rows fused into the first radix-8
+// stage (each thread packs the eight bins it transforms straight from the
+// staged E/O columns, so the packed rows never pass through LDS).
+// ---------------------------------------------------------------------------
+template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS>
+__device__ inline void fft_stage_c(double2* buf, const double2* tw) {
+    constexpr int N = 1 << LGN, LGNB = LGN - LGR, NB = 1 << LGNB, TOT = NROWS * NB;
+    constexpr int RS = N + (N >> 3), LGSPAN = LGN - LGNS - LGR;
+    constexpr int MAXB = (TOT + kGridThreads - 1) / kGridThreads;
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < TOT) {
+            const int row = t >> LGNB, j = t & (NB - 1), k = j & ((1 << LGNS) - 1);
+            const double2* rb = buf + row * RS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = rb[lp(j + r * NB)];
+                if (LGNS > 0 && r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, (r * k) << LGSPAN));
+                v[b][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < TOT) {
+            const int row = t >> LGNB, j = t & (NB - 1), k = j & ((1 << LGNS) - 1);
+            butterfly<DIR, R>(v[b]);
+            double2* rb = buf + row * RS;
+            const int base = ((j >> LGNS) << (LGNS + LGR)) + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) rb[lp(base + (q << LGNS))] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Stages after the first: radix 8 while at least 3 bits remain, then 4 or 2.
+template <int DIR, int LGN, int LGNS, int NROWS>
+__device__ inline void fft_rest_c(double2* buf, const double2* tw) {
+    if constexpr (LGN - LGNS >= 3) {
+        fft_stage_c<DIR, 8, 3, LGN, LGNS, NROWS>(buf, tw);
+        fft_rest_c<DIR, LGN, LGNS + 3, NROWS>(buf, tw);
+    } else if constexpr (LGN - LGNS == 2) {
+        fft_stage_c<DIR, 4, 2, LGN, LGNS, NROWS>(buf, tw);
+    } else if constexpr (LGN - LGNS == 1) {
+        fft_stage_c<DIR, 2, 1, LGN, LGNS, NROWS>(buf, tw);
+    }
+}
+
+// Packed spectrum bin b of one row (pack_spectrum, case n > 2L).
+__device__ inline double2 packed_bin(int b, int n, int L, const double2* Er, const double2* Or,
+                                     int mode, double invR) {
+    const int m1 = (b <= L) ? b : L;
+    const int m2 = (b > 0 && n - b <= L) ? n - b : L;
+    const double2 e1 = Er[m1], o1 = Or[m1], e2 = Er[m2], o2 = Or[m2];
+    double2 gn = cadd(e1, o1), gs = csub(e1, o1);
+    double2 hn = cadd(e2, o2), hs = csub(e2, o2);
+    if (mode == kSpecDlam) {
+        gn = dlam_q(gn, m1 * invR); gs = dlam_q(gs, m1 * invR);
+        hn = dlam_q(hn, m2 * invR); hs = dlam_q(hs, m2 * invR);
+    } else if (mode == kSpecScale) {
+        gn = cscale(gn, invR); gs = cscale(gs, invR);
+        hn = cscale(hn, invR); hs = cscale(hs, invR);
+    }
+    if (b == 0) { gn.y = 0.0; gs.y = 0.0; }
+    double2 z = c2(0.0, 0.0);
+    if (b <= L) z = c2(gn.x - gs.y, gn.y + gs.x);
+    if (b > 0 && n - b <= L) z = cadd(z, c2(hn.x + hs.y, -hn.y + hs.x));
+    return z;
+}
+
+template <int LGN>
+__global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, const double2* E,
+                                                                  const double2* O, double R,
+                                                                  int mmax, int nmem, double2* S,
+                                                                  double2* D) {
+    constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
+    extern __shared__ double2 lds[];
+    int jp, b;
+    grid_wg_index(g.nh, nmem, jp, b);
+    const int L = g.L;
+    const int jN = jp, jS = g.nlat - 1 - jp;
+    const double invR = 1.0 / R;
+    const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
+    const size_t ms = (size_t)nmem * 4;
+    const double cN = clamp_coslat(g.coslat[jN]);
+    const double cS = clamp_coslat(g.coslat[jS]);
+    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
+    // LDS: rows [4][RS] | twiddles [N] | E, O columns [4][L+1] each
+    double2* sE = lds + 4 * RS + N;
+    double2* sO = sE + (L + 1) * 4;
+    const double2* twl = lds + 4 * RS;
+    grid_prologue<4>(g, E, O, eo, ms, lds + 4 * RS, sE, sO);
+    __syncthreads();
+    // inverse stage 1 (radix 8, no twiddles) fed by the packed bins:
+    // rows 0 a_l, 1 a_s, 2 b_l, 3 b_s
+    static_assert(4 * NB <= 2 * kGridThreads, "two first-stage butterflies per thread at most");
+#pragma unroll
+    for (int bt = 0; bt < 2; ++bt) {
+        const int t = threadIdx.x + bt * kGridThreads;
+        if (t < 4 * NB) {
+            const int row = t / NB, j = t % NB;
+            const int mode = (row & 1) ? kSpecScale : kSpecDlam;
+            double2 v[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                v[r] = packed_bin(j + r * NB, N, L, sE + row * (L + 1), sO + row * (L + 1), mode,
+                                  invR);
+            butterfly<+1, 8>(v);
+            double2* rb = lds + row * RS;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rb[lp(8 * j + q)] = v[q];
+        }
+    }
+    __syncthreads();
+    fft_rest_c<+1, LGN, 3, 4>(lds, twl);
+    const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
+    for (int k = threadIdx.x; k < N; k += kGridThreads) {
+        const int e = lp(k);
+        const double2 al = lds[e], as = lds[RS + e], bl = lds[2 * RS + e], bs = lds[3 * RS + e];
+        const double jn = jac_point_q(al.x, as.x, bl.x, bs.x, qN);
+        const double js = jac_point_q(al.y, as.y, bl.y, bs.y, qS);
+        lds[e] = c2(jn, js);
+    }
+    __syncthreads();
+    fft_stage_c<-1, 8, 3, LGN, 0, 1>(lds, twl);
+    fft_rest_c<-1, LGN, 3, 1>(lds, twl);
+    const size_t so = (size_t)jp * (L + 1) * nmem + b;
+    unpack_sd(lds, N, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
+}
+
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).
 #define AEO_GRID_PICK(name)                                                        \
     static decltype(&name<true>) name##_for(const aeo_plan_s* p) {                 \
@@ -577,6 +716,24 @@ static size_t grid_lds(const aeo_plan_s* p, int rows, int nf) {
 }
 
 static size_t jac_lds(const aeo_plan_s* p) { return grid_lds(p, 4, 4); }
+
+// The Jacobian grid kernel for this plan: the compile-time power-of-two
+// instance for rows of 64 to 512 points without aliasing (n > 2L), else the
+// general one.
+using JacKernel = void (*)(GaussDev, const double2*, const double2*, double, int, int, double2*,
+                           double2*);
+static JacKernel jac_kernel_for(const aeo_plan_s* p) {
+    if (fft_p2(p->fft) && p->nlon > 2 * p->L) {
+        switch (p->fft.lgn) {
+            case 6: return &grid_jac_p2_kernel<6>;
+            case 7: return &grid_jac_p2_kernel<7>;
+            case 8: return &grid_jac_p2_kernel<8>;
+            case 9: return &grid_jac_p2_kernel<9>;
+            default: break;
+        }
+    }
+    return grid_jac_kernel_for(p);
+}
 
 // Rows per workgroup of the plain transforms' grid kernels for a batch of nf
 // fields: enough workgroups (nh x batches >= 512) to fill the chip, at most
@@ -674,9 +831,9 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     int rc = launch_stage<KindJac, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
-    if ((rc = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p)))) return rc;
-    hipLaunchKernelGGL(grid_jac_kernel_for(p), dim3(p->nh), dim3(kGridThreads), jac_lds(p), s, p->gdev(), p->d_E, p->d_O, R, lim, 1,
-                       p->d_S, p->d_D);
+    if ((rc = allow_lds((const void*)jac_kernel_for(p), jac_lds(p)))) return rc;
+    hipLaunchKernelGGL(jac_kernel_for(p), dim3(p->nh), dim3(kGridThreads), jac_lds(p), s,
+                       p->gdev(), p->d_E, p->d_O, R, lim, 1, p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
     sb.nmem = 1;
@@ -732,10 +889,10 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
     if (md->kind == kBVE) {
-        int rc0 = allow_lds((const void*)grid_jac_kernel_for(p), jac_lds(p));
+        int rc0 = allow_lds((const void*)jac_kernel_for(p), jac_lds(p));
         if (rc0) return rc0;
         pr.arm(kClsGrid);
-        launch_k(grid_jac_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), jac_lds(p), s,
+        launch_k(jac_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), jac_lds(p), s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     } else {

@@ -34,10 +34,13 @@ struct Prefetch {
     static constexpr int PC = (MB == 1 && NA <= 2) ? 2 : 1;
 };
 
-// One step of the Pbar recurrence: a (x p1 - b p2) with (a, b) = ab.  The
-// table kernel and the synthesis kernels share this exact expression.
+// One step of the Pbar recurrence Pbar_l = a (x Pbar_{l-1} - b Pbar_{l-2}),
+// written as fma(a x, p1, -(a b) p2) with ab = (a, a b): the only operation
+// on the dependent path from p1 is one fma (the FP64 latency chain of a
+// synthesis is one fma per degree).  The table kernel and the synthesis
+// kernels share this exact expression.
 __device__ inline double rec_step(double2 ab, double x, double p1, double p2) {
-    return ab.x * fma(x, p1, -(ab.y * p2));
+    return fma(ab.x * x, p1, -(ab.y * p2));
 }
 
 // p * c as the first term of a multiply-add chain (fma(p, c, 0) rounds alike)
@@ -962,22 +965,23 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                                 }
                             }
                         }
+                        // rows past nh have pv = 0 and read row nh - 1 (branch-free,
+                        // so every LDS read of the pass is issued before the first fma)
 #pragma unroll
                         for (int u = 0; u < kPre; ++u) {
-                            const int jp = j0 + kStageWaves * u;
-                            if (jp < nh) {
+                            const int jq = j0 + kStageWaves * u;
+                            const int jp = (jq < nh) ? jq : nh - 1;
 #pragma unroll
-                                for (int bb = 0; bb < MB; ++bb)
+                            for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-                                    for (int f = 0; f < NA; ++f) {
-                                        const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
+                                for (int f = 0; f < NA; ++f) {
+                                    const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
 #pragma unroll
-                                        for (int h = 0; h < kCH; ++h) {
-                                            acc[h][bb][f].x = fma(pv[h][u], x.x, acc[h][bb][f].x);
-                                            acc[h][bb][f].y = fma(pv[h][u], x.y, acc[h][bb][f].y);
-                                        }
+                                    for (int h = 0; h < kCH; ++h) {
+                                        acc[h][bb][f].x = fma(pv[h][u], x.x, acc[h][bb][f].x);
+                                        acc[h][bb][f].y = fma(pv[h][u], x.y, acc[h][bb][f].y);
                                     }
-                            }
+                                }
                         }
                     }
                 }
@@ -1251,22 +1255,21 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 // groups of kRecG degrees (i0 + 2 is even, so degree parity =
                 // parity of u): every LDS read of a group is issued before its
                 // recurrence and multiply-adds; out-of-block degrees get p = 0
-                for (int i = i0 + 2; i < i1; i += kRecG) {
+                int i = i0 + 2;
+                for (; i + kRecG <= i1; i += kRecG) {
                     double2 ab[kRecG];
                     double2 cv[kRecG][MB][NF];
 #pragma unroll
                     for (int u = 0; u < kRecG; ++u) {
-                        const int ii = (i + u < i1) ? i + u : i1 - 1;
-                        ab[u] = rcc[m + ii];
+                        ab[u] = rcc[m + i + u];
 #pragma unroll
                         for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
-                            for (int f = 0; f < NF; ++f) cv[u][bb][f] = cols[(bb * NF + f) * K + ii];
+                            for (int f = 0; f < NF; ++f) cv[u][bb][f] = cols[(bb * NF + f) * K + i + u];
                     }
 #pragma unroll
                     for (int u = 0; u < kRecG; ++u) {
-                        double p = rec_step(ab[u], x, p1, p2);
-                        p = (i + u < i1) ? p : 0.0;
+                        const double p = rec_step(ab[u], x, p1, p2);
                         p2 = p1;
                         p1 = p;
 #pragma unroll
@@ -1278,6 +1281,21 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                                 acc.y = fma(p, cv[u][bb][f].y, acc.y);
                             }
                     }
+                }
+                for (; i < i1; ++i) {   // the block's last degrees (fewer than a group)
+                    const double p = rec_step(rcc[m + i], x, p1, p2);
+                    p2 = p1;
+                    p1 = p;
+                    const bool odd = (i - i0) & 1;
+#pragma unroll
+                    for (int bb = 0; bb < MB; ++bb)
+#pragma unroll
+                        for (int f = 0; f < NF; ++f) {
+                            const double2 c = cols[(bb * NF + f) * K + i];
+                            double2& acc = odd ? accO[bb][f] : accE[bb][f];
+                            acc.x = fma(p, c.x, acc.x);
+                            acc.y = fma(p, c.y, acc.y);
+                        }
                 }
             }
             if constexpr (kLaneBlocks) {
