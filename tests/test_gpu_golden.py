@@ -119,3 +119,43 @@ def test_ensemble_members_match_golden_days():
         err = rel_l2(_tri(y[0].cpu().numpy()), g["zT"])
         print(f"  rel-L2 vs oracle: {err:.3e}", flush=True)
         assert err < 1e-11
+
+
+def test_c4_full_ensemble_graph_split():
+    """The full C4 configuration: 64 members each at (L, grid) = (21, 32x64),
+    (42, 64x128), (85, 128x256), (170, 256x512), one simulated day at 300 s
+    through the graph-replayed ensemble step with 6-slice split contractions.
+    Member 0 of the L = 85 and L = 170 classes carries the golden seed and is
+    checked against the oracle's one-day trajectory; every member of every
+    class is checked against the FP64 ensemble (bar 1e-11 relative L2)."""
+    import torch
+    from aeolus_amd.run.ensemble import Ensemble
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print("case C4 full ensemble 4 x 64 members, one day, split6 vs fp64", flush=True)
+    classes = [(21, 32, 64), (42, 64, 128), (85, 128, 256), (170, 256, 512)]
+    golden = {85: _load("bve85"), 170: _load("bve170")}
+    planets = [_planet("latlon", L, a, b) for L, a, b in classes]
+    states = []
+    for L, _, _ in classes:
+        seeds = [1000 * L + k for k in range(64)]
+        if L in golden:
+            seeds[0] = int(golden[L]["seed"])
+        states.append(torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(sd),
+                                                             rms=5e-5) for sd in seeds]),
+                                      device="cuda"))
+    out = {}
+    for ns in (0, 6):
+        ens = Ensemble([(pl, 64) for pl in planets], precision=ns)
+        out[ns] = ens.rk4(states, 300.0, 288, use_graph=True)
+        del ens
+    for (L, _, _), a, b in zip(classes, out[6], out[0]):
+        errs = [rel_l2(a[k].cpu().numpy(), b[k].cpu().numpy()) for k in range(64)]
+        print(f"  L={L}: split6 vs fp64 worst member {max(errs):.3e}", flush=True)
+        assert max(errs) < 1e-11
+        assert bool(torch.isfinite(a).all())
+        if L in golden:
+            for ns in (0, 6):
+                y = out[ns][[c[0] for c in classes].index(L)]
+                err = rel_l2(_tri(y[0].cpu().numpy()), golden[L]["zT"])
+                print(f"  L={L} member 0 precision {ns} vs oracle: {err:.3e}", flush=True)
+                assert err < (1e-12 if ns == 0 else 1e-11)
