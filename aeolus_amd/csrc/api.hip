@@ -84,7 +84,8 @@ static void free_model(aeo_model_s* m) {
                     (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
                     (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum,
                     (void*)m->d_fint, (void*)m->d_aE, (void*)m->d_aO, (void*)m->d_aS,
-                    (void*)m->d_aD, (void*)m->d_asum, (void*)m->d_abuf, (void*)m->d_kbuf})
+                    (void*)m->d_aD, (void*)m->d_asum, (void*)m->d_abuf, (void*)m->d_kbuf,
+                    (void*)m->d_gT, (void*)m->d_gQ, (void*)m->d_gQe, (void*)m->d_gkoff})
         dev_free(q);
     delete m;
 }
@@ -366,9 +367,14 @@ int aeo_model_set_precision(aeo_model_t model, int nslices) {
 
 int aeo_model_set_tiles(aeo_model_t model, int members_per_block) {
     AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_set_tiles: model is NULL");
-    AEO_REQUIRE(members_per_block == 1 || members_per_block == 2 || members_per_block == 4 ||
-                    members_per_block == 8,
-                AEO_ERR_INVALID, "aeo_model_set_tiles: members_per_block must be 1, 2, 4 or 8");
+    AEO_REQUIRE(members_per_block == 0 || members_per_block == 1 || members_per_block == 2 ||
+                    members_per_block == 4 || members_per_block == 8,
+                AEO_ERR_INVALID,
+                "aeo_model_set_tiles: members_per_block must be 0 (GEMM form), 1, 2, 4 or 8");
+    if (members_per_block == 0) {
+        int rc = gemm_stage_setup(model);
+        if (rc) return rc;
+    }
     model->mb = members_per_block;
     return AEO_OK;
 }

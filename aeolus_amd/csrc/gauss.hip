@@ -863,6 +863,9 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     md->prof.arm(kClsSynthesis);
     if (bve1_supported(md))
         return bve1_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true, s);
+    if (md->mb == 0)
+        return gemm_stage_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true,
+                                 s);
     int rc = (md->kind == kBVE)
                  ? launch_stage<KindBve, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
                                                                  md->nslices)
@@ -923,6 +926,8 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     pr.arm(kClsAnalysis);
     if (md->kind == kBVE && bve1_supported(md)) {
         rc = bve1_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next, s);
+    } else if (md->mb == 0) {
+        rc = gemm_stage_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next, s);
     } else if (md->kind == kBVE) {
         EpiBveS ep{L, fs, forcing, md->d_nulap, rk};
         rc = stage_ana<KindBve>(md, sa, ep, syn_next, s);
@@ -939,6 +944,10 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
 // ---------------------------------------------------------------------------
 // BVE adjoint launchers
 // ---------------------------------------------------------------------------
+// Members per workgroup of the adjoint stages: the forward's fused tile, or
+// 4 when the forward runs in GEMM form (which has no adjoint counterpart).
+static int adjoint_mb(const aeo_model_s* md) { return md->mb ? md->mb : 4; }
+
 int gauss_adjoint_prime(aeo_model_s* md, const double2* yfwd, const double2* kb, hipStream_t s) {
     aeo_plan_s* p = md->plan;
     const size_t fs = (size_t)(p->L + 1) * (p->L + 1);
@@ -955,7 +964,7 @@ int gauss_adjoint_prime(aeo_model_s* md, const double2* yfwd, const double2* kb,
     sa.O = md->d_aO;
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     md->prof.arm(kClsSynthesis);
-    return launch_stage<KindBveAdj, EpiNoneS, false, true>(p, md->mb, sa, EpiNoneS{}, s,
+    return launch_stage<KindBveAdj, EpiNoneS, false, true>(p, adjoint_mb(md), sa, EpiNoneS{}, s,
                                                            md->nslices);
 }
 
@@ -994,9 +1003,10 @@ int gauss_adjoint_stage(aeo_model_s* md, const double2* kb, int stage, double dt
     EpiBveAdjS ep{L, md->R, fs, md->d_nulap, md->d_lap, p->d_CpT, p->d_CmT, fwd_next, rk};
     pr.arm(kClsAnalysis);
     if (fwd_next)
-        return launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, md->mb, sa, ep, s,
+        return launch_stage<KindBveAdj, EpiBveAdjS, true, true>(p, adjoint_mb(md), sa, ep, s,
                                                                 md->nslices);
-    return launch_stage<KindBveAdj, EpiBveAdjS, true, false>(p, md->mb, sa, ep, s, md->nslices);
+    return launch_stage<KindBveAdj, EpiBveAdjS, true, false>(p, adjoint_mb(md), sa, ep, s,
+                                                             md->nslices);
 }
 
 }  // namespace aeo

@@ -209,8 +209,15 @@ struct aeo_model_s {
     double2* d_abuf = nullptr;     // 2 slots: step-end / step-start cotangents
     double2* d_kbuf = nullptr;     // 2 slots: stage tendency cotangents
     int nmem = 1;                  // members (independent states) in the group
-    int mb = 1;                    // members per workgroup in the Legendre stage kernel
+    int mb = 1;                    // members per workgroup in the Legendre stage kernel;
+                                   // 0: the member-batched GEMM form (split_gemm.hip)
     int nslices = 0;               // 0: FP64 contractions; > 0: split INT8 MFMA digits
+    // GEMM-form stage workspaces (allocated when those tiles are selected)
+    double* d_gT = nullptr;        // analysed tendency terms [m][l][member] complex
+    signed char* d_gQ = nullptr;   // synthesis digit columns
+    int* d_gQe = nullptr;          // their exponents
+    int* d_gkoff = nullptr;        // per-m offsets of the digit blocks
+    int gemm_ncp = 0;              // synthesis columns per m (padded)
     ProfRec prof;
 };
 
@@ -240,6 +247,17 @@ bool bve1_supported(const aeo_model_s* md);
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
                 hipStream_t s);
+
+// Member-batched split-precision BVE stage in GEMM form (split_gemm.hip):
+// bitwise the results of the fused stage kernel at the same precision.
+// setup allocates its workspaces (once); launch runs the analysis (ana),
+// the epilogue and, with syn, the synthesis of the resulting (ana) or given
+// (!ana) state.
+bool gemm_stage_supported(const aeo_model_s* md);
+int gemm_stage_setup(aeo_model_s* md);
+int gemm_stage_launch(aeo_model_s* md, const double2* ys, const double2* y0,
+                      const double2* forcing, double2* kout, int stage, double dt,
+                      double2* ynext, bool ana, bool syn, hipStream_t s);
 
 // Adjoint of the BVE stage (Gauss plans).  prime: synthesis of the forward
 // fields of yfwd and of the tendency cotangent kb into the adjoint E/O

@@ -172,7 +172,12 @@ int aeo_model_destroy(aeo_model_t model);
 int aeo_model_set_precision(aeo_model_t model, int nslices);
 
 /* Members per workgroup of the Legendre stage kernels (1, 2, 4 or 8): the
- * tile shape the autotuner chooses.  No counterpart in the reference. */
+ * tile shape the autotuner chooses.  0 selects the member-batched GEMM form
+ * of the split-precision BVE stage (analysis, epilogue and synthesis kernels
+ * whose MFMA tiles span the member batch; bitwise the fused kernel's
+ * results): BVE models on Gauss plans only, it allocates its workspaces
+ * here, and steps fail with a state error while the precision is FP64.
+ * No counterpart in the reference. */
 int aeo_model_set_tiles(aeo_model_t model, int members_per_block);
 
 /* Tendency of one state (BVE: one field; SWE: the (3, L+1, L+1) stack zeta,

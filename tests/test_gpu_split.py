@@ -71,6 +71,55 @@ def test_split_batched_members_match_single():
             assert torch.equal(out[k], ref), (mb, k)
 
 
+@pytest.mark.parametrize("L,nlat,nlon,nmem,ns", [(42, 64, 128, 3, 7), (85, 128, 256, 40, 6),
+                                                 (170, 256, 512, 20, 5)])
+def test_split_gemm_form_matches_fused_bitwise(L, nlat, nlon, nmem, ns):
+    """The member-batched GEMM form (tiles 0: analysis, epilogue and synthesis
+    kernels spanning the batch) against the fused stage kernel at the same
+    slice count: the same digits, exact INT32 sums and FP64 recombination, so
+    RK4 steps and tendencies are bitwise equal (ragged member groups
+    included: 3, 40 and 20 members)."""
+    import torch
+    from aeolus_amd.physics import BarotropicState
+    from aeolus_amd.physics.barotropic import BarotropicBatch
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    print(f"case split gemm form L={L} nmem={nmem} ns={ns}", flush=True)
+    pl = _planet(L, nlat, nlon)
+    zs = torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(50 + k), rms=5e-5)
+                                   for k in range(nmem)]), device="cuda")
+    fused = BarotropicBatch(pl, nmem, viscosity=1e5)
+    fused.set_precision(ns)
+    fused.set_tiles(4)
+    gemm = BarotropicBatch(pl, nmem, viscosity=1e5)
+    gemm.set_precision(ns)
+    gemm.set_tiles(0)
+    a = fused.rk4(zs, 600.0, 3)
+    b = gemm.rk4(zs, 600.0, 3)
+    assert torch.equal(a, b)
+    ka = fused.tendency(BarotropicState(zs))
+    kb = gemm.tendency(BarotropicState(zs))
+    assert torch.equal(ka, kb)
+    # and the FP64 path within the slice count's accuracy
+    gemm.set_precision(0)
+    with pytest.raises(RuntimeError, match="split precision"):
+        gemm.rk4(zs, 600.0, 1)
+    gemm.set_precision(ns)
+    ref = BarotropicBatch(pl, nmem, viscosity=1e5)
+    r = ref.rk4(zs, 600.0, 3)
+    err = rel_l2(b.cpu().numpy(), r.cpu().numpy())
+    print(f"  gemm form vs fp64 rel-L2 {err:.3e}", flush=True)
+    assert err < {5: 1e-9, 6: 5e-12, 7: 1e-13}[ns]
+
+
+def test_split_gemm_form_refused_where_unsupported():
+    from aeolus_amd.physics import ShallowWaterModel
+    print("case split gemm form refusals", flush=True)
+    pl = _planet(21, 32, 64)
+    swe = ShallowWaterModel(pl, gravity=9.80616, mean_depth=1e4)
+    with pytest.raises(ValueError, match="GEMM"):
+        swe.set_tiles(0)
+
+
 def test_split_swe_tendency_matches_fp64():
     import torch
     from aeolus_amd.physics import ShallowWaterModel
