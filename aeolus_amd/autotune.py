@@ -1,10 +1,15 @@
 """Tile-shape autotuner for the Legendre stage kernels.
 
 For a model (a member batch on one Gauss plan) it times a few fused RK4
-steps for each candidate number of members per workgroup, keeps the fastest
-and records the choice in a JSON cache keyed by device, model kind, L, grid,
-member count and precision mode (``$AEOLUS_AMD_CACHE`` or
+steps for each candidate tile shape, keeps the fastest and records the
+choice in a JSON cache keyed by device, model kind, L, grid, member count
+and precision mode (``$AEOLUS_AMD_CACHE`` or
 ``~/.cache/aeolus_amd/autotune.json``), so later runs reuse it.
+
+Candidates: the fused stage kernel with 1, 2, 4 or 8 members per workgroup,
+and, for split-precision BVE batches, the GEMM form (``members_per_block``
+0) with 64 or 128 synthesis columns per workgroup and 4 or 8 members per
+epilogue workgroup.  Every candidate gives bitwise the same result.
 """
 from __future__ import annotations
 
@@ -17,6 +22,7 @@ import torch
 __all__ = ["tune_members_per_block", "cache_path"]
 
 CANDIDATES = (1, 2, 4, 8)
+GEMM_CANDIDATES = ((64, 8), (64, 4), (128, 8), (128, 4))   # (syn_cols, epi_members)
 
 
 def cache_path() -> Path:
@@ -53,39 +59,54 @@ def _store(table: dict) -> None:
         pass  # a read-only home only loses the cache, not the tuning
 
 
+def _label(mb, gemm) -> str:
+    return f"mb{mb}" if gemm is None else f"gemm{gemm[0]}x{gemm[1]}"
+
+
+def _apply(model, mb, gemm) -> None:
+    model.set_tiles(mb)
+    if gemm is not None:
+        model.set_gemm_tiles(*gemm)
+
+
 def tune_members_per_block(model, y: torch.Tensor, dt: float, steps: int = 3,
-                           candidates=CANDIDATES, use_cache: bool = True) -> dict:
-    """Pick the fastest members-per-workgroup tile for ``model``; returns the
-    timings (ms per step) and the choice."""
+                           candidates=CANDIDATES, use_cache: bool = True,
+                           gemm_candidates=GEMM_CANDIDATES) -> dict:
+    """Pick the fastest tile shape for ``model``; returns the timings (ms per
+    step, by label ``mb<k>`` or ``gemm<cols>x<members>``) and the choice
+    (``mb``, and ``gemm`` = [syn_cols, epi_members] when ``mb`` is 0)."""
     key = _key(model)
     table = _load() if use_cache else {}
     if use_cache and key in table:
-        model.set_tiles(int(table[key]["mb"]))
-        return dict(table[key], cached=True)
+        e = table[key]
+        _apply(model, int(e["mb"]), tuple(e["gemm"]) if e.get("gemm") else None)
+        return dict(e, cached=True)
     nmem = getattr(model, "nmembers", 1)
+    cands = [(mb, None) for mb in candidates if mb <= max(1, nmem)]
+    if getattr(model, "nslices", 0) and hasattr(model, "set_gemm_tiles"):
+        cands += [(0, g) for g in gemm_candidates]
     out = torch.empty_like(y)
     times = {}
-    for mb in candidates:
-        if mb > max(1, nmem):
-            continue
+    for mb, gemm in cands:
         try:
-            model.set_tiles(mb)
+            _apply(model, mb, gemm)
             model.rk4(y, dt, 1, out=out)
             torch.cuda.synchronize()
         except (ValueError, RuntimeError):
-            continue  # tile does not fit (LDS) on this size
+            continue  # tile does not fit (LDS) or does not apply to this model
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
         model.rk4(y, dt, steps, out=out)
         t1.record()
         torch.cuda.synchronize()
-        times[mb] = t0.elapsed_time(t1) / steps
+        times[(mb, gemm)] = t0.elapsed_time(t1) / steps
     if not times:
         raise RuntimeError("autotune: no candidate tile runs on this model")
-    best = min(times, key=times.get)
-    model.set_tiles(best)
-    entry = {"mb": best, "ms_per_step": {str(k): v for k, v in times.items()}}
+    best_mb, best_gemm = min(times, key=times.get)
+    _apply(model, best_mb, best_gemm)
+    entry = {"mb": best_mb, "gemm": list(best_gemm) if best_gemm else None,
+             "ms_per_step": {_label(*k): v for k, v in times.items()}}
     if use_cache:
         table[key] = entry
         _store(table)

@@ -40,13 +40,15 @@ static int group_step(aeo_model_s* m, const double2* cur, double2* out, double d
     return AEO_OK;
 }
 
-// What a captured graph depends on beyond dt: the tiles and precision of
-// every model and the generation of its plan's split tables (rebuilding them
-// moves them, and the graph holds their addresses).
+// What a captured graph depends on beyond dt: the tiles (fused or GEMM form)
+// and precision of every model and the generation of its plan's split tables
+// (rebuilding them moves them, and the graph holds their addresses).
 static std::vector<long long> graph_key(const aeo_ensemble_s* e) {
     std::vector<long long> k;
     for (const aeo_model_s* m : e->models) {
         k.push_back(m->mb);
+        k.push_back(m->gemm_cg);
+        k.push_back(m->gemm_me);
         k.push_back(m->nslices);
         k.push_back(m->plan->tables_gen);
     }

@@ -218,6 +218,8 @@ struct aeo_model_s {
     int* d_gQe = nullptr;          // their exponents
     int* d_gkoff = nullptr;        // per-m offsets of the digit blocks
     int gemm_ncp = 0;              // synthesis columns per m (padded)
+    int gemm_cg = 64;              // synthesis columns per workgroup (64 or 128)
+    int gemm_me = 8;               // members per epilogue workgroup (4 or 8)
     ProfRec prof;
 };
 
@@ -255,6 +257,7 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
 // (!ana) state.
 bool gemm_stage_supported(const aeo_model_s* md);
 int gemm_stage_setup(aeo_model_s* md);
+int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members);
 int gemm_stage_launch(aeo_model_s* md, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
                       double2* ynext, bool ana, bool syn, hipStream_t s);

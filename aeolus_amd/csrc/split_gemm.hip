@@ -11,7 +11,7 @@
 //                    column over the latitudes, INT8 MFMA against the
 //                    analysis digit table of m, FP64 recombination
 //                    -> T[m][l][member]
-//   gemm_epi_kernel  (m, 8 members): tendency and RK4 stage bookkeeping from
+//   gemm_epi_kernel  (m, 4 or 8 members): tendency and RK4 stage bookkeeping from
 //                    T, then the synthesis fields psi, S psi, eta, S eta of
 //                    the new stage state with one exponent per column over
 //                    the degrees, written as digits Qd[m][parity][slice][col][k]
@@ -31,7 +31,6 @@ namespace aeo {
 constexpr int kGemmThreads = 512;
 constexpr int kGemmWaves = kGemmThreads / 64;
 constexpr int kAnaMembers = 32;      // members per analysis workgroup: 64 columns
-constexpr int kEpiMembers = 8;       // members per epilogue workgroup: 64 synthesis columns
 constexpr int kSynColAlign = 128;    // synthesis columns per m are padded to this
 
 struct GemmArgs {
@@ -207,18 +206,18 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_ana_kernel(GaussDev g, Gemm
 }
 
 // ---------------------------------------------------------------------------
-// Epilogue + synthesis digits for 8 members of column m.
+// Epilogue + synthesis digits for ME members of column m.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline size_t gemm_epi_lds(int L) {
+__host__ __device__ inline size_t gemm_epi_lds(int L, int me) {
     const size_t n = L + 1;
-    // state, psi [8][n] complex; C+, C-, lap [n]; exponents [64]; the
+    // state, psi [ME][n] complex; C+, C-, lap [n]; exponents [8 ME]; the
     // column-maximum scratch of col_max_exp needs none
-    return sizeof(double2) * 2 * kEpiMembers * n + sizeof(double) * (3 * n + (n & 1)) + 256;
+    return sizeof(double2) * 2 * me * n + sizeof(double) * (3 * n + (n & 1)) + 256;
 }
 
-template <int NS>
+template <int NS, int ME>
 __global__ __launch_bounds__(kGemmThreads) void gemm_epi_kernel(GemmArgs a) {
-    constexpr int ME = kEpiMembers, NF = 4, NCE = ME * NF * 2;
+    constexpr int NF = 4, NCE = ME * NF * 2;
     extern __shared__ __align__(16) unsigned char gsm[];
     const int m = blockIdx.x, b0 = blockIdx.y * ME;
     const int L = a.L, n = L + 1, K = L - m + 1;
@@ -326,7 +325,7 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_epi_kernel(GemmArgs a) {
 // ---------------------------------------------------------------------------
 template <int NS, int CG>
 __host__ __device__ inline size_t gemm_syn_lds(int L) {
-    return 256 + (size_t)NS * CG * (split_kpad(L, 0) + 16);
+    return sizeof(int) * CG + (size_t)NS * CG * (split_kpad(L, 0) + 16);
 }
 
 template <int NS, int CG>
@@ -337,8 +336,8 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
     const int L = g.L, nh = g.nh, n = L + 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kpad = split_kpad(L, m), KS = kpad + 16, nhp = rup(nh, 16);
-    int* qe = reinterpret_cast<int*>(gsm);                            // [CG]
-    signed char* qs = reinterpret_cast<signed char*>(gsm + 256);      // [NS][CG][KS]
+    int* qe = reinterpret_cast<int*>(gsm);                                   // [CG]
+    signed char* qs = reinterpret_cast<signed char*>(gsm + sizeof(int) * CG);  // [NS][CG][KS]
     const signed char* src = a.Qd + 2 * (size_t)NS * a.ncp * a.kpoff[m];
     const int nib = kpad / 16;
     for (int t = threadIdx.x; t < NS * CG * nib; t += kGemmThreads) {
@@ -429,14 +428,12 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-constexpr int kSynCols = 64;         // synthesis columns per workgroup
-
 static int gemm_ncp(int nmem) { return rup(nmem * 8, kSynColAlign); }
 
 bool gemm_stage_supported(const aeo_model_s* md) {
     const aeo_plan_s* p = md->plan;
     return md->kind == kBVE && p->kind == kGauss && split_nhk(p->nh) <= 256 &&
-           gemm_epi_lds(p->L) <= 163840 && gemm_syn_lds<7, kSynCols>(p->L) <= 163840 &&
+           gemm_epi_lds(p->L, 8) <= 163840 && gemm_syn_lds<7, 128>(p->L) <= 163840 &&
            gemm_ana_lds<7>(p->nh) <= 163840;
 }
 
@@ -489,11 +486,32 @@ static int allow_gemm_lds(K kern, size_t bytes) {
     return AEO_OK;
 }
 
+template <int NS, int ME>
+static int gemm_epi_launch(const aeo_plan_s* p, const GemmArgs& a, hipStream_t s) {
+    const size_t lds = gemm_epi_lds(p->L, ME);
+    int rc;
+    if ((rc = allow_gemm_lds(gemm_epi_kernel<NS, ME>, lds))) return rc;
+    launch_k(gemm_epi_kernel<NS, ME>, dim3(p->L + 1, (a.nmem + ME - 1) / ME), dim3(kGemmThreads),
+             lds, s, a);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
+template <int NS, int CG>
+static int gemm_syn_launch(const aeo_plan_s* p, const GemmArgs& a, hipStream_t s) {
+    const size_t lds = gemm_syn_lds<NS, CG>(p->L);
+    int rc;
+    if ((rc = allow_gemm_lds(gemm_syn_kernel<NS, CG>, lds))) return rc;
+    launch_k(gemm_syn_kernel<NS, CG>, dim3(p->L + 1, 2, a.ncp / CG), dim3(kGemmThreads), lds, s,
+             p->gdev(), a);
+    AEO_LAUNCH_CHECK();
+    return AEO_OK;
+}
+
 template <int NS>
 static int gemm_stage_ns(aeo_model_s* md, const GemmArgs& a, hipStream_t s) {
     const aeo_plan_s* p = md->plan;
     const GaussDev g = p->gdev();
-    const int n = p->L + 1;
     int rc;
     if (a.ana && a.mmax >= 0) {
         const size_t lds = gemm_ana_lds<NS>(p->nh);
@@ -502,20 +520,19 @@ static int gemm_stage_ns(aeo_model_s* md, const GemmArgs& a, hipStream_t s) {
                  dim3(kGemmThreads), lds, s, g, a);
         AEO_LAUNCH_CHECK();
     }
-    {
-        const size_t lds = gemm_epi_lds(p->L);
-        if ((rc = allow_gemm_lds(gemm_epi_kernel<NS>, lds))) return rc;
-        launch_k(gemm_epi_kernel<NS>, dim3(n, (a.nmem + kEpiMembers - 1) / kEpiMembers),
-                 dim3(kGemmThreads), lds, s, a);
-        AEO_LAUNCH_CHECK();
-    }
-    if (a.syn) {
-        const size_t lds = gemm_syn_lds<NS, kSynCols>(p->L);
-        if ((rc = allow_gemm_lds(gemm_syn_kernel<NS, kSynCols>, lds))) return rc;
-        launch_k(gemm_syn_kernel<NS, kSynCols>, dim3(n, 2, a.ncp / kSynCols), dim3(kGemmThreads),
-                 lds, s, g, a);
-        AEO_LAUNCH_CHECK();
-    }
+    rc = (md->gemm_me == 4) ? gemm_epi_launch<NS, 4>(p, a, s) : gemm_epi_launch<NS, 8>(p, a, s);
+    if (rc || !a.syn) return rc;
+    return (md->gemm_cg == 128) ? gemm_syn_launch<NS, 128>(p, a, s)
+                                : gemm_syn_launch<NS, 64>(p, a, s);
+}
+
+int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members) {
+    AEO_REQUIRE(syn_cols == 64 || syn_cols == 128, AEO_ERR_INVALID,
+                "aeo_model_set_gemm_tiles: syn_cols must be 64 or 128");
+    AEO_REQUIRE(epi_members == 4 || epi_members == 8, AEO_ERR_INVALID,
+                "aeo_model_set_gemm_tiles: epi_members must be 4 or 8");
+    md->gemm_cg = syn_cols;
+    md->gemm_me = epi_members;
     return AEO_OK;
 }
 

@@ -84,7 +84,15 @@ class BarotropicVorticity:
         self.nslices = k
 
     def set_tiles(self, members_per_block: int) -> None:
+        """Members per workgroup of the fused stage kernel (1, 2, 4, 8), or 0 for
+        the member-batched GEMM form of the split-precision stage."""
         nat.check(nat.lib().aeo_model_set_tiles(self._model, int(members_per_block)), "set_tiles")
+
+    def set_gemm_tiles(self, syn_cols: int = 64, epi_members: int = 8) -> None:
+        """Tile shape of the GEMM form: synthesis columns per workgroup (64,
+        128) and members per epilogue workgroup (4, 8)."""
+        nat.check(nat.lib().aeo_model_set_gemm_tiles(self._model, int(syn_cols),
+                                                     int(epi_members)), "set_gemm_tiles")
 
     def vorticity_to_streamfunction(self, vrt_state: BarotropicState) -> torch.Tensor:
         assert isinstance(vrt_state, BarotropicState)

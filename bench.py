@@ -73,8 +73,8 @@ def _args():
                    help="only the 256-member uneven-L ensemble, graph replay (for profiler "
                         "runs; precision from --precision, default 6 slices)")
     p.add_argument("--tiles", type=str, default=None,
-                   help="members per workgroup for the L = 21,42,85,170 classes, e.g. 8,8,8,4 "
-                        "(default: autotune)")
+                   help="tiles of the L = 21,42,85,170 classes, fused members per workgroup "
+                        "or the GEMM form, e.g. mb8,mb8,gemm128x4,gemm128x4 (default: autotune)")
     return p.parse_args()
 
 
@@ -274,9 +274,24 @@ def ensemble_modes(steps, ns_split):
                 for (L, _, _), d, r in zip(classes, day, ref)}
         res[ns] = {"ms_per_ensemble_step": ms, "member_steps_per_s": 4 * nper * 1000.0 / ms,
                    "rel_l2_vs_fp64_1day": errs,
-                   "members_per_block": {f"L{L}": t["mb"] for (L, _, _), t in zip(classes, tuning)}}
+                   "tiles": {f"L{L}": _tile_label(t) for (L, _, _), t in zip(classes, tuning)}}
         del ens
     return res
+
+
+def _tile_label(t) -> str:
+    """Autotuner choice as a label: mb<k> (fused, k members per workgroup) or
+    gemm<cols>x<members> (GEMM form)."""
+    return f"mb{t['mb']}" if t["mb"] else f"gemm{t['gemm'][0]}x{t['gemm'][1]}"
+
+
+def _apply_tile(model, label: str) -> None:
+    if label.startswith("gemm"):
+        cols, mem = (int(v) for v in label[4:].split("x"))
+        model.set_tiles(0)
+        model.set_gemm_tiles(cols, mem)
+    else:
+        model.set_tiles(int(label[2:] if label.startswith("mb") else label))
 
 
 def ensemble_only(args):
@@ -294,16 +309,16 @@ def ensemble_only(args):
                               device="cuda") for L, _, _ in classes]
     ens = Ensemble([(pl, nper) for pl in planets], precision=ns)
     if args.tiles:
-        tiles = [int(t) for t in args.tiles.split(",")]
+        tiles = args.tiles.split(",")
         for m, t in zip(ens.models, tiles):
-            m.set_tiles(t)
+            _apply_tile(m, t)
     else:
-        tiles = [t["mb"] for t in ens.autotune(states, DT, steps=2)]
+        tiles = [_tile_label(t) for t in ens.autotune(states, DT, steps=2)]
     outs = [torch.empty_like(s) for s in states]
     ms = _timed(lambda k: ens.rk4(states, DT, k, out=outs, use_graph=True), args.steps,
                 args.warmup)
     print(json.dumps({"workload": "BVE ensemble, 64 members each at L = 21/42/85/170",
-                      "precision": _mode_name(ns), "members_per_block": tiles,
+                      "precision": _mode_name(ns), "tiles": tiles,
                       "steps": args.steps, "ms_per_ensemble_step": ms,
                       "member_steps_per_s": 4 * nper * 1000.0 / ms}))
 

@@ -180,6 +180,12 @@ int aeo_model_set_precision(aeo_model_t model, int nslices);
  * No counterpart in the reference. */
 int aeo_model_set_tiles(aeo_model_t model, int members_per_block);
 
+/* Tile shape of the GEMM form (members_per_block 0 above): synthesis
+ * columns (member x field x re/im) per workgroup, 64 or 128, and members per
+ * epilogue workgroup, 4 or 8 (defaults 64 and 8).  Results do not depend on
+ * it.  No counterpart in the reference. */
+int aeo_model_set_gemm_tiles(aeo_model_t model, int syn_cols, int epi_members);
+
 /* Tendency of one state (BVE: one field; SWE: the (3, L+1, L+1) stack zeta,
  * delta, phi).  forcing: device, same shape, or NULL (BVE only).
  * Replaces BarotropicVorticity.tendency (phy/barotropic.py:120-167) and

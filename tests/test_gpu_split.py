@@ -96,6 +96,10 @@ def test_split_gemm_form_matches_fused_bitwise(L, nlat, nlon, nmem, ns):
     a = fused.rk4(zs, 600.0, 3)
     b = gemm.rk4(zs, 600.0, 3)
     assert torch.equal(a, b)
+    for tiles in ((128, 4), (64, 4), (128, 8)):
+        gemm.set_gemm_tiles(*tiles)
+        assert torch.equal(gemm.rk4(zs, 600.0, 3), a), tiles
+    gemm.set_gemm_tiles(64, 8)
     ka = fused.tendency(BarotropicState(zs))
     kb = gemm.tendency(BarotropicState(zs))
     assert torch.equal(ka, kb)
@@ -118,6 +122,12 @@ def test_split_gemm_form_refused_where_unsupported():
     swe = ShallowWaterModel(pl, gravity=9.80616, mean_depth=1e4)
     with pytest.raises(ValueError, match="GEMM"):
         swe.set_tiles(0)
+    from aeolus_amd.physics.barotropic import BarotropicBatch
+    b = BarotropicBatch(pl, 2)
+    with pytest.raises(ValueError, match="syn_cols"):
+        b.set_gemm_tiles(96, 8)
+    with pytest.raises(ValueError, match="epi_members"):
+        b.set_gemm_tiles(64, 2)
 
 
 def test_split_swe_tendency_matches_fp64():
