@@ -180,10 +180,24 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
     const int L = l_max, nh = p->nh;
     std::vector<long long> offs(L + 2, 0);
     for (int m = 0; m <= L; ++m) offs[m + 1] = offs[m] + (long long)nh * (L - m + 1);
+    // tw[t] = e^{-2 pi i t / nlon}; for nlon divisible by 4 the first quarter is
+    // evaluated and the rest are its exact quarter-turn rotations, so kernels
+    // that keep only the quarter table (twiddle_q) get bitwise the same values
     std::vector<double2> tw(nlon);
-    for (int t = 0; t < nlon; ++t) {
+    const int q4 = (nlon % 4 == 0) ? nlon / 4 : nlon;
+    for (int t = 0; t < q4; ++t) {
         const double ang = 2.0 * M_PI * (double)t / (double)nlon;
         tw[t] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    for (int t = q4; t < nlon; ++t) {
+        const double2 w = tw[t % q4];
+        const int q = t / q4;
+        double x = (q & 1) ? w.y : w.x, y = (q & 1) ? -w.x : w.y;
+        if (q & 2) {
+            x = -x;
+            y = -y;
+        }
+        tw[t] = make_double2(x, y);
     }
     std::vector<double> xn(x_nodes, x_nodes + nh);
     // Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}), stored m-major as

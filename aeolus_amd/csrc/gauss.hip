@@ -231,11 +231,12 @@ __device__ inline void grid_wg_index(int nh, int nmem, int& jp, int& b) {
 template <int NF>
 __device__ inline void grid_prologue(const GaussDev& g, const double2* __restrict__ E,
                                        const double2* __restrict__ O, size_t eo, size_t ms,
-                                       double2* twl, double2* sE, double2* sO) {
+                                       double2* twl, double2* sE, double2* sO, int ntw = -1) {
     // loads are unconditional (indices clamped in range) so the values stay
-    // in registers; only the LDS stores are predicated
+    // in registers; only the LDS stores are predicated.  ntw: twiddle entries
+    // staged (the first ntw of the table; default the whole table)
     constexpr int PER = 4;
-    const int n = g.nlon;
+    const int n = (ntw < 0) ? g.nlon : ntw;
     const int tot = (g.L + 1) * NF;
     for (int t0 = 0; t0 < tot || t0 < n; t0 += PER * kGridThreads) {
         double2 ve[PER], vo[PER], vt[PER];
@@ -535,7 +536,29 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
 // stage (each thread packs the eight bins it transforms straight from the
 // staged E/O columns, so the packed rows never pass through LDS).
 // ---------------------------------------------------------------------------
-template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS>
+// Twiddle W^t = e^{DIR 2 pi i t / N}, t in [0, N), from the quarter table
+// tw4[r] = e^{-2 pi i r / N}, r < N/4: W^(r + q N/4) is tw4[r] rotated by q
+// quarter turns (exact: a swap for odd q, a sign flip for q >= 2),
+// conjugated for DIR = +1.  (Written with a four-way conditional choosing
+// among double2 values instead, this gave wrong twiddles inside the grid
+// kernel on gfx950 while the same function alone was right.)
+template <int DIR, int LGN>
+__device__ inline double2 twiddle_q(const double2* tw4, int t) {
+    constexpr int Q = 1 << (LGN - 2);
+    const double2 w = tw4[t & (Q - 1)];
+    const int q = t >> (LGN - 2);
+    double x = (q & 1) ? w.y : w.x;
+    double y = (q & 1) ? -w.x : w.y;
+    if (q & 2) {
+        x = -x;
+        y = -y;
+    }
+    if (DIR > 0) y = -y;
+    return c2(x, y);
+}
+
+// QT: tw is the quarter table of twiddle_q, else the whole table.
+template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS, bool QT>
 __device__ inline void fft_stage_c(double2* buf, const double2* tw) {
     constexpr int N = 1 << LGN, LGNB = LGN - LGR, NB = 1 << LGNB, TOT = NROWS * NB;
     constexpr int RS = N + (N >> 3), LGSPAN = LGN - LGNS - LGR;
@@ -550,7 +573,13 @@ __device__ inline void fft_stage_c(double2* buf, const double2* tw) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = rb[lp(j + r * NB)];
-                if (LGNS > 0 && r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, (r * k) << LGSPAN));
+                if (LGNS > 0 && r > 0 && k > 0) {
+                    // explicit fma: both table forms give the same twiddle bits,
+                    // and the product must round alike in both instances
+                    const int e = (r * k) << LGSPAN;
+                    const double2 w = QT ? twiddle_q<DIR, LGN>(tw, e) : twiddle<DIR>(tw, e);
+                    x = c2(fma(x.x, w.x, -(x.y * w.y)), fma(x.x, w.y, x.y * w.x));
+                }
                 v[b][r] = x;
             }
         }
@@ -572,15 +601,15 @@ __device__ inline void fft_stage_c(double2* buf, const double2* tw) {
 }
 
 // Stages after the first: radix 8 while at least 3 bits remain, then 4 or 2.
-template <int DIR, int LGN, int LGNS, int NROWS>
+template <int DIR, int LGN, int LGNS, int NROWS, bool QT>
 __device__ inline void fft_rest_c(double2* buf, const double2* tw) {
     if constexpr (LGN - LGNS >= 3) {
-        fft_stage_c<DIR, 8, 3, LGN, LGNS, NROWS>(buf, tw);
-        fft_rest_c<DIR, LGN, LGNS + 3, NROWS>(buf, tw);
+        fft_stage_c<DIR, 8, 3, LGN, LGNS, NROWS, QT>(buf, tw);
+        fft_rest_c<DIR, LGN, LGNS + 3, NROWS, QT>(buf, tw);
     } else if constexpr (LGN - LGNS == 2) {
-        fft_stage_c<DIR, 4, 2, LGN, LGNS, NROWS>(buf, tw);
+        fft_stage_c<DIR, 4, 2, LGN, LGNS, NROWS, QT>(buf, tw);
     } else if constexpr (LGN - LGNS == 1) {
-        fft_stage_c<DIR, 2, 1, LGN, LGNS, NROWS>(buf, tw);
+        fft_stage_c<DIR, 2, 1, LGN, LGNS, NROWS, QT>(buf, tw);
     }
 }
 
@@ -606,7 +635,7 @@ __device__ inline double2 packed_bin(int b, int n, int L, const double2* Er, con
     return z;
 }
 
-template <int LGN>
+template <int LGN, bool COMPACT>
 __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, const double2* E,
                                                                   const double2* O, double R,
                                                                   int mmax, int nmem, double2* S,
@@ -623,34 +652,46 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     const double cN = clamp_coslat(g.coslat[jN]);
     const double cS = clamp_coslat(g.coslat[jS]);
     const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
-    // LDS: rows [4][RS] | twiddles [N] | E, O columns [4][L+1] each
-    double2* sE = lds + 4 * RS + N;
+    // LDS: rows [4][RS] | twiddle table [N] | E, O columns [4][L+1] each.
+    // COMPACT (member batches: more workgroups per CU): a quarter twiddle
+    // table [N/4], and the E/O columns (8 (L+1) <= 4 RS as N > 2L) alias the
+    // rows, which the first stage writes only after reading all of them
+    double2* twl = lds + 4 * RS;
+    double2* sE = COMPACT ? lds : twl + N;
     double2* sO = sE + (L + 1) * 4;
-    const double2* twl = lds + 4 * RS;
-    grid_prologue<4>(g, E, O, eo, ms, lds + 4 * RS, sE, sO);
+    grid_prologue<4>(g, E, O, eo, ms, twl, sE, sO, COMPACT ? N / 4 : N);
     __syncthreads();
     // inverse stage 1 (radix 8, no twiddles) fed by the packed bins:
     // rows 0 a_l, 1 a_s, 2 b_l, 3 b_s
-    static_assert(4 * NB <= 2 * kGridThreads, "two first-stage butterflies per thread at most");
+    constexpr int NBT = (4 * NB + kGridThreads - 1) / kGridThreads;
+    static_assert(NBT <= 2, "two first-stage butterflies per thread at most");
+    double2 v[NBT][8];
 #pragma unroll
-    for (int bt = 0; bt < 2; ++bt) {
+    for (int bt = 0; bt < NBT; ++bt) {
         const int t = threadIdx.x + bt * kGridThreads;
         if (t < 4 * NB) {
             const int row = t / NB, j = t % NB;
             const int mode = (row & 1) ? kSpecScale : kSpecDlam;
-            double2 v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r)
-                v[r] = packed_bin(j + r * NB, N, L, sE + row * (L + 1), sO + row * (L + 1), mode,
-                                  invR);
-            butterfly<+1, 8>(v);
+                v[bt][r] = packed_bin(j + r * NB, N, L, sE + row * (L + 1), sO + row * (L + 1),
+                                      mode, invR);
+            butterfly<+1, 8>(v[bt]);
+        }
+    }
+    if constexpr (COMPACT) __syncthreads();   // every E/O read before the rows are written
+#pragma unroll
+    for (int bt = 0; bt < NBT; ++bt) {
+        const int t = threadIdx.x + bt * kGridThreads;
+        if (t < 4 * NB) {
+            const int row = t / NB, j = t % NB;
             double2* rb = lds + row * RS;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) rb[lp(8 * j + q)] = v[q];
+            for (int q = 0; q < 8; ++q) rb[lp(8 * j + q)] = v[bt][q];
         }
     }
     __syncthreads();
-    fft_rest_c<+1, LGN, 3, 4>(lds, twl);
+    fft_rest_c<+1, LGN, 3, 4, COMPACT>(lds, twl);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
     for (int k = threadIdx.x; k < N; k += kGridThreads) {
         const int e = lp(k);
@@ -660,8 +701,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
         lds[e] = c2(jn, js);
     }
     __syncthreads();
-    fft_stage_c<-1, 8, 3, LGN, 0, 1>(lds, twl);
-    fft_rest_c<-1, LGN, 3, 1>(lds, twl);
+    fft_stage_c<-1, 8, 3, LGN, 0, 1, COMPACT>(lds, twl);
+    fft_rest_c<-1, LGN, 3, 1, COMPACT>(lds, twl);
     const size_t so = (size_t)jp * (L + 1) * nmem + b;
     unpack_sd(lds, N, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
 }
@@ -715,20 +756,32 @@ static size_t grid_lds(const aeo_plan_s* p, int rows, int nf) {
                               2 * (size_t)nf * (p->L + 1));
 }
 
-static size_t jac_lds(const aeo_plan_s* p) { return grid_lds(p, 4, 4); }
+static bool jac_p2(const aeo_plan_s* p) {
+    return fft_p2(p->fft) && p->nlon > 2 * p->L && p->fft.lgn >= 6 && p->fft.lgn <= 9;
+}
+
+// LDS of the Jacobian grid kernel jac_kernel_for(p, nmem) picks: for member
+// batches on the power-of-two instance four padded rows and a quarter
+// twiddle table (E/O staged in the rows), else grid_lds.
+static size_t jac_lds(const aeo_plan_s* p, int nmem) {
+    if (jac_p2(p) && nmem > 1)
+        return sizeof(double2) * (4 * (size_t)lds_row_stride(p->nlon) + p->nlon / 4);
+    return grid_lds(p, 4, 4);
+}
 
 // The Jacobian grid kernel for this plan: the compile-time power-of-two
 // instance for rows of 64 to 512 points without aliasing (n > 2L), else the
 // general one.
 using JacKernel = void (*)(GaussDev, const double2*, const double2*, double, int, int, double2*,
                            double2*);
-static JacKernel jac_kernel_for(const aeo_plan_s* p) {
-    if (fft_p2(p->fft) && p->nlon > 2 * p->L) {
+static JacKernel jac_kernel_for(const aeo_plan_s* p, int nmem) {
+    if (jac_p2(p)) {
+        const bool c = nmem > 1;
         switch (p->fft.lgn) {
-            case 6: return &grid_jac_p2_kernel<6>;
-            case 7: return &grid_jac_p2_kernel<7>;
-            case 8: return &grid_jac_p2_kernel<8>;
-            case 9: return &grid_jac_p2_kernel<9>;
+            case 6: return c ? &grid_jac_p2_kernel<6, true> : &grid_jac_p2_kernel<6, false>;
+            case 7: return c ? &grid_jac_p2_kernel<7, true> : &grid_jac_p2_kernel<7, false>;
+            case 8: return c ? &grid_jac_p2_kernel<8, true> : &grid_jac_p2_kernel<8, false>;
+            case 9: return c ? &grid_jac_p2_kernel<9, true> : &grid_jac_p2_kernel<9, false>;
             default: break;
         }
     }
@@ -831,8 +884,8 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     int rc = launch_stage<KindJac, EpiNoneS, false, true>(p, 1, sa, EpiNoneS{}, s);
     if (rc) return rc;
     const int lim = dealias ? (2 * L) / 3 : L;
-    if ((rc = allow_lds((const void*)jac_kernel_for(p), jac_lds(p)))) return rc;
-    hipLaunchKernelGGL(jac_kernel_for(p), dim3(p->nh), dim3(kGridThreads), jac_lds(p), s,
+    if ((rc = allow_lds((const void*)jac_kernel_for(p, 1), jac_lds(p, 1)))) return rc;
+    hipLaunchKernelGGL(jac_kernel_for(p, 1), dim3(p->nh), dim3(kGridThreads), jac_lds(p, 1), s,
                        p->gdev(), p->d_E, p->d_O, R, lim, 1, p->d_S, p->d_D);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
@@ -892,10 +945,11 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
     if (md->kind == kBVE) {
-        int rc0 = allow_lds((const void*)jac_kernel_for(p), jac_lds(p));
+        int rc0 = allow_lds((const void*)jac_kernel_for(p, md->nmem), jac_lds(p, md->nmem));
         if (rc0) return rc0;
         pr.arm(kClsGrid);
-        launch_k(jac_kernel_for(p), dim3(p->nh * md->nmem), dim3(kGridThreads), jac_lds(p), s,
+        launch_k(jac_kernel_for(p, md->nmem), dim3(p->nh * md->nmem), dim3(kGridThreads),
+                 jac_lds(p, md->nmem), s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
                  md->nmem, md->d_S, md->d_D);
     } else {
