@@ -36,7 +36,7 @@ _SIGNATURES = {
     "aeo_plan_enable_split": [_vp, _c_int],
     "aeo_model_set_precision": [_vp, _c_int],
     "aeo_model_set_tiles": [_vp, _c_int],
-    "aeo_model_set_gemm_tiles": [_vp, _c_int, _c_int],
+    "aeo_model_set_gemm_tiles": [_vp, _c_int, _c_int, _c_int],
     "aeo_plan_info": [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)],
     "aeo_plan_basis": [_vp, _vp, _vp],
     "aeo_analysis": [_vp, _vp, _c_int, _vp, _vp],

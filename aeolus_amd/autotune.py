@@ -8,8 +8,9 @@ and precision mode (``$AEOLUS_AMD_CACHE`` or
 
 Candidates: the fused stage kernel with 1, 2, 4 or 8 members per workgroup,
 and, for split-precision BVE batches, the GEMM form (``members_per_block``
-0) with 64 or 128 synthesis columns per workgroup and 4 or 8 members per
-epilogue workgroup.  Every candidate gives bitwise the same result.
+0) with 64 or 128 synthesis columns per workgroup, 4 or 8 members per
+epilogue workgroup and 2 or 4 MFMA column tiles per synthesis wave.  Every
+candidate gives bitwise the same result.
 """
 from __future__ import annotations
 
@@ -22,7 +23,8 @@ import torch
 __all__ = ["tune_members_per_block", "cache_path"]
 
 CANDIDATES = (1, 2, 4, 8)
-GEMM_CANDIDATES = ((64, 8), (64, 4), (128, 8), (128, 4))   # (syn_cols, epi_members)
+# (syn_cols, epi_members, syn_ntiles)
+GEMM_CANDIDATES = tuple((c, e, t) for c in (64, 128) for e in (4, 8) for t in (2, 4))
 
 
 def cache_path() -> Path:
@@ -60,7 +62,7 @@ def _store(table: dict) -> None:
 
 
 def _label(mb, gemm) -> str:
-    return f"mb{mb}" if gemm is None else f"gemm{gemm[0]}x{gemm[1]}"
+    return f"mb{mb}" if gemm is None else "gemm" + "x".join(str(v) for v in gemm)
 
 
 def _apply(model, mb, gemm) -> None:
@@ -73,8 +75,9 @@ def tune_members_per_block(model, y: torch.Tensor, dt: float, steps: int = 3,
                            candidates=CANDIDATES, use_cache: bool = True,
                            gemm_candidates=GEMM_CANDIDATES) -> dict:
     """Pick the fastest tile shape for ``model``; returns the timings (ms per
-    step, by label ``mb<k>`` or ``gemm<cols>x<members>``) and the choice
-    (``mb``, and ``gemm`` = [syn_cols, epi_members] when ``mb`` is 0)."""
+    step, by label ``mb<k>`` or ``gemm<cols>x<members>x<tiles>``) and the choice
+    (``mb``, and ``gemm`` = [syn_cols, epi_members, syn_ntiles] when ``mb``
+    is 0)."""
     key = _key(model)
     table = _load() if use_cache else {}
     if use_cache and key in table:

@@ -393,9 +393,9 @@ int aeo_model_set_tiles(aeo_model_t model, int members_per_block) {
     return AEO_OK;
 }
 
-int aeo_model_set_gemm_tiles(aeo_model_t model, int syn_cols, int epi_members) {
+int aeo_model_set_gemm_tiles(aeo_model_t model, int syn_cols, int epi_members, int syn_ntiles) {
     AEO_REQUIRE(model != nullptr, AEO_ERR_INVALID, "aeo_model_set_gemm_tiles: model is NULL");
-    return gemm_stage_tiles(model, syn_cols, epi_members);
+    return gemm_stage_tiles(model, syn_cols, epi_members, syn_ntiles);
 }
 
 int aeo_plan_destroy(aeo_plan_t plan) {

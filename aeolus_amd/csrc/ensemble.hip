@@ -49,6 +49,7 @@ static std::vector<long long> graph_key(const aeo_ensemble_s* e) {
         k.push_back(m->mb);
         k.push_back(m->gemm_cg);
         k.push_back(m->gemm_me);
+        k.push_back(m->gemm_ntw);
         k.push_back(m->nslices);
         k.push_back(m->plan->tables_gen);
     }

@@ -220,6 +220,7 @@ struct aeo_model_s {
     int gemm_ncp = 0;              // synthesis columns per m (padded)
     int gemm_cg = 64;              // synthesis columns per workgroup (64 or 128)
     int gemm_me = 8;               // members per epilogue workgroup (4 or 8)
+    int gemm_ntw = 4;              // 16-column tiles per synthesis wave (2 or 4)
     ProfRec prof;
 };
 
@@ -257,7 +258,7 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
 // (!ana) state.
 bool gemm_stage_supported(const aeo_model_s* md);
 int gemm_stage_setup(aeo_model_s* md);
-int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members);
+int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members, int syn_ntiles);
 int gemm_stage_launch(aeo_model_s* md, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
                       double2* ynext, bool ana, bool syn, hipStream_t s);

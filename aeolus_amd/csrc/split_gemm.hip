@@ -16,7 +16,8 @@
 //                    the new stage state with one exponent per column over
 //                    the degrees, written as digits Qd[m][parity][slice][col][k]
 //   gemm_syn_kernel  (m, parity, column group): INT8 MFMA of the synthesis
-//                    digit table of m against Qd, FP64 recombination -> E/O
+//                    digit table of m against Qd (a wave tile: 16 latitudes x
+//                    2 or 4 column tiles of 16), FP64 recombination -> E/O
 // Digits, exponents, the exact INT32 sums and the recombination are those of
 // the fused split kernel, so both forms give bitwise equal results (the
 // order of exact integer sums does not matter, and every FP64 expression is
@@ -328,9 +329,9 @@ __host__ __device__ inline size_t gemm_syn_lds(int L) {
     return sizeof(int) * CG + (size_t)NS * CG * (split_kpad(L, 0) + 16);
 }
 
-template <int NS, int CG>
+template <int NS, int CG, int NTW>
 __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, GemmArgs a) {
-    constexpr int NT = CG / 16, NG = NT / 4;     // a wave tile is 16 latitudes x 4 N-tiles
+    constexpr int NT = CG / 16, NG = NT / NTW;   // a wave tile is 16 latitudes x NTW N-tiles
     extern __shared__ __align__(16) unsigned char gsm[];
     const int m = blockIdx.x, p = blockIdx.y, c0 = blockIdx.z * CG;
     const int L = g.L, nh = g.nh, n = L + 1;
@@ -357,11 +358,11 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
     for (int tt = wave; tt < T; tt += kGemmWaves) {
         const int jt = tt / NG, ng = tt - jt * NG;
         const int j0 = jt * 16;
-        intx4 acc[NS][4];
+        intx4 acc[NS][NTW];
 #pragma unroll
         for (int q = 0; q < NS; ++q)
 #pragma unroll
-            for (int h = 0; h < 4; ++h) acc[q][h] = intx4{0, 0, 0, 0};
+            for (int h = 0; h < NTW; ++h) acc[q][h] = intx4{0, 0, 0, 0};
         for (int k0 = 0; k0 < kfull; k0 += kSplitK) {
             intx4 av[NS];
 #pragma unroll
@@ -370,11 +371,11 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
                     Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + k0 +
                     (lane >> 4) * 16);
 #pragma unroll
-            for (int h = 0; h < 4; ++h)
+            for (int h = 0; h < NTW; ++h)
 #pragma unroll
                 for (int t2 = 0; t2 < NS; ++t2) {
                     const intx4 bv = *reinterpret_cast<const intx4*>(
-                        qs + ((size_t)t2 * CG + (4 * ng + h) * 16 + (lane & 15)) * KS + k0 +
+                        qs + ((size_t)t2 * CG + (NTW * ng + h) * 16 + (lane & 15)) * KS + k0 +
                         (lane >> 4) * 16);
 #pragma unroll
                     for (int q = 0; q + t2 < NS; ++q)
@@ -390,11 +391,11 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
                     Pt + ((size_t)(p * NS + q) * nhp + j0 + (lane & 15)) * kpad + kfull +
                     (lane >> 4) * 8);
 #pragma unroll
-            for (int h = 0; h < 4; ++h)
+            for (int h = 0; h < NTW; ++h)
 #pragma unroll
                 for (int t2 = 0; t2 < NS; ++t2) {
                     const long bv = *reinterpret_cast<const long*>(
-                        qs + ((size_t)t2 * CG + (4 * ng + h) * 16 + (lane & 15)) * KS + kfull +
+                        qs + ((size_t)t2 * CG + (NTW * ng + h) * 16 + (lane & 15)) * KS + kfull +
                         (lane >> 4) * 8);
 #pragma unroll
                     for (int q = 0; q + t2 < NS; ++q)
@@ -406,8 +407,8 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_syn_kernel(GaussDev g, Gemm
 #pragma unroll
         for (int r = 0; r < 4; ++r) erow[r] = er[j0 + (lane >> 4) * 4 + r];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int c = (4 * ng + h) * 16 + (lane & 15);
+        for (int h = 0; h < NTW; ++h) {
+            const int c = (NTW * ng + h) * 16 + (lane & 15);
             const int cg = c0 + c;
             const int b = cg >> 3, f = (cg >> 1) & 3;
 #pragma unroll
@@ -497,13 +498,13 @@ static int gemm_epi_launch(const aeo_plan_s* p, const GemmArgs& a, hipStream_t s
     return AEO_OK;
 }
 
-template <int NS, int CG>
+template <int NS, int CG, int NTW>
 static int gemm_syn_launch(const aeo_plan_s* p, const GemmArgs& a, hipStream_t s) {
     const size_t lds = gemm_syn_lds<NS, CG>(p->L);
     int rc;
-    if ((rc = allow_gemm_lds(gemm_syn_kernel<NS, CG>, lds))) return rc;
-    launch_k(gemm_syn_kernel<NS, CG>, dim3(p->L + 1, 2, a.ncp / CG), dim3(kGemmThreads), lds, s,
-             p->gdev(), a);
+    if ((rc = allow_gemm_lds(gemm_syn_kernel<NS, CG, NTW>, lds))) return rc;
+    launch_k(gemm_syn_kernel<NS, CG, NTW>, dim3(p->L + 1, 2, a.ncp / CG), dim3(kGemmThreads), lds,
+             s, p->gdev(), a);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
@@ -522,17 +523,23 @@ static int gemm_stage_ns(aeo_model_s* md, const GemmArgs& a, hipStream_t s) {
     }
     rc = (md->gemm_me == 4) ? gemm_epi_launch<NS, 4>(p, a, s) : gemm_epi_launch<NS, 8>(p, a, s);
     if (rc || !a.syn) return rc;
-    return (md->gemm_cg == 128) ? gemm_syn_launch<NS, 128>(p, a, s)
-                                : gemm_syn_launch<NS, 64>(p, a, s);
+    if (md->gemm_ntw == 2)
+        return (md->gemm_cg == 128) ? gemm_syn_launch<NS, 128, 2>(p, a, s)
+                                    : gemm_syn_launch<NS, 64, 2>(p, a, s);
+    return (md->gemm_cg == 128) ? gemm_syn_launch<NS, 128, 4>(p, a, s)
+                                : gemm_syn_launch<NS, 64, 4>(p, a, s);
 }
 
-int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members) {
+int gemm_stage_tiles(aeo_model_s* md, int syn_cols, int epi_members, int syn_ntiles) {
     AEO_REQUIRE(syn_cols == 64 || syn_cols == 128, AEO_ERR_INVALID,
                 "aeo_model_set_gemm_tiles: syn_cols must be 64 or 128");
     AEO_REQUIRE(epi_members == 4 || epi_members == 8, AEO_ERR_INVALID,
                 "aeo_model_set_gemm_tiles: epi_members must be 4 or 8");
+    AEO_REQUIRE(syn_ntiles == 2 || syn_ntiles == 4, AEO_ERR_INVALID,
+                "aeo_model_set_gemm_tiles: syn_ntiles must be 2 or 4");
     md->gemm_cg = syn_cols;
     md->gemm_me = epi_members;
+    md->gemm_ntw = syn_ntiles;
     return AEO_OK;
 }
 

@@ -88,11 +88,14 @@ class BarotropicVorticity:
         the member-batched GEMM form of the split-precision stage."""
         nat.check(nat.lib().aeo_model_set_tiles(self._model, int(members_per_block)), "set_tiles")
 
-    def set_gemm_tiles(self, syn_cols: int = 64, epi_members: int = 8) -> None:
+    def set_gemm_tiles(self, syn_cols: int = 64, epi_members: int = 8,
+                       syn_ntiles: int = 4) -> None:
         """Tile shape of the GEMM form: synthesis columns per workgroup (64,
-        128) and members per epilogue workgroup (4, 8)."""
+        128), members per epilogue workgroup (4, 8) and 16-column MFMA tiles
+        per synthesis wave (2, 4)."""
         nat.check(nat.lib().aeo_model_set_gemm_tiles(self._model, int(syn_cols),
-                                                     int(epi_members)), "set_gemm_tiles")
+                                                     int(epi_members), int(syn_ntiles)),
+                  "set_gemm_tiles")
 
     def vorticity_to_streamfunction(self, vrt_state: BarotropicState) -> torch.Tensor:
         assert isinstance(vrt_state, BarotropicState)

@@ -74,7 +74,7 @@ def _args():
                         "runs; precision from --precision, default 6 slices)")
     p.add_argument("--tiles", type=str, default=None,
                    help="tiles of the L = 21,42,85,170 classes, fused members per workgroup "
-                        "or the GEMM form, e.g. mb8,mb8,gemm128x4,gemm128x4 (default: autotune)")
+                        "or the GEMM form, e.g. mb8,mb8,gemm64x8x2,gemm64x8x2 (default: autotune)")
     return p.parse_args()
 
 
@@ -281,15 +281,14 @@ def ensemble_modes(steps, ns_split):
 
 def _tile_label(t) -> str:
     """Autotuner choice as a label: mb<k> (fused, k members per workgroup) or
-    gemm<cols>x<members> (GEMM form)."""
-    return f"mb{t['mb']}" if t["mb"] else f"gemm{t['gemm'][0]}x{t['gemm'][1]}"
+    gemm<cols>x<members>x<tiles> (GEMM form)."""
+    return f"mb{t['mb']}" if t["mb"] else "gemm" + "x".join(str(v) for v in t["gemm"])
 
 
 def _apply_tile(model, label: str) -> None:
     if label.startswith("gemm"):
-        cols, mem = (int(v) for v in label[4:].split("x"))
         model.set_tiles(0)
-        model.set_gemm_tiles(cols, mem)
+        model.set_gemm_tiles(*(int(v) for v in label[4:].split("x")))
     else:
         model.set_tiles(int(label[2:] if label.startswith("mb") else label))
 

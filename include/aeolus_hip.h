@@ -181,10 +181,11 @@ int aeo_model_set_precision(aeo_model_t model, int nslices);
 int aeo_model_set_tiles(aeo_model_t model, int members_per_block);
 
 /* Tile shape of the GEMM form (members_per_block 0 above): synthesis
- * columns (member x field x re/im) per workgroup, 64 or 128, and members per
- * epilogue workgroup, 4 or 8 (defaults 64 and 8).  Results do not depend on
- * it.  No counterpart in the reference. */
-int aeo_model_set_gemm_tiles(aeo_model_t model, int syn_cols, int epi_members);
+ * columns (member x field x re/im) per workgroup, 64 or 128; members per
+ * epilogue workgroup, 4 or 8; 16-column MFMA tiles per synthesis wave, 2 or 4
+ * (defaults 64, 8, 4).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int aeo_model_set_gemm_tiles(aeo_model_t model, int syn_cols, int epi_members, int syn_ntiles);
 
 /* Tendency of one state (BVE: one field; SWE: the (3, L+1, L+1) stack zeta,
  * delta, phi).  forcing: device, same shape, or NULL (BVE only).

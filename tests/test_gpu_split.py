@@ -96,10 +96,10 @@ def test_split_gemm_form_matches_fused_bitwise(L, nlat, nlon, nmem, ns):
     a = fused.rk4(zs, 600.0, 3)
     b = gemm.rk4(zs, 600.0, 3)
     assert torch.equal(a, b)
-    for tiles in ((128, 4), (64, 4), (128, 8)):
+    for tiles in ((128, 4, 4), (64, 4, 2), (128, 8, 2)):
         gemm.set_gemm_tiles(*tiles)
         assert torch.equal(gemm.rk4(zs, 600.0, 3), a), tiles
-    gemm.set_gemm_tiles(64, 8)
+    gemm.set_gemm_tiles(64, 8, 4)
     ka = fused.tendency(BarotropicState(zs))
     kb = gemm.tendency(BarotropicState(zs))
     assert torch.equal(ka, kb)
@@ -128,6 +128,8 @@ def test_split_gemm_form_refused_where_unsupported():
         b.set_gemm_tiles(96, 8)
     with pytest.raises(ValueError, match="epi_members"):
         b.set_gemm_tiles(64, 2)
+    with pytest.raises(ValueError, match="syn_ntiles"):
+        b.set_gemm_tiles(64, 8, 3)
 
 
 def test_split_swe_tendency_matches_fp64():
