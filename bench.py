@@ -48,9 +48,16 @@ DAY_STEPS = 288          # one simulated day at dt = 300 s
 DT = 300.0
 TOL = 1e-11
 SPLITS = (4, 5, 6, 7)
-KCLASS = {0: "synthesis prime (leg_stage_kernel, SYN only)",
-          1: "grid_jac_kernel (row-pair FFTs + Jacobian)",
-          2: "leg_stage_kernel<KindBve, EpiBveS> (analysis + RK4 epilogue + next synthesis)"}
+def kclass(cls: int, nslices: int) -> str:
+    """The kernel a single-member BVE step launches for a profiling class
+    (0 synthesis prime, 1 grid stage, 2 Legendre stage) at a precision."""
+    if cls == 0:
+        return ("bve1_stage_kernel (synthesis only)" if nslices == 0
+                else "leg_stage_kernel<KindBve, EpiNoneS> (synthesis only)")
+    if cls == 1:
+        return "grid_jac_p2_kernel (row-pair FFTs + Jacobian)"
+    return ("bve1_stage_kernel (analysis + RK4 epilogue + next synthesis)" if nslices == 0
+            else "leg_stage_kernel<KindBve, EpiBveS> (analysis + RK4 epilogue + next synthesis)")
 
 
 def _args():
@@ -72,6 +79,10 @@ def _args():
     p.add_argument("--ensemble-only", action="store_true",
                    help="only the 256-member uneven-L ensemble, graph replay (for profiler "
                         "runs; precision from --precision, default 6 slices)")
+    p.add_argument("--batch-only", type=int, nargs="?", const=170, default=None, metavar="L",
+                   help="one 64-member BVE batch at L = 170 (or 85) alone: `--steps` RK4 "
+                        "steps after `--warmup`, `--precision` and the first `--tiles` label "
+                        "(default: autotune), one JSON line (profiler runs)")
     p.add_argument("--tiles", type=str, default=None,
                    help="tiles of the L = 21,42,85,170 classes, fused members per workgroup "
                         "or the GEMM form, e.g. mb8,mb8,gemm64x8x2,gemm64x8x2 (default: autotune)")
@@ -320,6 +331,35 @@ def ensemble_only(args):
                       "precision": _mode_name(ns), "tiles": tiles,
                       "steps": args.steps, "ms_per_ensemble_step": ms,
                       "member_steps_per_s": 4 * nper * 1000.0 / ms}))
+
+
+def batch_only(args):
+    """One 64-member BVE batch (the L = 170 or L = 85 class of the ensemble) on
+    its own: `--steps` fused RK4 steps after `--warmup`, one JSON line."""
+    import torch
+    from aeolus_amd.physics.barotropic import BarotropicBatch
+    from aeolus_amd.autotune import tune_members_per_block
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    L = int(args.batch_only)
+    grids = {170: (256, 512), 85: (128, 256)}
+    if L not in grids:
+        raise SystemExit("--batch-only: L must be 170 or 85")
+    nper = 64
+    ns = 6 if args.precision is None else args.precision
+    model = BarotropicBatch(_planet(L, *grids[L]), nper)
+    model.set_precision(ns)
+    y = torch.as_tensor(np.stack([spectrum_l15(L, np.random.default_rng(1000 * L + k),
+                                               rms=5e-5) for k in range(nper)]), device="cuda")
+    if args.tiles:
+        label = args.tiles.split(",")[0]
+        _apply_tile(model, label)
+    else:
+        label = _tile_label(tune_members_per_block(model, y, DT, steps=2))
+    out = torch.empty_like(y)
+    ms = _timed(lambda k: model.rk4(y, DT, k, out=out), args.steps, args.warmup)
+    print(json.dumps({"workload": f"BVE batch, 64 members at L = {L}",
+                      "precision": _mode_name(ns), "tiles": label, "steps": args.steps,
+                      "ms_per_step": ms, "member_steps_per_s": nper * 1000.0 / ms}))
 
 
 def geodesic_modes(steps, warmup):
@@ -614,6 +654,9 @@ def main():
     if args.ensemble_only:
         ensemble_only(args)
         return
+    if args.batch_only:
+        batch_only(args)
+        return
     if args.pe_only:
         print(json.dumps({"workload": "dry primitive equations, 8 sigma levels",
                           "pe_modes": pe_modes(min(args.steps, 100), args.pe_only)}))
@@ -628,11 +671,11 @@ def main():
     nbytes = _stage_bytes(dom, 170, 256, best)
     achieved = nbytes / (h["dom_ms"] * 1e-3) / 1e9
     flops = _stage_flops(dom, 170, 256)
-    traffic = _measured_traffic(KCLASS[dom], _mode_name(best))
+    traffic = _measured_traffic(kclass(dom, best), _mode_name(best))
     secondary = {
         "bve170_modes": {_mode_name(k): v for k, v in h["modes"].items()},
         "bve170_ms_per_step_kernel_timed_pass": h["ms_instr"],
-        "bve170_kernel_class_ms": {KCLASS[k]: v for k, v in h["cls_ms"].items()},
+        "bve170_kernel_class_ms": {kclass(k, best): v for k, v in h["cls_ms"].items()},
         "dominant_kernel_gflops_per_s": flops / (h["dom_ms"] * 1e-3) / 1e9,
     }
     if not args.headline_only:
@@ -673,7 +716,7 @@ def main():
                    "precision": _mode_name(best), "accuracy_bar": "rel-L2 <= 1e-11 vs FP64 "
                    "after 1 simulated day"},
         "secondary": secondary,
-        "roofline": {"bound": "hbm", "kernel": KCLASS[dom], "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kclass(dom, best), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic[0], "traffic_profile": traffic[1],
                      "kernel_sources": kernel_sources_hash(),
