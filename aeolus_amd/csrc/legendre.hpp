@@ -543,6 +543,20 @@ __device__ inline void col_max_exp(int ncol, int len, int* ex, const F& val) {
 }
 
 
+// Column m of stage workgroup (bx, bz) for launches that read a row-major
+// [l][m] batch (grid.x = stage_grid_x(L + 1)): workgroups are dealt to the
+// 8 XCDs round-robin by id (XCD = bx mod 8, grid.x a multiple of 8), so
+// groups of 8 consecutive m -- which share the 128-byte lines of the batch --
+// go to one XCD (one L2) instead of eight, and the groups are dealt
+// round-robin, rotated by bz, so every XCD gets light and heavy columns.
+// Ids past the last column return at once.  Launches on the m-major internal
+// layout keep m = bx (heaviest columns first).
+__host__ __device__ inline int stage_grid_x(int n) { return (n + 63) / 64 * 64; }
+__device__ inline int stage_column(int bx, int bz) {
+    const int x = (bx + bz) & 7, s = bx >> 3;
+    return (((s >> 3) << 3) + x) * 8 + (s & 7);
+}
+
 template <class KS, class Epi, int MB, bool ANA, bool SYN, int NS>
 __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, StageArgs a,
                                                                   Epi ep) {
@@ -552,9 +566,10 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     constexpr int kPre = Prefetch<MB, NA, NF>::PA;
     constexpr int kPC = Prefetch<MB, NA, NF>::PC;
     extern __shared__ double2 lds[];
-    const int m = blockIdx.x;
+    const int m = a.ys_mmajor ? (int)blockIdx.x : stage_column(blockIdx.x, blockIdx.z);
     const int b0 = blockIdx.z * MB;
     const int L = g.L, nh = g.nh, n = L + 1;
+    if (m >= n) return;
     const int K = L - m + 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int top = (a.lmax_a < L) ? a.lmax_a : L;      // analysed degrees l <= top

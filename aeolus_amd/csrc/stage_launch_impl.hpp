@@ -18,7 +18,7 @@ static int launch_stage_mb(const aeo_plan_s* p, const StageArgs& a, const Epi& e
         AEO_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    dim3 grid(p->L + 1, 1, (a.nmem + MB - 1) / MB);
+    dim3 grid(a.ys_mmajor ? p->L + 1 : stage_grid_x(p->L + 1), 1, (a.nmem + MB - 1) / MB);
     launch_k(kern, grid, dim3(kStageThreads), lds, s, p->gdev(), a, ep);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
