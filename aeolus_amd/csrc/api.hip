@@ -81,7 +81,8 @@ static void free_model(aeo_model_s* m) {
     free_prof(m->prof);
     for (void* q : {(void*)m->d_nulap, (void*)m->d_lap, (void*)m->d_nu4lap2,
                     (void*)m->d_lap_phi_s, (void*)m->d_nu4_lap2_phi_s, (void*)m->d_E,
-                    (void*)m->d_O, (void*)m->d_S, (void*)m->d_D, (void*)m->d_gw,
+                    (void*)m->d_O, (void*)m->d_E1, (void*)m->d_O1, (void*)m->d_S,
+                    (void*)m->d_D, (void*)m->d_gw,
                     (void*)m->d_part, (void*)m->d_ystage, (void*)m->d_ybuf, (void*)m->d_ksum,
                     (void*)m->d_fint, (void*)m->d_aE, (void*)m->d_aO, (void*)m->d_aS,
                     (void*)m->d_aD, (void*)m->d_asum, (void*)m->d_abuf, (void*)m->d_kbuf,
@@ -526,6 +527,10 @@ static int model_common(aeo_plan_t prod, double radius, aeo_model_s* m) {
         const size_t plane = (size_t)prod->nh * (L + 1);
         if ((rc = dev_alloc(&m->d_E, plane * nsyn * nm))) return rc;
         if ((rc = dev_alloc(&m->d_O, plane * nsyn * nm))) return rc;
+        if (m->kind == kBVE && nm == 1) {
+            if ((rc = dev_alloc(&m->d_E1, plane * nsyn))) return rc;
+            if ((rc = dev_alloc(&m->d_O1, plane * nsyn))) return rc;
+        }
         if ((rc = dev_alloc(&m->d_S, plane * nana * nm))) return rc;
         if ((rc = dev_alloc(&m->d_D, plane * nana * nm))) return rc;
     } else {

@@ -2,18 +2,27 @@
 // latency-shaped case (one state, so one workgroup per zonal wavenumber m is
 // all the parallelism the Legendre stage has).
 //
-// It computes exactly what leg_stage_kernel<KindBve, EpiBveS, 1, ANA, SYN, 0>
-// computes, in the same arithmetic order (so results are bitwise equal), but
-// is written for a short critical path:
+// It computes what leg_stage_kernel<KindBve, EpiBveS, 1, ANA, SYN, 0>
+// computes (per-degree analysis sums in the same order; the synthesis sums
+// of a split column associated per degree half), written for a short
+// critical path:
 //   * a small argument block and 32-bit indexing (no scalar reloads of a
 //     large kernel-argument struct, no 64-bit address arithmetic per load);
 //   * every global load that does not depend on this launch's results --
-//     the state column, S/D, the analysis table rows, the coupling columns,
+//     the analysis table rows, the state column, S/D, the coupling columns,
 //     the recurrence column, the synthesis seeds, the RK4 inputs y0 / ksum /
 //     forcing -- is issued at the start, before the first wait;
-//   * the heavy columns (small m) get two workgroups, each synthesising half
-//     of the latitudes; both run the (cheap, table-bound) analysis and the
-//     epilogue, only the first writes the global RK4 outputs.
+//   * the heavy columns (small m) get two workgroups that split the degrees
+//     of the column at an even s: each analyses its own degrees (plus the
+//     neighbour degree the S operator needs), runs the epilogue, writes the
+//     RK4 outputs of its degrees and synthesises their even/odd Legendre sums
+//     over all latitudes into its own E/O plane (the second half into plane
+//     1; the grid stage adds the planes in the fixed order plane 0 + plane 1).
+//     No analysis-table row is loaded twice and no workgroup waits on another;
+//   * analysis lanes cover degrees l and l + 64 (equal parity, so one LDS
+//     read of S or D serves both), synthesis lanes two latitudes j and j + 64
+//     (one broadcast read of a coefficient serves both): the per-degree LDS
+//     traffic, which bounds both loops, is halved.
 // Per workgroup: analysis of column m (lanes over degree, waves over
 // latitude, reduced over waves in LDS), tendency -J + nu lap' zeta (+ forcing)
 // with row 0 pinned and the RK4 stage update, then the four synthesis fields
@@ -27,8 +36,9 @@ namespace aeo {
 
 constexpr int kB1Threads = 512;
 constexpr int kB1Waves = kB1Threads / 64;
-constexpr int kB1Pre = 16;    // analysis-table rows prefetched per lane and chunk
-constexpr int kB1Chunks = 2;  // prefetched 64-degree chunks
+constexpr int kB1Pre = 16;       // analysis-table rows prefetched per lane and slot
+constexpr int kB1SynWaves = 4;   // synthesis degree blocks, one wave each
+constexpr int kB1SplitQ8 = 128;  // split point of a heavy column: s ~ K q8 / 256
 
 // Raw buffer over `count` doubles (the gfx9 raw-buffer descriptor word 3).
 __device__ inline __amdgpu_buffer_rsrc_t table_rsrc(const double* p, size_t count) {
@@ -42,7 +52,8 @@ __device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
 
 struct Bve1Args {
     int L, nh, top;           // top: analysed degrees l <= top (2L/3)
-    int msplit;               // columns m < msplit run as two latitude halves
+    int msplit;               // columns m < msplit run as two degree halves
+    int split_q8;             // their split point: i = l - m < s = (K q8 / 256) & ~1
     int ana, syn;             // analysis + epilogue; synthesis of the (new) state
     const double* Pa;         // analysis table, block m: [jp][l - m]
     const double* Ps;         // synthesis table, block m: [l - m][jp]
@@ -55,8 +66,10 @@ struct Bve1Args {
     double f10;               // f at (1, 0)
     const double2* S;         // [jp][m]
     const double2* D;
-    double2* E;               // [jp][m][4]
+    double2* E;               // plane 0: [jp][m][4]
     double2* O;
+    double2* E1;              // plane 1 (degree half 1 of the split columns): [jp][m][4]
+    double2* O1;
     const double2* ys;        // stage input state, internal [m][l]
     const double2* y0;        // step input state
     double2* ksum;
@@ -67,23 +80,42 @@ struct Bve1Args {
     double cst, dt6;
 };
 
+// Degree split point of a heavy column with K degrees: even, 2 <= s <= K - 2.
+__host__ __device__ inline int bve1_split_point(int K, int q8) {
+    int s = ((K * q8) >> 8) & ~1;
+    const int hi = (K - 2) & ~1;
+    return s < 2 ? 2 : (s > hi ? hi : s);
+}
+
+// kB1R: latitudes per synthesis lane (j + 64 q, q < kB1R): 2 when nh > 64.
+template <int kB1R>
 __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     extern __shared__ double2 lds[];
     const int L = a.L, n = L + 1, nh = a.nh;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // workgroup -> (m, latitude half): the split columns first (heaviest first)
-    int m, half;
+    // workgroup -> (m, degree half): the split columns first (heaviest first)
+    int m, h;
     if ((int)blockIdx.x < 2 * a.msplit) {
         m = blockIdx.x >> 1;
-        half = blockIdx.x & 1;
+        h = blockIdx.x & 1;
     } else {
         m = blockIdx.x - a.msplit;
-        half = -1;
+        h = -1;
     }
     const int K = L - m + 1;
     const int top = a.top;
-    const bool active = a.ana && m <= top;
-    const bool writer = half <= 0;
+    // degrees i = l - m in [d0, d1) are this workgroup's (RK4 outputs and
+    // synthesis); the state is needed on [e0, e1) (the S operator's
+    // neighbours) and analysed on [e0, a1]
+    int d0 = 0, d1 = K;
+    if (h >= 0) {
+        const int s = bve1_split_point(K, a.split_q8);
+        if (h == 0) d1 = s; else d0 = s;
+    }
+    const int e0 = (d0 > 0) ? d0 - 1 : 0, e1 = (d1 < K) ? d1 + 1 : K;
+    const int a1 = (e1 - 1 < top - m) ? e1 - 1 : top - m;
+    const bool active = a.ana && e0 <= a1;
+    const bool slot2 = active && e0 + 64 <= a1;      // degrees past e0 + 63 (uniform)
     const long long boff = (long long)nh * ((long long)m * n - (long long)m * (m - 1) / 2);
     const int col = m * n;                       // internal [m][l] column offset
 
@@ -97,43 +129,60 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     double* cmc = cpc + n;                       // [n] C- of m
     double* lapc = cmc + n;                      // [n] lap
     double2* scratch = reinterpret_cast<double2*>(lapc + n + (n & 1));
-    double2* X = scratch;                        // [2][nh] S then D
-    double2* accl = X + 2 * nh;                  // [n] analysed column
-    double2* red = accl + n;                     // [8][64] analysis wave partials
-    double2* sred = scratch;                     // [2 lg][2][2 parity][4][64] synthesis tree
+    // S then D, planes nh + 1 apart: the S and D reads of one latitude (lanes
+    // of both parities in one instruction) fall in different LDS banks
+    const int nhx = nh + 1;
+    double2* X = scratch;                        // [2][nh + 1]
+    double2* accl = X + 2 * nhx;                 // [n] analysed column
+    double2* red = accl + n;                     // [8][2][64] analysis wave partials
+    double2* sred = scratch;                     // [2][2 parity][4][kB1R][64] synthesis tree
 
     // ---- prologue: all independent global loads first
-    double pa[kB1Chunks][kB1Pre];
+    double pa[2][kB1Pre];
+    const int par = (e0 + lane) & 1;             // parity of both of this lane's degrees
     if (active) {
         const double* P = a.Pa + boff;
-        // buffer loads over block m: lanes past the band get an offset past the
-        // buffer's range, which reads 0 without a memory request (no branch per
-        // load; clamping them onto one address instead serialises the wave)
+        // buffer loads over block m: lanes past the range get an offset past
+        // the buffer's range, which reads 0 without a memory request (no
+        // branch per load; clamping them onto one address instead serialises
+        // the wave)
         const __amdgpu_buffer_rsrc_t rs = table_rsrc(P, (size_t)K * nh);
 #pragma unroll
-        for (int c = 0; c < kB1Chunks; ++c)
+        for (int u = 0; u < kB1Pre; ++u) {
+            const int i = e0 + lane, jp = wave + kB1Waves * u;
+            const unsigned off = (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
+            pa[0][u] = load_f64(rs, off);
+        }
+        if (slot2) {
 #pragma unroll
             for (int u = 0; u < kB1Pre; ++u) {
-                const int l = m + 64 * c + lane, jp = wave + kB1Waves * u;
-                const unsigned off = (l <= top && jp < nh) ? (unsigned)(jp * K + (l - m)) * 8u
-                                                           : kOutOfRange;
-                pa[c][u] = load_f64(rs, off);
+                const int i = e0 + 64 + lane, jp = wave + kB1Waves * u;
+                const unsigned off =
+                    (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
+                pa[1][u] = load_f64(rs, off);
             }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kB1Pre; ++u) pa[1][u] = 0.0;
+        }
     }
     // synthesis seeds of this lane's first latitude pass
-    const int kb = wave & 3, lg = wave >> 2;
-    const int Kb = ((K + 3) / 4 + 1) & ~1;
-    const int i0 = kb * Kb, i1 = (i0 + Kb < K) ? i0 + Kb : K;
-    const int lat_lo = (half < 0) ? 0 : 64 * half;             // latitude range of this workgroup
-    const int lat_hi = (half < 0) ? nh : ((64 * half + 64 < nh) ? 64 * half + 64 : nh);
-    const bool syn_wave = a.syn && (half < 0 || lg == 0);      // split halves use waves 0-3
-    const int lat_first = (half < 0) ? lg * 64 + lane : lat_lo + lane;
-    double sx = 0.0, sp0 = 0.0, sp1 = 0.0;
-    if (syn_wave && lat_first < lat_hi && i0 < K) {
-        const double* P = a.Ps + boff + lat_first;
-        sx = a.xn[lat_first];
-        sp0 = P[i0 * nh];
-        if (i0 + 1 < K) sp1 = P[(i0 + 1) * nh];
+    const int kb = wave;
+    const int Kd = d1 - d0;
+    const int Kb = ((Kd + kB1SynWaves - 1) / kB1SynWaves + 1) & ~1;
+    const int i0 = d0 + kb * Kb, i1 = (i0 + Kb < d1) ? i0 + Kb : d1;
+    const bool syn_wave = a.syn && wave < kB1SynWaves;
+    double sx[kB1R], sp0[kB1R], sp1[kB1R];
+#pragma unroll
+    for (int q = 0; q < kB1R; ++q) {
+        sx[q] = sp0[q] = sp1[q] = 0.0;
+        const int jp = lane + 64 * q;
+        if (syn_wave && jp < nh && i0 < i1) {
+            const double* P = a.Ps + boff + jp;
+            sx[q] = a.xn[jp];
+            sp0[q] = P[i0 * nh];
+            if (i0 + 1 < i1) sp1[q] = P[(i0 + 1) * nh];
+        }
     }
     // column data (thread t <-> degree l = t) and S/D (thread t <-> latitude t)
     double2 vs = c2(0.0, 0.0), vr = c2(0.0, 0.0), vy0 = c2(0.0, 0.0), vks = c2(0.0, 0.0);
@@ -167,7 +216,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     if (active && tid < nh) {
         X[tid] = vS;
-        X[nh + tid] = vD;
+        X[nhx + tid] = vD;
     }
     for (int t = tid + kB1Threads; t < n; t += kB1Threads) {    // n > 512 (L > 511)
         st[t] = a.ys[col + t];
@@ -180,61 +229,73 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     if (active)
         for (int t = tid + kB1Threads; t < nh; t += kB1Threads) {
             X[t] = a.S[t * n + m];
-            X[nh + t] = a.D[t * n + m];
+            X[nhx + t] = a.D[t * n + m];
         }
     __syncthreads();
 
-    // ---- analysis: lanes over degree (64 per chunk), waves over latitude
+    // ---- analysis: lanes over degrees e0 + lane and e0 + 64 + lane, waves
+    // over latitude (rows wave + 8u), reduced over waves in order
     if (active) {
         const double* P = a.Pa + boff;
-        for (int l0 = m, c = 0; l0 <= top; l0 += 64, ++c) {
-            double2 acc = c2(0.0, 0.0);
-            if (l0 + lane <= top) {
-                const int par = (l0 + lane - m) & 1;
-                for (int j0 = wave; j0 < nh; j0 += kB1Waves * kB1Pre) {
-                    double pv[kB1Pre];
-                    if (j0 == wave && c < kB1Chunks) {
+        double2 acc0 = c2(0.0, 0.0), acc1 = c2(0.0, 0.0);
+        for (int j0 = wave; j0 < nh; j0 += kB1Waves * kB1Pre) {
+            double pv[2][kB1Pre];
+            if (j0 == wave) {
 #pragma unroll
-                        for (int u = 0; u < kB1Pre; ++u) pv[u] = (c == 0) ? pa[0][u] : pa[kB1Chunks - 1][u];
-                    } else {
-                        const int l = l0 + lane;
+                for (int u = 0; u < kB1Pre; ++u) {
+                    pv[0][u] = pa[0][u];
+                    pv[1][u] = pa[1][u];
+                }
+            } else {                                      // nh > 128 (L > 170)
 #pragma unroll
-                        for (int u = 0; u < kB1Pre; ++u) {
-                            const int jp = j0 + kB1Waves * u;
-                            pv[u] = (jp < nh && l <= top) ? P[jp * K + (l - m)] : 0.0;
-                        }
-                    }
-                    // rows past nh have pv = 0 and read row nh - 1 (branch-free);
-                    // four LDS reads in flight per group
+                for (int q = 0; q < 2; ++q) {
+                    const int i = e0 + 64 * q + lane;
 #pragma unroll
-                    for (int u0 = 0; u0 < kB1Pre; u0 += 4) {
-                        double2 x[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int jq = j0 + kB1Waves * (u0 + u);
-                            x[u] = X[par * nh + ((jq < nh) ? jq : nh - 1)];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            acc.x = fma(pv[u0 + u], x[u].x, acc.x);
-                            acc.y = fma(pv[u0 + u], x[u].y, acc.y);
-                        }
+                    for (int u = 0; u < kB1Pre; ++u) {
+                        const int jp = j0 + kB1Waves * u;
+                        pv[q][u] = (jp < nh && i <= a1) ? P[jp * K + i] : 0.0;
                     }
                 }
             }
-            red[wave * 64 + lane] = acc;
-            __syncthreads();
-            if (tid < 64 && l0 + tid <= top) {
-                double2 sv = red[tid];
+            // rows past nh have pv = 0 and read row nh - 1 (branch-free);
+            // four LDS reads in flight per group
 #pragma unroll
-                for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[w * 64 + tid]);
-                accl[l0 + tid] = sv;
+            for (int u0 = 0; u0 < kB1Pre; u0 += 4) {
+                double2 x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jq = j0 + kB1Waves * (u0 + u);
+                    x[u] = X[par * nhx + ((jq < nh) ? jq : nh - 1)];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc0.x = fma(pv[0][u0 + u], x[u].x, acc0.x);
+                    acc0.y = fma(pv[0][u0 + u], x[u].y, acc0.y);
+                    if (slot2) {
+                        acc1.x = fma(pv[1][u0 + u], x[u].x, acc1.x);
+                        acc1.y = fma(pv[1][u0 + u], x[u].y, acc1.y);
+                    }
+                }
             }
-            __syncthreads();
         }
+        red[(wave * 2 + 0) * 64 + lane] = acc0;
+        red[(wave * 2 + 1) * 64 + lane] = acc1;
+        __syncthreads();
+        if (tid < 128) {
+            const int q = tid >> 6, i = e0 + 64 * q + lane;
+            if (i <= a1) {
+                double2 sv = red[q * 64 + lane];
+#pragma unroll
+                for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[(w * 2 + q) * 64 + lane]);
+                accl[m + i] = sv;
+            }
+        }
+        __syncthreads();
     }
 
-    // ---- epilogue: tendency and RK4 stage bookkeeping for degree l = tid
+    // ---- epilogue: tendency and RK4 stage bookkeeping for degree l = tid;
+    // the global RK4 outputs of the degrees this workgroup owns (the padding
+    // rows l < m belong to the first half)
     if (a.ana) {
         for (int l = tid; l < n; l += kB1Threads) {
             double2 y0v = vy0, ksv = vks, fv = vf;
@@ -245,6 +306,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                 ksv = (a.stage >= 1) ? a.ksum[col + l] : c2(0.0, 0.0);
                 fv = a.forcing ? a.forcing[col + l] : c2(0.0, 0.0);
             }
+            const int i = l - m;
+            const bool writer = (h < 0) || (h == 0 ? i < d1 : i >= d0);
             const int idx = col + l;
             double2 k = c2(0.0, 0.0);
             if (l > 0) {
@@ -275,27 +338,31 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     if (!a.syn) return;
 
-    // ---- synthesis fields of the column: psi, S psi, eta, S eta ([f][i], i = l - m)
-    for (int t = tid; t < 2 * K; t += kB1Threads) {
-        const int f = (t < K) ? 0 : 2, i = (t < K) ? t : t - K, l = m + i;
-        double2 v;
-        if (f == 0) {
-            if (l == 0) {
-                v = c2(0.0, 0.0);
+    // ---- synthesis fields of the column: psi, eta on [e0, e1), then
+    // S psi, S eta on [d0, d1) ([f][i], i = l - m)
+    {
+        const int ne = e1 - e0;
+        for (int t = tid; t < 2 * ne; t += kB1Threads) {
+            const int f = (t < ne) ? 0 : 2, i = e0 + ((t < ne) ? t : t - ne), l = m + i;
+            double2 v;
+            if (f == 0) {
+                if (l == 0) {
+                    v = c2(0.0, 0.0);
+                } else {
+                    const double2 z = st[l];
+                    const double d = lapc[l];
+                    v = c2(z.x / d, z.y / d);
+                }
             } else {
-                const double2 z = st[l];
-                const double d = lapc[l];
-                v = c2(z.x / d, z.y / d);
+                v = st[l];
+                if (l == 1 && m == 0) v.x += a.f10;
             }
-        } else {
-            v = st[l];
-            if (l == 1 && m == 0) v.x += a.f10;
+            cols[f * K + i] = v;
         }
-        cols[f * K + i] = v;
     }
     __syncthreads();
-    for (int t = tid; t < 2 * K; t += kB1Threads) {
-        const int src = (t < K) ? 0 : 2, i = (t < K) ? t : t - K, l = m + i;
+    for (int t = tid; t < 2 * Kd; t += kB1Threads) {
+        const int src = (t < Kd) ? 0 : 2, i = d0 + ((t < Kd) ? t : t - Kd), l = m + i;
         const double2* s = cols + src * K;
         double2 v = c2(0.0, 0.0);
         if (i >= 1) v = cfma(s[i - 1], cpc[l - 1], v);
@@ -304,30 +371,47 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     __syncthreads();
 
-    // ---- Pbar recurrence synthesis: wave = (degree block kb, latitude group)
-    auto slot = [&](int g2, int s2, int par, int f) -> double2& {
-        return sred[(((g2 * 2 + s2) * 2 + par) * 4 + f) * 64 + lane];
+    // ---- Pbar recurrence synthesis: wave = degree block kb of [d0, d1),
+    // lane = latitudes jbase + lane + 64 q (q < R)
+    auto slot = [&](int g2, int par2, int f, int q) -> double2& {
+        return sred[(((g2 * 2 + par2) * 4 + f) * kB1R + q) * 64 + lane];
     };
-    const int glat = (half < 0) ? lg : 0;         // LDS tree group of this wave
-    for (int jbase = (half < 0 ? 0 : lat_lo); jbase < lat_hi; jbase += (half < 0 ? 128 : 64)) {
-        const int jp = (half < 0) ? jbase + lg * 64 + lane : jbase + lane;
-        const bool ok = syn_wave && jp < lat_hi;
-        double2 accE[4], accO[4];
+    double2* Ep = (h == 1) ? a.E1 : a.E;
+    double2* Op = (h == 1) ? a.O1 : a.O;
+    for (int jbase = 0; jbase < nh; jbase += 64 * kB1R) {
+        double2 accE[kB1R][4], accO[kB1R][4];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) accE[f] = accO[f] = c2(0.0, 0.0);
-        if (ok && i0 < i1) {
-            const bool first = (jbase == (half < 0 ? 0 : lat_lo));
-            const double* P = a.Ps + boff + jp;
-            const double x = first ? sx : a.xn[jp];
-            double p2 = first ? sp0 : P[i0 * nh];
-            double p1 = first ? sp1 : ((i0 + 1 < i1) ? P[(i0 + 1) * nh] : 0.0);
+        for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) accE[q][f] = accO[q][f] = c2(0.0, 0.0);
+        if (syn_wave && i0 < i1) {
+            const bool first = (jbase == 0);
+            double x[kB1R], p1[kB1R], p2[kB1R];
+#pragma unroll
+            for (int q = 0; q < kB1R; ++q) {
+                const int jp = jbase + lane + 64 * q;
+                if (first) {
+                    x[q] = sx[q];
+                    p2[q] = sp0[q];
+                    p1[q] = sp1[q];
+                } else {
+                    const bool ok = jp < nh;
+                    const double* P = a.Ps + boff + (ok ? jp : 0);
+                    x[q] = ok ? a.xn[jp] : 0.0;
+                    p2[q] = ok ? P[i0 * nh] : 0.0;
+                    p1[q] = (ok && i0 + 1 < i1) ? P[(i0 + 1) * nh] : 0.0;
+                }
+            }
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 const double2* cf = cols + f * K;
                 const double2 ce = cf[i0];
                 const double2 co = cf[(i0 + 1 < i1) ? i0 + 1 : i0];
-                accE[f] = c2(pe_mul(p2, ce.x), pe_mul(p2, ce.y));
-                accO[f] = c2(pe_mul(p1, co.x), pe_mul(p1, co.y));
+#pragma unroll
+                for (int q = 0; q < kB1R; ++q) {
+                    accE[q][f] = c2(pe_mul(p2[q], ce.x), pe_mul(p2[q], ce.y));
+                    accO[q][f] = c2(pe_mul(p1[q], co.x), pe_mul(p1[q], co.y));
+                }
             }
             int i = i0 + 2;
             for (; i + 4 <= i1; i += 4) {
@@ -340,61 +424,82 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const double p = rec_step(ab[u], x, p1, p2);
-                    p2 = p1;
-                    p1 = p;
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
-                        double2& acc = (u & 1) ? accO[f] : accE[f];
-                        acc.x = fma(p, cv[u][f].x, acc.x);
-                        acc.y = fma(p, cv[u][f].y, acc.y);
+                    for (int q = 0; q < kB1R; ++q) {
+                        const double p = rec_step(ab[u], x[q], p1[q], p2[q]);
+                        p2[q] = p1[q];
+                        p1[q] = p;
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) {
+                            double2& acc = (u & 1) ? accO[q][f] : accE[q][f];
+                            acc.x = fma(p, cv[u][f].x, acc.x);
+                            acc.y = fma(p, cv[u][f].y, acc.y);
+                        }
                     }
                 }
             }
             for (; i < i1; ++i) {       // the block's last degrees (fewer than a group)
-                const double p = rec_step(rcc[m + i], x, p1, p2);
-                p2 = p1;
-                p1 = p;
+                const double2 ab = rcc[m + i];
                 const bool odd = (i - i0) & 1;
+                double2 c[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    const double2 c = cols[f * K + i];
-                    double2& acc = odd ? accO[f] : accE[f];
-                    acc.x = fma(p, c.x, acc.x);
-                    acc.y = fma(p, c.y, acc.y);
+                for (int f = 0; f < 4; ++f) c[f] = cols[f * K + i];
+#pragma unroll
+                for (int q = 0; q < kB1R; ++q) {
+                    const double p = rec_step(ab, x[q], p1[q], p2[q]);
+                    p2[q] = p1[q];
+                    p1[q] = p;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        double2& acc = odd ? accO[q][f] : accE[q][f];
+                        acc.x = fma(p, c[f].x, acc.x);
+                        acc.y = fma(p, c[f].y, acc.y);
+                    }
                 }
             }
         }
-        // blocks meet as (b0 + b2) + (b1 + b3), the order of the generic kernel
+        // blocks meet as (b0 + b2) + (b1 + b3)
         if (syn_wave && kb >= 2) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                slot(glat, kb - 2, 0, f) = accE[f];
-                slot(glat, kb - 2, 1, f) = accO[f];
-            }
+            for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    slot(kb - 2, 0, f, q) = accE[q][f];
+                    slot(kb - 2, 1, f, q) = accO[q][f];
+                }
         }
         __syncthreads();
         if (syn_wave && kb < 2) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                accE[f] = cadd(accE[f], slot(glat, kb, 0, f));
-                accO[f] = cadd(accO[f], slot(glat, kb, 1, f));
-            }
+            for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    accE[q][f] = cadd(accE[q][f], slot(kb, 0, f, q));
+                    accO[q][f] = cadd(accO[q][f], slot(kb, 1, f, q));
+                }
         }
         if (syn_wave && kb == 1) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                slot(glat, 1, 0, f) = accE[f];
-                slot(glat, 1, 1, f) = accO[f];
-            }
+            for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    slot(1, 0, f, q) = accE[q][f];
+                    slot(1, 1, f, q) = accO[q][f];
+                }
         }
         __syncthreads();
-        if (syn_wave && kb == 0 && ok) {
-            const int o = (jp * n + m) * 4;
+        if (syn_wave && kb == 0) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                a.E[o + f] = cadd(accE[f], slot(glat, 1, 0, f));
-                a.O[o + f] = cadd(accO[f], slot(glat, 1, 1, f));
+            for (int q = 0; q < kB1R; ++q) {
+                const int jp = jbase + lane + 64 * q;
+                if (jp < nh) {
+                    const int o = (jp * n + m) * 4;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        Ep[o + f] = cadd(accE[q][f], slot(1, 0, f, q));
+                        Op[o + f] = cadd(accO[q][f], slot(1, 1, f, q));
+                    }
+                }
             }
         }
         __syncthreads();
@@ -402,8 +507,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
 }
 
 static size_t bve1_lds(int L, int nh) {
-    const size_t n = L + 1;
-    const size_t ana = 2 * (size_t)nh + n + kB1Waves * 64, syn = 2 * 2 * 2 * 4 * 64;
+    const size_t n = L + 1, R = (nh > 64) ? 2 : 1;
+    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * 2 * 64, syn = 2 * 2 * 4 * R * 64;
     return sizeof(double2) * (6 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 
@@ -427,6 +532,16 @@ static int bve1_cus() {
     return cus;
 }
 
+// Split columns of a synthesising launch: as many of the heaviest columns
+// as the CUs left over by one workgroup per column (each split column keeps
+// at least 8 degrees per half).
+int bve1_msplit(const aeo_plan_s* p) {
+    const int n = p->L + 1;
+    int ms = bve1_cus() - n;
+    if (ms > n - 16) ms = n - 16;
+    return ms < 0 ? 0 : ms;
+}
+
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
                 hipStream_t s) {
@@ -436,13 +551,9 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.L = L;
     a.nh = p->nh;
     a.top = (2 * L) / 3;
-    // latitude halves only when one pass of 128 latitudes covers the hemisphere
-    int ms = 0;
-    if (syn && p->nh <= 128 && p->nh > 64) {
-        ms = bve1_cus() - n;
-        ms = ms < 0 ? 0 : (ms > n ? n : ms);
-    }
+    const int ms = syn ? bve1_msplit(p) : 0;
     a.msplit = ms;
+    a.split_q8 = kB1SplitQ8;
     a.ana = ana ? 1 : 0;
     a.syn = syn ? 1 : 0;
     a.Pa = p->d_Pa;
@@ -458,6 +569,8 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.D = md->d_D;
     a.E = md->d_E;
     a.O = md->d_O;
+    a.E1 = md->d_E1;
+    a.O1 = md->d_O1;
     a.ys = ys;
     a.y0 = y0;
     a.ksum = md->d_ksum;
@@ -468,10 +581,15 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.cst = (stage == 2) ? dt : 0.5 * dt;
     a.dt6 = dt / 6.0;
     const size_t lds = bve1_lds(L, p->nh);
+    const void* kern = (p->nh > 64) ? (const void*)bve1_stage_kernel<2>
+                                    : (const void*)bve1_stage_kernel<1>;
     if (lds > 65536)
-        AEO_CHECK_HIP(hipFuncSetAttribute((const void*)bve1_stage_kernel,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    launch_k(bve1_stage_kernel, dim3(n + ms), dim3(kB1Threads), lds, s, a);
+        AEO_CHECK_HIP(
+            hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (p->nh > 64)
+        launch_k(bve1_stage_kernel<2>, dim3(n + ms), dim3(kB1Threads), lds, s, a);
+    else
+        launch_k(bve1_stage_kernel<1>, dim3(n + ms), dim3(kB1Threads), lds, s, a);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }

@@ -128,7 +128,7 @@ __device__ inline void fft_stage(double2* buf, int nrows, int n, int Ns, const d
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = rb[lp(j + r * nb)];
-                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
+                if (r > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
                 v[b][r] = x;
             }
         }
@@ -173,7 +173,7 @@ __device__ inline void fft_stage_generic(double2* buf, int nrows, int n, int Ns,
             const double2* rb = buf + row * lds_row_stride(n);
             for (int r = 0; r < R; ++r) {
                 double2 x = rb[lp(j + r * nb)];
-                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
+                if (r > 0) x = cmul(x, twiddle<DIR>(tw, r * k * span));
                 const int e = ((r * q) % R) * nb;  // W_R^{rq} = W_n^{(rq mod R) n/R}
                 acc = cadd(acc, e ? cmul(x, twiddle<DIR>(tw, e)) : x);
             }
@@ -213,7 +213,7 @@ __device__ inline void fft_stage_p2(double2* buf, int nrows, int lgn, int lgNs, 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double2 x = rb[lp(j + (r << lgnb))];
-                if (r > 0 && k > 0) x = cmul(x, twiddle<DIR>(tw, (r * k) << lgspan));
+                if (r > 0) x = cmul(x, twiddle<DIR>(tw, (r * k) << lgspan));
                 v[b][r] = x;
             }
         }

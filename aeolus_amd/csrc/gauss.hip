@@ -228,18 +228,24 @@ __device__ inline void grid_wg_index(int nh, int nmem, int& jp, int& b) {
 // major (sE/sO[f * (L+1) + m], m <= L), so the per-row spectrum packing reads
 // consecutive wavenumbers.  Each thread issues all of its loads before its first
 // LDS store, so the gather costs one memory latency.
+// Columns m < msplit of a single-member synthesis arrive as two planes (the
+// degree halves of the split stage workgroups): E = E + E1, O = O + O1.
 template <int NF>
 __device__ inline void grid_prologue(const GaussDev& g, const double2* __restrict__ E,
                                        const double2* __restrict__ O, size_t eo, size_t ms,
-                                       double2* twl, double2* sE, double2* sO, int ntw = -1) {
+                                       double2* twl, double2* sE, double2* sO, int ntw = -1,
+                                       const double2* __restrict__ E1 = nullptr,
+                                       const double2* __restrict__ O1 = nullptr,
+                                       int msplit = 0) {
     // loads are unconditional (indices clamped in range) so the values stay
     // in registers; only the LDS stores are predicated.  ntw: twiddle entries
     // staged (the first ntw of the table; default the whole table)
     constexpr int PER = 4;
     const int n = (ntw < 0) ? g.nlon : ntw;
     const int tot = (g.L + 1) * NF;
+    const int tot1 = msplit * NF;                // plane-1 entries (m < msplit)
     for (int t0 = 0; t0 < tot || t0 < n; t0 += PER * kGridThreads) {
-        double2 ve[PER], vo[PER], vt[PER];
+        double2 ve[PER], vo[PER], vt[PER], ve1[PER], vo1[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int t = t0 + threadIdx.x + u * kGridThreads;
@@ -248,14 +254,21 @@ __device__ inline void grid_prologue(const GaussDev& g, const double2* __restric
             ve[u] = E[eo + m * ms + f];
             vo[u] = O[eo + m * ms + f];
             vt[u] = g.fft.tw[(t < n) ? t : n - 1];
+            if (tot1 > 0) {
+                const int t1 = (t < tot1) ? t : tot1 - 1;
+                const int m1 = t1 / NF, f1 = t1 - m1 * NF;
+                ve1[u] = E1[eo + m1 * ms + f1];
+                vo1[u] = O1[eo + m1 * ms + f1];
+            }
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int t = t0 + threadIdx.x + u * kGridThreads;
             if (t < tot) {
                 const int m = t / NF, f = t - m * NF;
-                sE[f * (g.L + 1) + m] = ve[u];
-                sO[f * (g.L + 1) + m] = vo[u];
+                const bool two = t < tot1;
+                sE[f * (g.L + 1) + m] = two ? cadd(ve[u], ve1[u]) : ve[u];
+                sO[f * (g.L + 1) + m] = two ? cadd(vo[u], vo1[u]) : vo[u];
             }
             if (t < n) twl[t] = vt[u];
         }
@@ -334,7 +347,8 @@ template <bool P2>
 __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, const double2* E,
                                                                const double2* O, double R,
                                                                int mmax, int nmem, double2* S,
-                                                               double2* D) {
+                                                               double2* D, const double2* E1,
+                                                               const double2* O1, int msplit) {
     extern __shared__ double2 lds[];
     int jp, b;
     grid_wg_index(g.nh, nmem, jp, b);
@@ -351,7 +365,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_kernel(GaussDev g, cons
     double2* sE = lds + 4 * rs + n;
     double2* sO = sE + (L + 1) * 4;
     const double2* twl = lds + 4 * rs;
-    grid_prologue<4>(g, E, O, eo, ms, lds + 4 * rs, sE, sO);
+    grid_prologue<4>(g, E, O, eo, ms, lds + 4 * rs, sE, sO, -1, E1, O1, msplit);
     __syncthreads();
     // rows: 0 a_l, 1 a_s, 2 b_l, 3 b_s
     for (int r = 0; r < 4; ++r) {
@@ -543,9 +557,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_adj_kernel(GaussDev g, cons
 // among double2 values instead, this gave wrong twiddles inside the grid
 // kernel on gfx950 while the same function alone was right.)
 template <int DIR, int LGN>
-__device__ inline double2 twiddle_q(const double2* tw4, int t) {
-    constexpr int Q = 1 << (LGN - 2);
-    const double2 w = tw4[t & (Q - 1)];
+__device__ inline double2 rotate_q(double2 w, int t) {
     const int q = t >> (LGN - 2);
     double x = (q & 1) ? w.y : w.x;
     double y = (q & 1) ? -w.x : w.y;
@@ -556,6 +568,10 @@ __device__ inline double2 twiddle_q(const double2* tw4, int t) {
     if (DIR > 0) y = -y;
     return c2(x, y);
 }
+template <int DIR, int LGN>
+__device__ inline double2 twiddle_q(const double2* tw4, int t) {
+    return rotate_q<DIR, LGN>(tw4[t & ((1 << (LGN - 2)) - 1)], t);
+}
 
 // QT: tw is the quarter table of twiddle_q, else the whole table.
 template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS, bool QT>
@@ -563,7 +579,9 @@ __device__ inline void fft_stage_c(double2* buf, const double2* tw) {
     constexpr int N = 1 << LGN, LGNB = LGN - LGR, NB = 1 << LGNB, TOT = NROWS * NB;
     constexpr int RS = N + (N >> 3), LGSPAN = LGN - LGNS - LGR;
     constexpr int MAXB = (TOT + kGridThreads - 1) / kGridThreads;
-    double2 v[MAXB][R];
+    // every data and table read of the stage is issued before the first
+    // product (one LDS latency per stage instead of one per radix input)
+    double2 v[MAXB][R], w[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int t = threadIdx.x + b * kGridThreads;
@@ -571,16 +589,38 @@ __device__ inline void fft_stage_c(double2* buf, const double2* tw) {
             const int row = t >> LGNB, j = t & (NB - 1), k = j & ((1 << LGNS) - 1);
             const double2* rb = buf + row * RS;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                double2 x = rb[lp(j + r * NB)];
-                if (LGNS > 0 && r > 0 && k > 0) {
-                    // explicit fma: both table forms give the same twiddle bits,
-                    // and the product must round alike in both instances
+            for (int r = 0; r < R; ++r) v[b][r] = rb[lp(j + r * NB)];
+            // no k > 0 test: W^0 = (1, 0) leaves x unchanged
+            if (LGNS > 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
                     const int e = (r * k) << LGSPAN;
-                    const double2 w = QT ? twiddle_q<DIR, LGN>(tw, e) : twiddle<DIR>(tw, e);
-                    x = c2(fma(x.x, w.x, -(x.y * w.y)), fma(x.x, w.y, x.y * w.x));
+                    w[b][r] = QT ? tw[e & ((N >> 2) - 1)] : tw[e];
                 }
-                v[b][r] = x;
+            }
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (LGNS > 0) {
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int t = threadIdx.x + b * kGridThreads;
+            if (t < TOT) {
+                const int k = (t & (NB - 1)) & ((1 << LGNS) - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    // explicit fma: both table forms give the same twiddle
+                    // bits, and the product must round alike in both instances
+                    const int e = (r * k) << LGSPAN;
+                    double2 wr = w[b][r];
+                    if (QT) {
+                        wr = rotate_q<DIR, LGN>(wr, e);
+                    } else if (DIR > 0) {
+                        wr.y = -wr.y;
+                    }
+                    const double2 x = v[b][r];
+                    v[b][r] = c2(fma(x.x, wr.x, -(x.y * wr.y)), fma(x.x, wr.y, x.y * wr.x));
+                }
             }
         }
     }
@@ -613,14 +653,25 @@ __device__ inline void fft_rest_c(double2* buf, const double2* tw) {
     }
 }
 
-// Packed spectrum bin b of one row (pack_spectrum, case n > 2L).
-__device__ inline double2 packed_bin(int b, int n, int L, const double2* Er, const double2* Or,
-                                     int mode, double invR) {
+// Packed spectrum bin b of one row (pack_spectrum, case n > 2L), in two
+// parts so a caller can issue the LDS reads of many bins before combining:
+// packed_bin_load gathers E/O at m1 = b and m2 = n - b, packed_bin_combine
+// forms the bin.
+struct PackedBinIn {
+    double2 e1, o1, e2, o2;
+};
+__device__ inline PackedBinIn packed_bin_load(int b, int n, int L, const double2* Er,
+                                              const double2* Or) {
     const int m1 = (b <= L) ? b : L;
     const int m2 = (b > 0 && n - b <= L) ? n - b : L;
-    const double2 e1 = Er[m1], o1 = Or[m1], e2 = Er[m2], o2 = Or[m2];
-    double2 gn = cadd(e1, o1), gs = csub(e1, o1);
-    double2 hn = cadd(e2, o2), hs = csub(e2, o2);
+    return PackedBinIn{Er[m1], Or[m1], Er[m2], Or[m2]};
+}
+__device__ inline double2 packed_bin_combine(const PackedBinIn& in, int b, int n, int L, int mode,
+                                             double invR) {
+    const int m1 = (b <= L) ? b : L;
+    const int m2 = (b > 0 && n - b <= L) ? n - b : L;
+    double2 gn = cadd(in.e1, in.o1), gs = csub(in.e1, in.o1);
+    double2 hn = cadd(in.e2, in.o2), hs = csub(in.e2, in.o2);
     if (mode == kSpecDlam) {
         gn = dlam_q(gn, m1 * invR); gs = dlam_q(gs, m1 * invR);
         hn = dlam_q(hn, m2 * invR); hs = dlam_q(hs, m2 * invR);
@@ -639,7 +690,8 @@ template <int LGN, bool COMPACT>
 __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, const double2* E,
                                                                   const double2* O, double R,
                                                                   int mmax, int nmem, double2* S,
-                                                                  double2* D) {
+                                                                  double2* D, const double2* E1,
+                                                                  const double2* O1, int msplit) {
     constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
     extern __shared__ double2 lds[];
     int jp, b;
@@ -659,24 +711,56 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     double2* twl = lds + 4 * RS;
     double2* sE = COMPACT ? lds : twl + N;
     double2* sO = sE + (L + 1) * 4;
-    grid_prologue<4>(g, E, O, eo, ms, twl, sE, sO, COMPACT ? N / 4 : N);
+    grid_prologue<4>(g, E, O, eo, ms, twl, sE, sO, COMPACT ? N / 4 : N, E1, O1, msplit);
     __syncthreads();
     // inverse stage 1 (radix 8, no twiddles) fed by the packed bins:
     // rows 0 a_l, 1 a_s, 2 b_l, 3 b_s
     constexpr int NBT = (4 * NB + kGridThreads - 1) / kGridThreads;
     static_assert(NBT <= 2, "two first-stage butterflies per thread at most");
     double2 v[NBT][8];
+    if constexpr (COMPACT) {
+        // member batches: bin by bin (registers kept for occupancy)
 #pragma unroll
-    for (int bt = 0; bt < NBT; ++bt) {
-        const int t = threadIdx.x + bt * kGridThreads;
-        if (t < 4 * NB) {
-            const int row = t / NB, j = t % NB;
-            const int mode = (row & 1) ? kSpecScale : kSpecDlam;
+        for (int bt = 0; bt < NBT; ++bt) {
+            const int t = threadIdx.x + bt * kGridThreads;
+            if (t < 4 * NB) {
+                const int row = t / NB, j = t % NB;
+                const int mode = (row & 1) ? kSpecScale : kSpecDlam;
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
-                v[bt][r] = packed_bin(j + r * NB, N, L, sE + row * (L + 1), sO + row * (L + 1),
-                                      mode, invR);
-            butterfly<+1, 8>(v[bt]);
+                for (int r = 0; r < 8; ++r)
+                    v[bt][r] = packed_bin_combine(
+                        packed_bin_load(j + r * NB, N, L, sE + row * (L + 1), sO + row * (L + 1)),
+                        j + r * NB, N, L, mode, invR);
+                butterfly<+1, 8>(v[bt]);
+            }
+        }
+    } else {
+        // single member (one workgroup per CU): every gather issued before
+        // the first combine
+        PackedBinIn pin[NBT][8];
+#pragma unroll
+        for (int bt = 0; bt < NBT; ++bt) {
+            const int t = threadIdx.x + bt * kGridThreads;
+            if (t < 4 * NB) {
+                const int row = t / NB, j = t % NB;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    pin[bt][r] = packed_bin_load(j + r * NB, N, L, sE + row * (L + 1),
+                                                 sO + row * (L + 1));
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int bt = 0; bt < NBT; ++bt) {
+            const int t = threadIdx.x + bt * kGridThreads;
+            if (t < 4 * NB) {
+                const int row = t / NB, j = t % NB;
+                const int mode = (row & 1) ? kSpecScale : kSpecDlam;
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    v[bt][r] = packed_bin_combine(pin[bt][r], j + r * NB, N, L, mode, invR);
+                butterfly<+1, 8>(v[bt]);
+            }
         }
     }
     if constexpr (COMPACT) __syncthreads();   // every E/O read before the rows are written
@@ -693,6 +777,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     __syncthreads();
     fft_rest_c<+1, LGN, 3, 4, COMPACT>(lds, twl);
     const double qN = 1.0 / (cN * cN), qS = 1.0 / (cS * cS);
+#pragma unroll
     for (int k = threadIdx.x; k < N; k += kGridThreads) {
         const int e = lp(k);
         const double2 al = lds[e], as = lds[RS + e], bl = lds[2 * RS + e], bs = lds[3 * RS + e];
@@ -773,7 +858,7 @@ static size_t jac_lds(const aeo_plan_s* p, int nmem) {
 // instance for rows of 64 to 512 points without aliasing (n > 2L), else the
 // general one.
 using JacKernel = void (*)(GaussDev, const double2*, const double2*, double, int, int, double2*,
-                           double2*);
+                           double2*, const double2*, const double2*, int);
 static JacKernel jac_kernel_for(const aeo_plan_s* p, int nmem) {
     if (jac_p2(p)) {
         const bool c = nmem > 1;
@@ -886,7 +971,8 @@ int gauss_jacobian(aeo_plan_s* p, double R, const double2* a, const double2* b, 
     const int lim = dealias ? (2 * L) / 3 : L;
     if ((rc = allow_lds((const void*)jac_kernel_for(p, 1), jac_lds(p, 1)))) return rc;
     hipLaunchKernelGGL(jac_kernel_for(p, 1), dim3(p->nh), dim3(kGridThreads), jac_lds(p, 1), s,
-                       p->gdev(), p->d_E, p->d_O, R, lim, 1, p->d_S, p->d_D);
+                       p->gdev(), p->d_E, p->d_O, R, lim, 1, p->d_S, p->d_D,
+                       (const double2*)nullptr, (const double2*)nullptr, 0);
     AEO_LAUNCH_CHECK();
     StageArgs sb{};
     sb.nmem = 1;
@@ -947,11 +1033,14 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     if (md->kind == kBVE) {
         int rc0 = allow_lds((const void*)jac_kernel_for(p, md->nmem), jac_lds(p, md->nmem));
         if (rc0) return rc0;
+        // a single-member synthesis leaves its split columns in two planes
+        const int ms1 = bve1_supported(md) ? bve1_msplit(p) : 0;
         pr.arm(kClsGrid);
         launch_k(jac_kernel_for(p, md->nmem), dim3(p->nh * md->nmem), dim3(kGridThreads),
                  jac_lds(p, md->nmem), s,
                  p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
-                 md->nmem, md->d_S, md->d_D);
+                 md->nmem, md->d_S, md->d_D, (const double2*)md->d_E1,
+                 (const double2*)md->d_O1, ms1);
     } else {
         const size_t swe_lds = grid_lds(p, kSweLdsRows, 10);
         int rc0 = allow_lds((const void*)grid_swe_kernel_for(p), swe_lds);

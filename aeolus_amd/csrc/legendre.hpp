@@ -506,9 +506,9 @@ struct StageLds {
                                       (size_t)epi_pre_slots<Epi, MB>(ana) * n);
         size_t a = 0, c = 0;
         if (ana) {
-            a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n) +
+            a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * (nh + 1) + n) +
                                    (size_t)kStageWaves * MB * Epi::NA * 64);
-            if (NS > 0) a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * nh + n)) +
+            if (NS > 0) a = sizeof(double2) * ((size_t)MB * Epi::NA * (2 * (nh + 1) + n)) +
                             sizeof(int) * 2 * NTA * 16 + 2 * NS * NTA * 16 * split_nhk(nh) + 64;
         }
         if (syn) {
@@ -699,8 +699,12 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
             if (i0 + 1 < K) sp1 = P[(size_t)(i0 + 1) * nh];
         }
     }
-    double2* X = work;                                  // [MB][NA][2][nh]  (S then D)
-    double2* accl = X + (size_t)MB * NA * 2 * nh;       // [MB][NA][n]
+    // [MB][NA][2][nh + 1] (S then D): planes nh + 1 apart, so the S and D
+    // reads of one latitude (lanes of both parities in one instruction) fall
+    // in different LDS banks
+    const int nhx = nh + 1;
+    double2* X = work;
+    double2* accl = X + (size_t)MB * NA * 2 * nhx;      // [MB][NA][n]
     {
         const int tot_s = MB * NST * n;
         const int tot_x = active ? MB * NA * nh : 0;
@@ -764,8 +768,8 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     const int f = t % NA;
                     const int bb = (t / NA) % MB;
                     const int jp = t / (NA * MB);
-                    X[((bb * NA + f) * 2 + 0) * nh + jp] = vx[u][0];
-                    X[((bb * NA + f) * 2 + 1) * nh + jp] = vx[u][1];
+                    X[((bb * NA + f) * 2 + 0) * nhx + jp] = vx[u][0];
+                    X[((bb * NA + f) * 2 + 1) * nhx + jp] = vx[u][1];
                 }
                 // u = 0: unconditional at the clamped index (threads past n
                 // rewrite entry n - 1 with its own value), so the loads are
@@ -806,7 +810,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                 const int p = t / NCP, c = t - p * NCP;
                 if (c >= NCA) return 0.0;
                 const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                return cpart(X[((bb * NA + f) * 2 + p) * nh + j], ri);
+                return cpart(X[((bb * NA + f) * 2 + p) * nhx + j], ri);
             });
             __syncthreads();
             // rows (p, c) over waves, latitude pairs over lanes (one 16-bit
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                     continue;
                 }
                 const int bb = c / (NA * 2), f = (c >> 1) % NA, ri = c & 1;
-                const double2* xr = X + ((bb * NA + f) * 2 + p) * nh;
+                const double2* xr = X + ((bb * NA + f) * 2 + p) * nhx;
                 const int e = gexp[p * NCP + c];
                 for (int j = 2 * lane; j < nhk; j += 128) {
                     const double v0 = (j < nh) ? cpart(xr[j], ri) : 0.0;
@@ -990,7 +994,7 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                             for (int bb = 0; bb < MB; ++bb)
 #pragma unroll
                                 for (int f = 0; f < NA; ++f) {
-                                    const double2 x = X[((bb * NA + f) * 2 + par) * nh + jp];
+                                    const double2 x = X[((bb * NA + f) * 2 + par) * nhx + jp];
 #pragma unroll
                                     for (int h = 0; h < kCH; ++h) {
                                         acc[h][bb][f].x = fma(pv[h][u], x.x, acc[h][bb][f].x);

@@ -192,6 +192,8 @@ struct aeo_model_s {
     // workspaces
     double2* d_E = nullptr;
     double2* d_O = nullptr;
+    double2* d_E1 = nullptr;       // single member: plane 1 of the split columns' E/O
+    double2* d_O1 = nullptr;
     double2* d_S = nullptr;
     double2* d_D = nullptr;
     double* d_gw = nullptr;
@@ -242,11 +244,14 @@ int gauss_model_stage(aeo_model_s* m, const double2* ys, const double2* y0,
                       const double2* forcing, double2* kout, int stage, double dt,
                       double2* ynext, bool syn_next, hipStream_t s);
 
-// Single-member FP64 BVE stage kernel (bve1.hip): the same results as the
-// generic stage kernel for models with one member and FP64 contractions.
+// Single-member FP64 BVE stage kernel (bve1.hip): the results of the generic
+// stage kernel (to rounding) for models with one member and FP64 contractions.
 // ana: analysis + tendency + RK4 epilogue of stage `stage` (-1: tendency into
 // kout); syn: synthesis of the resulting (ana) or given (!ana) state into E/O.
 bool bve1_supported(const aeo_model_s* md);
+// Heavy columns a synthesising launch splits into two degree halves (the
+// second half's E/O in d_E1/d_O1; the Jacobian grid stage adds the planes).
+int bve1_msplit(const aeo_plan_s* p);
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
                 hipStream_t s);

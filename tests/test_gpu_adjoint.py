@@ -222,8 +222,10 @@ def test_batch_adjoint_matches_single_members():
     _, ck = batch.rk4_record(z, dt, ns)
     g = batch.rk4_adjoint(ck, dt, ns, yb)
     for k in range(3):
+        # the single member's forward (checkpoints) runs the dedicated
+        # single-member kernel: equal to rounding, not bitwise
         _, ck1 = single.rk4_record(z[k], dt, ns)
-        assert torch.equal(g[k], single.rk4_adjoint(ck1, dt, ns, yb[k]))
+        assert rel_l2(_np(g[k]), _np(single.rk4_adjoint(ck1, dt, ns, yb[k]))) < 1e-12
 
 
 def test_adjoint_calls_need_the_setup_call():

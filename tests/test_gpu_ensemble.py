@@ -1,5 +1,7 @@
-"""Uneven-L ensemble: graph-captured step == eager step == per-member runs
-(bitwise), the autotuner picks a tile that runs, odd and even step counts."""
+"""Uneven-L ensemble: graph-captured step == eager step (bitwise) == per-member
+runs (bitwise for split precision; to rounding in FP64, where a single member
+runs the dedicated single-member kernel), the autotuner picks a tile that
+runs, odd and even step counts."""
 import math
 import warnings
 
@@ -48,7 +50,13 @@ def test_ensemble_graph_matches_eager_and_single(precision, tmp_path, monkeypatc
         single.set_precision(precision)
         for k in range(n):
             ref = single.rk4(y0[k], 300.0, 5)
-            assert torch.equal(y[k], ref), (L, k)
+            if precision:
+                assert torch.equal(y[k], ref), (L, k)
+            else:
+                # FP64 single members run the dedicated single-member kernel
+                # (split columns' synthesis sums associated per degree half)
+                err = float((y[k] - ref).abs().pow(2).sum().sqrt() / ref.abs().pow(2).sum().sqrt())
+                assert err < 1e-13, (L, k, err)
     # the cached choice is reused
     again = ens.autotune(states, 300.0, steps=2)
     assert all(t.get("cached") for t in again)

@@ -135,7 +135,9 @@ def test_swe_rk4_matches_oracle_short():
 def test_bve_batch_members_match_single_runs(L, nlat, nlon, nmem):
     """Every member of a batched step (the generic stage kernel) equals the
     single-member step (the dedicated single-member kernel, whose heavy
-    columns run as two latitude halves at L = 85 and 170) bitwise."""
+    columns run as two degree halves whose Legendre sums the grid stage adds)
+    to rounding: the per-degree analysis sums are the same, the synthesis sums
+    of a split column are associated per degree half."""
     import torch
     from aeolus_amd.physics import BarotropicVorticity
     from aeolus_amd.physics.barotropic import BarotropicBatch
@@ -150,9 +152,9 @@ def test_bve_batch_members_match_single_runs(L, nlat, nlon, nmem):
     tend = batch.tendency(BarotropicState_(torch.as_tensor(zs, device="cuda")))
     for k in range(nmem):
         ref = single.rk4(torch.as_tensor(zs[k], device="cuda"), 600.0, 5)
-        assert torch.equal(out[k], ref), k
+        assert rel_l2(_np(out[k]), _np(ref)) < 1e-13, k
         tk = single.tendency(BarotropicState_(torch.as_tensor(zs[k], device="cuda")))
-        assert torch.equal(tend[k], tk), k
+        assert rel_l2(_np(tend[k]), _np(tk)) < 1e-13, k
 
 
 def BarotropicState_(c):
