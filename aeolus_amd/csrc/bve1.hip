@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     double2* X = scratch;                        // [2][nh + 1]
     double2* accl = X + 2 * nhx;                 // [n] analysed column
     double2* red = accl + n;                     // [8][2][64] analysis wave partials
-    double2* sred = scratch;                     // [2][2 parity][4][kB1R][64] synthesis tree
+    double2* sred = scratch;                     // [NB/2][2 parity][4][kB1R][64] synthesis tree
 
     // ---- prologue: all independent global loads first
     double pa[2][kB1Pre];
@@ -293,9 +293,36 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         __syncthreads();
     }
 
+    // synthesis fields psi = zeta / lap (0 at l = 0) and eta = zeta (+ f at
+    // (1, 0)) of degree l on [e0, e1) ([f][i], i = l - m; S psi and S eta
+    // follow once every degree is in)
+    auto syn_fields = [&](int l, double2 z) {
+        const int i = l - m;
+        if (i >= e0 && i < e1) {
+            double2 v = c2(0.0, 0.0);
+            if (l > 0) {
+                const double d = lapc[l];
+                v = c2(z.x / d, z.y / d);
+            }
+            cols[i] = v;
+            if (l == 1 && m == 0) z.x += a.f10;
+            cols[2 * K + i] = z;
+        }
+    };
+
     // ---- epilogue: tendency and RK4 stage bookkeeping for degree l = tid;
     // the global RK4 outputs of the degrees this workgroup owns (the padding
-    // rows l < m belong to the first half)
+    // rows l < m belong to the first half); stored right away (stores held
+    // back to the end of the kernel delay its completion: measured slower)
+    auto store_out = [&](int l, double2 kv, double2 yv) {
+        const int idx = col + l;
+        if (a.stage < 0) {
+            a.kout[idx] = kv;
+        } else {
+            if (a.stage < 3) a.ksum[idx] = kv;
+            a.ynext[idx] = yv;
+        }
+    };
     if (a.ana) {
         for (int l = tid; l < n; l += kB1Threads) {
             double2 y0v = vy0, ksv = vks, fv = vf;
@@ -308,58 +335,32 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
             }
             const int i = l - m;
             const bool writer = (h < 0) || (h == 0 ? i < d1 : i >= d0);
-            const int idx = col + l;
             double2 k = c2(0.0, 0.0);
             if (l > 0) {
                 const double2 acc = accl[l];
                 k = cfma(st[l], nl, c2(-acc.x, -acc.y));
                 if (a.forcing) k = cadd(k, fv);
             }
-            double2 nx;
+            double2 nx, kv = k;                  // kv: the kout or ksum value
             if (a.stage < 0) {
-                if (writer) a.kout[idx] = k;
                 nx = k;
+            } else if (a.stage == 0) {
+                nx = cfma(k, a.cst, y0v);
+            } else if (a.stage < 3) {
+                kv = cfma(k, 2.0, ksv);
+                nx = cfma(k, a.cst, y0v);
             } else {
-                if (a.stage == 0) {
-                    if (writer) a.ksum[idx] = k;
-                    nx = cfma(k, a.cst, y0v);
-                } else if (a.stage < 3) {
-                    if (writer) a.ksum[idx] = cfma(k, 2.0, ksv);
-                    nx = cfma(k, a.cst, y0v);
-                } else {
-                    const double2 s = cadd(ksv, k);
-                    nx = cfma(s, a.dt6, y0v);
-                }
-                if (writer) a.ynext[idx] = nx;
+                const double2 s = cadd(ksv, k);
+                nx = cfma(s, a.dt6, y0v);
             }
-            st[l] = nx;
+            if (writer) store_out(l, kv, nx);
+            // psi and eta of the new state, straight into the synthesis columns
+            if (a.syn) syn_fields(l, nx);
         }
-        __syncthreads();
+    } else if (a.syn) {
+        for (int l = m + e0 + tid; l < m + e1; l += kB1Threads) syn_fields(l, st[l]);
     }
     if (!a.syn) return;
-
-    // ---- synthesis fields of the column: psi, eta on [e0, e1), then
-    // S psi, S eta on [d0, d1) ([f][i], i = l - m)
-    {
-        const int ne = e1 - e0;
-        for (int t = tid; t < 2 * ne; t += kB1Threads) {
-            const int f = (t < ne) ? 0 : 2, i = e0 + ((t < ne) ? t : t - ne), l = m + i;
-            double2 v;
-            if (f == 0) {
-                if (l == 0) {
-                    v = c2(0.0, 0.0);
-                } else {
-                    const double2 z = st[l];
-                    const double d = lapc[l];
-                    v = c2(z.x / d, z.y / d);
-                }
-            } else {
-                v = st[l];
-                if (l == 1 && m == 0) v.x += a.f10;
-            }
-            cols[f * K + i] = v;
-        }
-    }
     __syncthreads();
     for (int t = tid; t < 2 * Kd; t += kB1Threads) {
         const int src = (t < Kd) ? 0 : 2, i = d0 + ((t < Kd) ? t : t - Kd), l = m + i;
@@ -458,46 +459,53 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                 }
             }
         }
-        // blocks meet as (b0 + b2) + (b1 + b3)
-        if (syn_wave && kb >= 2) {
+        // blocks meet pairwise, b and b + half for half = NB/2, ..., 2, then
+        // wave 0 + wave 1 (four blocks: (b0 + b2) + (b1 + b3)); the last sum
+        // and store of latitude slot q runs on wave q % 2 (a + b == b + a)
+#pragma unroll
+        for (int half = kB1SynWaves / 2; half >= 2; half /= 2) {
+            if (syn_wave && kb >= half && kb < 2 * half) {
+#pragma unroll
+                for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        slot(kb - half, 0, f, q) = accE[q][f];
+                        slot(kb - half, 1, f, q) = accO[q][f];
+                    }
+            }
+            __syncthreads();
+            if (syn_wave && kb < half) {
+#pragma unroll
+                for (int q = 0; q < kB1R; ++q)
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        accE[q][f] = cadd(accE[q][f], slot(kb, 0, f, q));
+                        accO[q][f] = cadd(accO[q][f], slot(kb, 1, f, q));
+                    }
+            }
+            __syncthreads();
+        }
+        if (syn_wave && kb < 2) {
 #pragma unroll
             for (int q = 0; q < kB1R; ++q)
+                if ((q & 1) != kb)
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    slot(kb - 2, 0, f, q) = accE[q][f];
-                    slot(kb - 2, 1, f, q) = accO[q][f];
-                }
+                    for (int f = 0; f < 4; ++f) {
+                        slot(0, 0, f, q) = accE[q][f];
+                        slot(0, 1, f, q) = accO[q][f];
+                    }
         }
         __syncthreads();
         if (syn_wave && kb < 2) {
 #pragma unroll
-            for (int q = 0; q < kB1R; ++q)
-#pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    accE[q][f] = cadd(accE[q][f], slot(kb, 0, f, q));
-                    accO[q][f] = cadd(accO[q][f], slot(kb, 1, f, q));
-                }
-        }
-        if (syn_wave && kb == 1) {
-#pragma unroll
-            for (int q = 0; q < kB1R; ++q)
-#pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    slot(1, 0, f, q) = accE[q][f];
-                    slot(1, 1, f, q) = accO[q][f];
-                }
-        }
-        __syncthreads();
-        if (syn_wave && kb == 0) {
-#pragma unroll
             for (int q = 0; q < kB1R; ++q) {
                 const int jp = jbase + lane + 64 * q;
-                if (jp < nh) {
+                if ((q & 1) == kb && jp < nh) {
                     const int o = (jp * n + m) * 4;
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        Ep[o + f] = cadd(accE[q][f], slot(1, 0, f, q));
-                        Op[o + f] = cadd(accO[q][f], slot(1, 1, f, q));
+                        Ep[o + f] = cadd(accE[q][f], slot(0, 0, f, q));
+                        Op[o + f] = cadd(accO[q][f], slot(0, 1, f, q));
                     }
                 }
             }
@@ -508,7 +516,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
 
 static size_t bve1_lds(int L, int nh) {
     const size_t n = L + 1, R = (nh > 64) ? 2 : 1;
-    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * 2 * 64, syn = 2 * 2 * 4 * R * 64;
+    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * 2 * 64,
+                 syn = (kB1SynWaves / 2) * 2 * 4 * R * 64;
     return sizeof(double2) * (6 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 

@@ -275,6 +275,64 @@ __device__ inline void grid_prologue(const GaussDev& g, const double2* __restric
     }
 }
 
+// Prologue of the single-member power-of-two Jacobian kernel: like
+// grid_prologue<4>, but it stores the packed half-spectra of each field's
+// north/south rows instead of E/O (field f: d/dlambda for even f, 1/R for odd
+// f): Z1[f][m] = the bin-m term and Z2[f][m] = the mirrored bin (n - m) term
+// of packed_bin_combine, so a packed bin is Z1[b] + Z2[n - b] (two reads and
+// one add instead of four reads and the spectrum arithmetic per bin).
+__device__ inline void grid_prologue_z(const GaussDev& g, const double2* __restrict__ E,
+                                       const double2* __restrict__ O, size_t eo, size_t ms,
+                                       double2* twl, double2* sZ1, double2* sZ2,
+                                       const double2* __restrict__ E1,
+                                       const double2* __restrict__ O1, int msplit,
+                                       double invR) {
+    constexpr int NF = 4, PER = 4;
+    const int n = g.nlon;
+    const int tot = (g.L + 1) * NF;
+    const int tot1 = msplit * NF;
+    for (int t0 = 0; t0 < tot || t0 < n; t0 += PER * kGridThreads) {
+        double2 ve[PER], vo[PER], vt[PER], ve1[PER], vo1[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = t0 + threadIdx.x + u * kGridThreads;
+            const int te = (t < tot) ? t : tot - 1;
+            const int m = te / NF, f = te - m * NF;
+            ve[u] = E[eo + m * ms + f];
+            vo[u] = O[eo + m * ms + f];
+            vt[u] = g.fft.tw[(t < n) ? t : n - 1];
+            if (tot1 > 0) {
+                const int t1 = (t < tot1) ? t : tot1 - 1;
+                const int m1 = t1 / NF, f1 = t1 - m1 * NF;
+                ve1[u] = E1[eo + m1 * ms + f1];
+                vo1[u] = O1[eo + m1 * ms + f1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = t0 + threadIdx.x + u * kGridThreads;
+            if (t < tot) {
+                const int m = t / NF, f = t - m * NF;
+                const bool two = t < tot1;
+                const double2 e = two ? cadd(ve[u], ve1[u]) : ve[u];
+                const double2 o = two ? cadd(vo[u], vo1[u]) : vo[u];
+                double2 gn = cadd(e, o), gs = csub(e, o);
+                if (f & 1) {
+                    gn = cscale(gn, invR);
+                    gs = cscale(gs, invR);
+                } else {
+                    gn = dlam_q(gn, m * invR);
+                    gs = dlam_q(gs, m * invR);
+                }
+                sZ2[f * (g.L + 1) + m] = c2(gn.x + gs.y, -gn.y + gs.x);
+                if (m == 0) { gn.y = 0.0; gs.y = 0.0; }
+                sZ1[f * (g.L + 1) + m] = c2(gn.x - gs.y, gn.y + gs.x);
+            }
+            if (t < n) twl[t] = vt[u];
+        }
+    }
+}
+
 // Synthesis of nf fields (columns): E/O -> grid rows.  grid (nh, field
 // batches of `cap` rows, cap <= row_cap); block 256.
 template <bool P2>
@@ -709,9 +767,12 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     // table [N/4], and the E/O columns (8 (L+1) <= 4 RS as N > 2L) alias the
     // rows, which the first stage writes only after reading all of them
     double2* twl = lds + 4 * RS;
-    double2* sE = COMPACT ? lds : twl + N;
-    double2* sO = sE + (L + 1) * 4;
-    grid_prologue<4>(g, E, O, eo, ms, twl, sE, sO, COMPACT ? N / 4 : N, E1, O1, msplit);
+    double2* sE = COMPACT ? lds : twl + N;      // !COMPACT: Z1 of grid_prologue_z
+    double2* sO = sE + (L + 1) * 4;             // !COMPACT: Z2
+    if constexpr (COMPACT)
+        grid_prologue<4>(g, E, O, eo, ms, twl, sE, sO, N / 4, E1, O1, msplit);
+    else
+        grid_prologue_z(g, E, O, eo, ms, twl, sE, sO, E1, O1, msplit, invR);
     __syncthreads();
     // inverse stage 1 (radix 8, no twiddles) fed by the packed bins:
     // rows 0 a_l, 1 a_s, 2 b_l, 3 b_s
@@ -735,18 +796,22 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
             }
         }
     } else {
-        // single member (one workgroup per CU): every gather issued before
-        // the first combine
-        PackedBinIn pin[NBT][8];
+        // single member (one workgroup per CU): the packed half-spectra of
+        // the prologue, every read issued before the first add
+        double2 z1[NBT][8], z2[NBT][8];
 #pragma unroll
         for (int bt = 0; bt < NBT; ++bt) {
             const int t = threadIdx.x + bt * kGridThreads;
             if (t < 4 * NB) {
                 const int row = t / NB, j = t % NB;
 #pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    pin[bt][r] = packed_bin_load(j + r * NB, N, L, sE + row * (L + 1),
-                                                 sO + row * (L + 1));
+                for (int r = 0; r < 8; ++r) {
+                    const int b = j + r * NB;
+                    const int m1 = (b <= L) ? b : L;
+                    const int m2 = (b > 0 && N - b <= L) ? N - b : L;
+                    z1[bt][r] = sE[row * (L + 1) + m1];
+                    z2[bt][r] = sO[row * (L + 1) + m2];
+                }
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -754,11 +819,15 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
         for (int bt = 0; bt < NBT; ++bt) {
             const int t = threadIdx.x + bt * kGridThreads;
             if (t < 4 * NB) {
-                const int row = t / NB, j = t % NB;
-                const int mode = (row & 1) ? kSpecScale : kSpecDlam;
+                const int j = t % NB;
 #pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    v[bt][r] = packed_bin_combine(pin[bt][r], j + r * NB, N, L, mode, invR);
+                for (int r = 0; r < 8; ++r) {
+                    const int b = j + r * NB;
+                    double2 z = c2(0.0, 0.0);
+                    if (b <= L) z = z1[bt][r];
+                    if (b > 0 && N - b <= L) z = cadd(z, z2[bt][r]);
+                    v[bt][r] = z;
+                }
                 butterfly<+1, 8>(v[bt]);
             }
         }
