@@ -38,7 +38,7 @@ constexpr int kB1Threads = 512;
 constexpr int kB1Waves = kB1Threads / 64;
 constexpr int kB1Pre = 16;       // analysis-table rows prefetched per lane and slot
 constexpr int kB1SynWaves = 4;   // synthesis degree blocks, one wave each
-constexpr int kB1SplitQ8 = 128;  // split point of a heavy column: s ~ K q8 / 256
+constexpr int kB1SplitQ8 = 93;   // split point of a heavy column: s ~ K q8 / 256
 
 // Raw buffer over `count` doubles (the gfx9 raw-buffer descriptor word 3).
 __device__ inline __amdgpu_buffer_rsrc_t table_rsrc(const double* p, size_t count) {
@@ -80,9 +80,13 @@ struct Bve1Args {
     double cst, dt6;
 };
 
-// Degree split point of a heavy column with K degrees: even, 2 <= s <= K - 2.
+// Degree split point of a heavy column with K degrees: even, 2 <= s <= K - 2,
+// about K q8 / 256 but at most 62, so the first half's analysis (degrees
+// 0..s, the neighbour included) fits one 64-lane slot (a second slot doubles
+// its table loads: measured 0.0803 -> 0.0821 ms per L = 170 step).
 __host__ __device__ inline int bve1_split_point(int K, int q8) {
     int s = ((K * q8) >> 8) & ~1;
+    if (s > 62) s = 62;
     const int hi = (K - 2) & ~1;
     return s < 2 ? 2 : (s > hi ? hi : s);
 }
