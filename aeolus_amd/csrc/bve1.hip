@@ -64,8 +64,10 @@ struct Bve1Args {
     const double* lap;        // -l(l+1)/R^2, l = 0 entry 0
     const double* nulap;      // nu lap'_l
     double f10;               // f at (1, 0)
-    const double2* S;         // [jp][m]
-    const double2* D;
+    const double2* S;         // [jp][m]; hemi: FN, the north row's unweighted spectrum
+    const double2* D;         // hemi: FS
+    const double* wrow;       // hemi: nlat row weights (S = w (FN + FS), D = w (FN - FS))
+    int hemi, nlat;
     double2* E;               // plane 0: [jp][m][4]
     double2* O;
     double2* E1;              // plane 1 (degree half 1 of the split columns): [jp][m][4]
@@ -204,10 +206,12 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         if (a.stage >= 1) vks = a.ksum[col + tl];
         if (a.forcing) vf = a.forcing[col + tl];
     }
+    double vw = 0.0;
     if (active) {
         const int tj = (tid < nh) ? tid : nh - 1;
         vS = a.S[tj * n + m];
         vD = a.D[tj * n + m];
+        if (a.hemi) vw = (2 * tj + 1 == a.nlat) ? 0.5 * a.wrow[tj] : a.wrow[tj];
     }
     // stores
     if (tid < n) {
@@ -219,6 +223,11 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         accl[tid] = c2(0.0, 0.0);
     }
     if (active && tid < nh) {
+        if (a.hemi) {
+            const double2 fn = vS, fs = vD;
+            vS = cscale(cadd(fn, fs), vw);
+            vD = cscale(csub(fn, fs), vw);
+        }
         X[tid] = vS;
         X[nhx + tid] = vD;
     }
@@ -232,8 +241,15 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     if (active)
         for (int t = tid + kB1Threads; t < nh; t += kB1Threads) {
-            X[t] = a.S[t * n + m];
-            X[nhx + t] = a.D[t * n + m];
+            double2 s = a.S[t * n + m], d = a.D[t * n + m];
+            if (a.hemi) {
+                const double w = (2 * t + 1 == a.nlat) ? 0.5 * a.wrow[t] : a.wrow[t];
+                const double2 fn = s, fs = d;
+                s = cscale(cadd(fn, fs), w);
+                d = cscale(csub(fn, fs), w);
+            }
+            X[t] = s;
+            X[nhx + t] = d;
         }
     __syncthreads();
 
@@ -557,7 +573,7 @@ int bve1_msplit(const aeo_plan_s* p) {
 
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
-                hipStream_t s) {
+                bool hemi, hipStream_t s) {
     const aeo_plan_s* p = md->plan;
     const int L = p->L, n = L + 1;
     Bve1Args a{};
@@ -580,6 +596,9 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     a.S = md->d_S;
     a.D = md->d_D;
+    a.wrow = p->d_w_rows;
+    a.hemi = hemi ? 1 : 0;
+    a.nlat = p->nlat;
     a.E = md->d_E;
     a.O = md->d_O;
     a.E1 = md->d_E1;

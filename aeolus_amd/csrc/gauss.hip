@@ -861,6 +861,150 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     unpack_sd(lds, N, mmax, w, S + so, D + so, (size_t)nmem, threadIdx.x, kGridThreads);
 }
 
+// Single-member Jacobian grid stage, one workgroup per latitude row (both
+// rows of a pair are separate workgroups: 2 nh workgroups fill the chip where
+// the row-pair kernel leaves half of it idle).  The four fields of the row are
+// paired into two complex rows, (a_l + i a_s) and (b_l + i b_s); one inverse
+// FFT each gives the four grid rows, the Jacobian is formed per point and one
+// forward FFT of the real product row gives its Fourier coefficients F(m),
+// m <= mmax, stored unweighted: the north row's into FN[jp][m], the south
+// row's into FS[jp][m].  The single-member stage kernel forms S = w (FN + FS),
+// D = w (FN - FS) while staging them (phy/barotropic.py:120-167).
+template <int LGN>
+__global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
+    GaussDev g, const double2* __restrict__ E, const double2* __restrict__ O, double R, int mmax,
+    double2* __restrict__ FN, double2* __restrict__ FS, const double2* __restrict__ E1,
+    const double2* __restrict__ O1, int msplit) {
+    constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
+    extern __shared__ double2 lds[];
+    // workgroup -> (row pair jp, hemisphere hs): the two rows of a pair on
+    // one XCD (ids dealt to the 8 XCDs round-robin), sharing the E/O loads
+    int jp, hs;
+    const int id = blockIdx.x;
+    if ((g.nh & 7) == 0) {
+        const int x = id & 7, k = id >> 3;
+        hs = k & 1;
+        jp = (k >> 1) * 8 + x;
+    } else {
+        jp = id >> 1;
+        hs = id & 1;
+    }
+    const int L = g.L;
+    const int j = hs ? g.nlat - 1 - jp : jp;
+    const double invR = 1.0 / R;
+    const double c = clamp_coslat(g.coslat[j]);
+    const size_t eo = (size_t)jp * (L + 1) * 4;
+    // LDS: rows [2][RS] | twiddle table [N] | Z1, Z2 [2][L+1] each
+    double2* twl = lds + 2 * RS;
+    double2* sZ1 = twl + N;
+    double2* sZ2 = sZ1 + 2 * (L + 1);
+    {
+        // prologue: every load first (E/O of the 4 fields, plane 1 of the
+        // split columns, the twiddle table), then the packed half-spectra
+        // Z1[r][m] = g_2r(m) + i g_2r+1(m) (the bin-m term) and
+        // Z2[r][m] = conj(g_2r(m)) + i conj(g_2r+1(m)) (the mirrored bin's),
+        // g_f = E_f +- O_f with d/dlambda (even f) or 1/R (odd f)
+        constexpr int PER = 4;
+        const int tot = (L + 1) * 4, tot1 = msplit * 4;
+        for (int t0 = 0; t0 < tot || t0 < N; t0 += PER * kGridThreads) {
+            double2 ve[PER], vo[PER], vt[PER], ve1[PER], vo1[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int t = t0 + threadIdx.x + u * kGridThreads;
+                const int te = (t < tot) ? t : tot - 1;
+                ve[u] = E[eo + te];
+                vo[u] = O[eo + te];
+                vt[u] = g.fft.tw[(t < N) ? t : N - 1];
+                if (tot1 > 0) {
+                    const int t1 = (t < tot1) ? t : tot1 - 1;
+                    ve1[u] = E1[eo + t1];
+                    vo1[u] = O1[eo + t1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int t = t0 + threadIdx.x + u * kGridThreads;
+                if (t < tot) {
+                    const int m = t >> 2, f = t & 3;
+                    const bool two = t < tot1;
+                    const double2 e = two ? cadd(ve[u], ve1[u]) : ve[u];
+                    const double2 o = two ? cadd(vo[u], vo1[u]) : vo[u];
+                    double2 gv = hs ? csub(e, o) : cadd(e, o);
+                    gv = (f & 1) ? cscale(gv, invR) : dlam_q(gv, m * invR);
+                    // the pair (f, f + 1) is combined after a barrier: even
+                    // fields parked in Z2, odd ones in Z1
+                    if (f & 1)
+                        sZ1[(f >> 1) * (L + 1) + m] = gv;
+                    else
+                        sZ2[(f >> 1) * (L + 1) + m] = gv;
+                }
+                if (t < N) twl[t] = vt[u];
+            }
+        }
+    }
+    __syncthreads();
+    // combine the parked fields into Z1/Z2 (one thread per (pair r, m))
+    for (int t = threadIdx.x; t < 2 * (L + 1); t += kGridThreads) {
+        const int r = t / (L + 1), m = t - r * (L + 1);
+        double2 ge = sZ2[r * (L + 1) + m], go = sZ1[r * (L + 1) + m];
+        const double2 z2 = c2(ge.x + go.y, -ge.y + go.x);
+        if (m == 0) { ge.y = 0.0; go.y = 0.0; }
+        const double2 z1 = c2(ge.x - go.y, ge.y + go.x);
+        sZ1[r * (L + 1) + m] = z1;
+        sZ2[r * (L + 1) + m] = z2;
+    }
+    __syncthreads();
+    // inverse stage 1 (radix 8, no twiddles) fed by the packed bins
+    {
+        double2 v[8], z1[8], z2[8];
+        const int t = threadIdx.x;
+        if (t < 2 * NB) {
+            const int row = t / NB, jj = t % NB;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int b = jj + r * NB;
+                const int m1 = (b <= L) ? b : L;
+                const int m2 = (b > 0 && N - b <= L) ? N - b : L;
+                z1[r] = sZ1[row * (L + 1) + m1];
+                z2[r] = sZ2[row * (L + 1) + m2];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (t < 2 * NB) {
+            const int row = t / NB, jj = t % NB;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int b = jj + r * NB;
+                double2 z = c2(0.0, 0.0);
+                if (b <= L) z = z1[r];
+                if (b > 0 && N - b <= L) z = cadd(z, z2[r]);
+                v[r] = z;
+            }
+            butterfly<+1, 8>(v);
+            double2* rb = lds + row * RS;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rb[lp(8 * jj + q)] = v[q];
+        }
+    }
+    __syncthreads();
+    fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
+    // the Jacobian row (real) into row 0 as a complex row with zero
+    // imaginary part
+    const double q = 1.0 / (c * c);
+#pragma unroll
+    for (int k = threadIdx.x; k < N; k += kGridThreads) {
+        const int e = lp(k);
+        const double2 za = lds[e], zb = lds[RS + e];
+        const double jv = jac_point_q(za.x, za.y, zb.x, zb.y, q);
+        lds[e] = c2(jv, 0.0);
+    }
+    __syncthreads();
+    fft_stage_c<-1, 8, 3, LGN, 0, 1, false>(lds, twl);
+    fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
+    double2* F = (hs ? FS : FN) + (size_t)jp * (L + 1);
+    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[m] = lds[lp(m)];
+}
+
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).
 #define AEO_GRID_PICK(name)                                                        \
     static decltype(&name<true>) name##_for(const aeo_plan_s* p) {                 \
@@ -940,6 +1084,26 @@ static JacKernel jac_kernel_for(const aeo_plan_s* p, int nmem) {
         }
     }
     return grid_jac_kernel_for(p);
+}
+
+// The single-member BVE stage pairs with the per-row Jacobian grid kernel
+// (grid_jac_hemi_kernel) where the power-of-two instance applies; its S/D
+// arrays then hold the unweighted row spectra FN/FS.
+static bool bve1_hemi(const aeo_model_s* md) {
+    return md->kind == kBVE && bve1_supported(md) && jac_p2(md->plan);
+}
+using HemiKernel = void (*)(GaussDev, const double2*, const double2*, double, int, double2*,
+                            double2*, const double2*, const double2*, int);
+static HemiKernel hemi_kernel_for(const aeo_plan_s* p) {
+    switch (p->fft.lgn) {
+        case 6: return &grid_jac_hemi_kernel<6>;
+        case 7: return &grid_jac_hemi_kernel<7>;
+        case 8: return &grid_jac_hemi_kernel<8>;
+        default: return &grid_jac_hemi_kernel<9>;
+    }
+}
+static size_t hemi_lds(const aeo_plan_s* p) {
+    return sizeof(double2) * (2 * (size_t)lds_row_stride(p->nlon) + p->nlon + 4 * (size_t)(p->L + 1));
 }
 
 // Rows per workgroup of the plain transforms' grid kernels for a batch of nf
@@ -1070,7 +1234,8 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     sa.k = col_consts(p, md->R, md->d_lap, f10);
     md->prof.arm(kClsSynthesis);
     if (bve1_supported(md))
-        return bve1_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true, s);
+        return bve1_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true,
+                           false, s);
     if (md->mb == 0)
         return gemm_stage_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true,
                                  s);
@@ -1099,7 +1264,17 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     const size_t fs = (size_t)(L + 1) * (L + 1);
     const double f10 = 2.0 * md->omega * sqrt(4.0 * M_PI / 3.0);
     ProfRec& pr = md->prof;
-    if (md->kind == kBVE) {
+    if (bve1_hemi(md)) {
+        // single member: one workgroup per latitude row; the split columns
+        // of the last synthesis arrive in two planes
+        int rc0 = allow_lds((const void*)hemi_kernel_for(p), hemi_lds(p));
+        if (rc0) return rc0;
+        pr.arm(kClsGrid);
+        launch_k(hemi_kernel_for(p), dim3(2 * p->nh), dim3(kGridThreads), hemi_lds(p), s,
+                 p->gdev(), (const double2*)md->d_E, (const double2*)md->d_O, md->R, cut,
+                 md->d_S, md->d_D, (const double2*)md->d_E1, (const double2*)md->d_O1,
+                 bve1_msplit(p));
+    } else if (md->kind == kBVE) {
         int rc0 = allow_lds((const void*)jac_kernel_for(p, md->nmem), jac_lds(p, md->nmem));
         if (rc0) return rc0;
         // a single-member synthesis leaves its split columns in two planes
@@ -1137,7 +1312,8 @@ int gauss_model_stage(aeo_model_s* md, const double2* ys, const double2* y0,
     int rc;
     pr.arm(kClsAnalysis);
     if (md->kind == kBVE && bve1_supported(md)) {
-        rc = bve1_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next, s);
+        rc = bve1_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next,
+                         bve1_hemi(md), s);
     } else if (md->mb == 0) {
         rc = gemm_stage_launch(md, ys, y0, forcing, kout, stage, dt, ynext, true, syn_next, s);
     } else if (md->kind == kBVE) {
