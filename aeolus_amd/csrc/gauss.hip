@@ -987,20 +987,63 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         }
     }
     __syncthreads();
-    fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
-    // the Jacobian row (real) into row 0 as a complex row with zero
-    // imaginary part
     const double q = 1.0 / (c * c);
+    if constexpr (LGN % 3 == 0) {
+        // radix-8 stages only: the last inverse stage of lane j leaves both
+        // rows' values at the points j + N/8 q, exactly the inputs of the
+        // first forward stage's lane j, so the last inverse stage, the
+        // Jacobian at those points and the first forward stage run in
+        // registers (the arithmetic of the staged path, no LDS round trip)
+        static_assert(LGN == 6 || LGN == 9, "radix-8 rows of 64 or 512 points");
+        if constexpr (LGN == 9) fft_stage_c<+1, 8, 3, LGN, 3, 2, false>(lds, twl);
+        constexpr int LS = LGN - 3;                  // the last stage's LGNS (N/8 = 2^LS lanes)
+        const int jl = threadIdx.x;
+        double2 v[8];
+        if (jl < NB) {
+            double2 xa[8], xb[8], wt[8];
 #pragma unroll
-    for (int k = threadIdx.x; k < N; k += kGridThreads) {
-        const int e = lp(k);
-        const double2 za = lds[e], zb = lds[RS + e];
-        const double jv = jac_point_q(za.x, za.y, zb.x, zb.y, q);
-        lds[e] = c2(jv, 0.0);
+            for (int r = 0; r < 8; ++r) {
+                xa[r] = lds[lp(jl + r * NB)];
+                xb[r] = lds[RS + lp(jl + r * NB)];
+                if (r > 0) wt[r] = twl[r * jl];      // W^(r k), k = jl (span 1)
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 1; r < 8; ++r) {
+                const double2 w = c2(wt[r].x, -wt[r].y);   // inverse: conjugate
+                xa[r] = c2(fma(xa[r].x, w.x, -(xa[r].y * w.y)), fma(xa[r].x, w.y, xa[r].y * w.x));
+                xb[r] = c2(fma(xb[r].x, w.x, -(xb[r].y * w.y)), fma(xb[r].x, w.y, xb[r].y * w.x));
+            }
+            butterfly<+1, 8>(xa);
+            butterfly<+1, 8>(xb);
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                v[r] = c2(jac_point_q(xa[r].x, xa[r].y, xb[r].x, xb[r].y, q), 0.0);
+            butterfly<-1, 8>(v);
+        }
+        static_assert(LS >= 3, "fused stages need N >= 64");
+        __syncthreads();                             // every row read before row 0 is rewritten
+        if (jl < NB) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) lds[lp(8 * jl + r)] = v[r];
+        }
+        __syncthreads();
+        fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
+    } else {
+        fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
+        // the Jacobian row (real) into row 0 as a complex row with zero
+        // imaginary part
+#pragma unroll
+        for (int k = threadIdx.x; k < N; k += kGridThreads) {
+            const int e = lp(k);
+            const double2 za = lds[e], zb = lds[RS + e];
+            const double jv = jac_point_q(za.x, za.y, zb.x, zb.y, q);
+            lds[e] = c2(jv, 0.0);
+        }
+        __syncthreads();
+        fft_stage_c<-1, 8, 3, LGN, 0, 1, false>(lds, twl);
+        fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
     }
-    __syncthreads();
-    fft_stage_c<-1, 8, 3, LGN, 0, 1, false>(lds, twl);
-    fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
     double2* F = (hs ? FS : FN) + (size_t)jp * (L + 1);
     for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[m] = lds[lp(m)];
 }
