@@ -901,9 +901,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     {
         // prologue: every load first (E/O of the 4 fields, plane 1 of the
         // split columns, the twiddle table), then the packed half-spectra
-        // Z1[r][m] = g_2r(m) + i g_2r+1(m) (the bin-m term) and
-        // Z2[r][m] = conj(g_2r(m)) + i conj(g_2r+1(m)) (the mirrored bin's),
-        // g_f = E_f +- O_f with d/dlambda (even f) or 1/R (odd f)
+        // Z1[r][m] = g_2r(m) + i g_2r+1(m) (the bin-m term, real at m = 0)
+        // and Z2[r][m] = conj(g_2r(m)) + i conj(g_2r+1(m)) (the mirrored
+        // bin's), g_f = E_f +- O_f with d/dlambda (even f) or 1/R (odd f)
         constexpr int PER = 4;
         const int tot = (L + 1) * 4, tot1 = msplit * 4;
         for (int t0 = 0; t0 < tot || t0 < N; t0 += PER * kGridThreads) {
@@ -924,34 +924,29 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
                 const int t = t0 + threadIdx.x + u * kGridThreads;
+                const int te = (t < tot) ? t : tot - 1;
+                const int m = te >> 2, f = te & 3;
+                const bool two = te < tot1;
+                const double2 e = two ? cadd(ve[u], ve1[u]) : ve[u];
+                const double2 o = two ? cadd(vo[u], vo1[u]) : vo[u];
+                double2 gv = hs ? csub(e, o) : cadd(e, o);
+                gv = (f & 1) ? cscale(gv, invR) : dlam_q(gv, m * invR);
+                // fields f and f ^ 1 of one m sit in neighbouring lanes (t0
+                // and the u offsets are multiples of 4): swap them, then the
+                // even lane writes Z1 and the odd one Z2
+                const double2 gp = c2(__shfl_xor(gv.x, 1), __shfl_xor(gv.y, 1));
                 if (t < tot) {
-                    const int m = t >> 2, f = t & 3;
-                    const bool two = t < tot1;
-                    const double2 e = two ? cadd(ve[u], ve1[u]) : ve[u];
-                    const double2 o = two ? cadd(vo[u], vo1[u]) : vo[u];
-                    double2 gv = hs ? csub(e, o) : cadd(e, o);
-                    gv = (f & 1) ? cscale(gv, invR) : dlam_q(gv, m * invR);
-                    // the pair (f, f + 1) is combined after a barrier: even
-                    // fields parked in Z2, odd ones in Z1
-                    if (f & 1)
-                        sZ1[(f >> 1) * (L + 1) + m] = gv;
-                    else
-                        sZ2[(f >> 1) * (L + 1) + m] = gv;
+                    double2 ge = (f & 1) ? gp : gv, go = (f & 1) ? gv : gp;
+                    if (f & 1) {
+                        sZ2[(f >> 1) * (L + 1) + m] = c2(ge.x + go.y, -ge.y + go.x);
+                    } else {
+                        if (m == 0) { ge.y = 0.0; go.y = 0.0; }
+                        sZ1[(f >> 1) * (L + 1) + m] = c2(ge.x - go.y, ge.y + go.x);
+                    }
                 }
                 if (t < N) twl[t] = vt[u];
             }
         }
-    }
-    __syncthreads();
-    // combine the parked fields into Z1/Z2 (one thread per (pair r, m))
-    for (int t = threadIdx.x; t < 2 * (L + 1); t += kGridThreads) {
-        const int r = t / (L + 1), m = t - r * (L + 1);
-        double2 ge = sZ2[r * (L + 1) + m], go = sZ1[r * (L + 1) + m];
-        const double2 z2 = c2(ge.x + go.y, -ge.y + go.x);
-        if (m == 0) { ge.y = 0.0; go.y = 0.0; }
-        const double2 z1 = c2(ge.x - go.y, ge.y + go.x);
-        sZ1[r * (L + 1) + m] = z1;
-        sZ2[r * (L + 1) + m] = z2;
     }
     __syncthreads();
     // inverse stage 1 (radix 8, no twiddles) fed by the packed bins
