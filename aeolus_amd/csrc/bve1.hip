@@ -64,8 +64,8 @@ struct Bve1Args {
     const double* lap;        // -l(l+1)/R^2, l = 0 entry 0
     const double* nulap;      // nu lap'_l
     double f10;               // f at (1, 0)
-    const double2* S;         // [jp][m]; hemi: FN, the north row's unweighted spectrum
-    const double2* D;         // hemi: FS
+    const double2* S;         // [jp][m]; hemi: FN [m][jp], the north row's unweighted spectrum
+    const double2* D;         // hemi: FS [m][jp]
     const double* wrow;       // hemi: nlat row weights (S = w (FN + FS), D = w (FN - FS))
     int hemi, nlat;
     double2* E;               // plane 0: [jp][m][4]
@@ -209,8 +209,9 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     double vw = 0.0;
     if (active) {
         const int tj = (tid < nh) ? tid : nh - 1;
-        vS = a.S[tj * n + m];
-        vD = a.D[tj * n + m];
+        const int sdo = a.hemi ? m * nh + tj : tj * n + m;
+        vS = a.S[sdo];
+        vD = a.D[sdo];
         if (a.hemi) vw = (2 * tj + 1 == a.nlat) ? 0.5 * a.wrow[tj] : a.wrow[tj];
     }
     // stores
@@ -241,7 +242,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     if (active)
         for (int t = tid + kB1Threads; t < nh; t += kB1Threads) {
-            double2 s = a.S[t * n + m], d = a.D[t * n + m];
+            const int sdo = a.hemi ? m * nh + t : t * n + m;
+            double2 s = a.S[sdo], d = a.D[sdo];
             if (a.hemi) {
                 const double w = (2 * t + 1 == a.nlat) ? 0.5 * a.wrow[t] : a.wrow[t];
                 const double2 fn = s, fs = d;

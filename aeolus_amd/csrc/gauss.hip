@@ -867,8 +867,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
 // paired into two complex rows, (a_l + i a_s) and (b_l + i b_s); one inverse
 // FFT each gives the four grid rows, the Jacobian is formed per point and one
 // forward FFT of the real product row gives its Fourier coefficients F(m),
-// m <= mmax, stored unweighted: the north row's into FN[jp][m], the south
-// row's into FS[jp][m].  The single-member stage kernel forms S = w (FN + FS),
+// m <= mmax, stored unweighted: the north row's into FN[m][jp], the south
+// row's into FS[m][jp].  The single-member stage kernel forms S = w (FN + FS),
 // D = w (FN - FS) while staging them (phy/barotropic.py:120-167).
 template <int LGN>
 __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
@@ -1039,8 +1039,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         fft_stage_c<-1, 8, 3, LGN, 0, 1, false>(lds, twl);
         fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
     }
-    double2* F = (hs ? FS : FN) + (size_t)jp * (L + 1);
-    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[m] = lds[lp(m)];
+    // column-major [m][jp]: the stage kernel of column m reads a contiguous
+    // column (the row-major layout cost it one 128-byte line per 16 bytes)
+    double2* F = (hs ? FS : FN) + jp;
+    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[(size_t)m * g.nh] = lds[lp(m)];
 }
 
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).

@@ -252,8 +252,8 @@ bool bve1_supported(const aeo_model_s* md);
 // Heavy columns a synthesising launch splits into two degree halves (the
 // second half's E/O in d_E1/d_O1; the Jacobian grid stage adds the planes).
 int bve1_msplit(const aeo_plan_s* p);
-// hemi: S/D hold the unweighted row spectra FN/FS of the per-row Jacobian
-// grid kernel (the stage forms S = w (FN + FS), D = w (FN - FS)).
+// hemi: S/D hold the unweighted row spectra FN/FS ([m][jp]) of the per-row
+// Jacobian grid kernel (the stage forms S = w (FN + FS), D = w (FN - FS)).
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
                 bool hemi, hipStream_t s);

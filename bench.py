@@ -55,7 +55,8 @@ def kclass(cls: int, nslices: int) -> str:
         return ("bve1_stage_kernel (synthesis only)" if nslices == 0
                 else "leg_stage_kernel<KindBve, EpiNoneS> (synthesis only)")
     if cls == 1:
-        return "grid_jac_p2_kernel (row-pair FFTs + Jacobian)"
+        return ("grid_jac_hemi_kernel (per-row FFTs + Jacobian)" if nslices == 0
+                else "grid_jac_p2_kernel (row-pair FFTs + Jacobian)")
     return ("bve1_stage_kernel (analysis + RK4 epilogue + next synthesis)" if nslices == 0
             else "leg_stage_kernel<KindBve, EpiBveS> (analysis + RK4 epilogue + next synthesis)")
 
@@ -150,13 +151,24 @@ def _profiled(model, cls, run, steps):
     return ms, tot.value / max(cnt.value, 1), cnt.value
 
 
+def _split_cols(L, cus=256):
+    """Columns the single-member FP64 stage splits into two degree halves
+    (bve1_msplit: the CUs left over by one workgroup per column, each half
+    keeping at least 8 degrees)."""
+    n = L + 1
+    return max(0, min(cus - n, n - 16))
+
+
 def _stage_bytes(cls, L, nlat, nslices):
     """Algorithmic HBM bytes of one launch of a BVE stage kernel class:
     tables and intermediates read or written once, state passes.  FP64
     synthesis reads no table rows beyond the recurrence seeds (2 rows per
-    degree block, 4 blocks) and the (a, b) recurrence and C+/C- columns;
-    split-precision synthesis reads its digit table (nslices INT8 digits per
-    entry)."""
+    degree block: 4 blocks per column, 8 in a column split into two degree
+    halves) and the (a, b) recurrence and C+/C- columns; split-precision
+    synthesis reads its digit table (nslices INT8 digits per entry).  In FP64
+    the split columns' E/O arrive in two planes (written by the stage, read
+    by the grid stage) and the grid stage writes the unweighted row spectra
+    FN/FS in place of S/D (the same bytes)."""
     nh = (nlat + 1) // 2
     n = L + 1
     B = (L + 1) * (L + 2) // 2
@@ -164,10 +176,11 @@ def _stage_bytes(cls, L, nlat, nslices):
     Bc = (cut + 1) * (cut + 2) // 2
     spec = n * n * 16
     tab = 8 if nslices == 0 else nslices          # bytes per table entry
-    eo = 2 * 4 * nh * n * 16                      # E/O, 4 fields
-    sd = 2 * 1 * nh * (cut + 1) * 16              # S/D, 1 field, m <= cut
+    ms = _split_cols(L) if nslices == 0 else 0
+    eo = 2 * 4 * nh * (n + ms) * 16               # E/O, 4 fields (+ plane 1 of split columns)
+    sd = 2 * 1 * nh * (cut + 1) * 16              # S/D (or FN/FS), 1 field, m <= cut
     cols = n * n * (16 + 8 + 8)                   # (a, b), C+, C- columns
-    syn = nh * B * tab if nslices else 4 * 2 * nh * n * 8
+    syn = nh * B * tab if nslices else 4 * 2 * nh * (n + ms) * 8
     if cls == 0:
         return syn + cols + spec + eo
     if cls == 1:
