@@ -98,7 +98,10 @@ template <int kB1R>
 __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     extern __shared__ double2 lds[];
     const int L = a.L, n = L + 1, nh = a.nh;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // wave index as a wave-uniform (scalar) value: block bounds and loop
+    // counters derived from it stay in scalar registers
+    const int tid = threadIdx.x, lane = tid & 63,
+              wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // workgroup -> (m, degree half): the split columns first (heaviest first)
     int m, h;
     if ((int)blockIdx.x < 2 * a.msplit) {
@@ -436,17 +439,31 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                     accO[q][f] = c2(pe_mul(p1[q], co.x), pe_mul(p1[q], co.y));
                 }
             }
+            // two degrees per step, software-pipelined: the next pair's
+            // coefficients are read while this pair's products run (the
+            // broadcast LDS reads, not the FMAs, bound this loop)
             int i = i0 + 2;
-            for (; i + 4 <= i1; i += 4) {
-                double2 ab[4], cv[4][4];
+            double2 abn[2], cvn[2][4];
+            auto load2 = [&](int ii) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ab[u] = rcc[m + i + u];
+                for (int u = 0; u < 2; ++u) {
+                    abn[u] = rcc[m + ii + u];
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) cv[u][f] = cols[f * K + i + u];
+                    for (int f = 0; f < 4; ++f) cvn[u][f] = cols[f * K + ii + u];
                 }
+            };
+            if (i + 2 <= i1) load2(i);
+            for (; i + 2 <= i1; i += 2) {
+                double2 ab[2], cv[2][4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < 2; ++u) {
+                    ab[u] = abn[u];
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) cv[u][f] = cvn[u][f];
+                }
+                if (i + 4 <= i1) load2(i + 2);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
 #pragma unroll
                     for (int q = 0; q < kB1R; ++q) {
                         const double p = rec_step(ab[u], x[q], p1[q], p2[q]);
@@ -461,7 +478,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                     }
                 }
             }
-            for (; i < i1; ++i) {       // the block's last degrees (fewer than a group)
+            for (; i < i1; ++i) {       // the block's last degree (odd count)
                 const double2 ab = rcc[m + i];
                 const bool odd = (i - i0) & 1;
                 double2 c[4];
