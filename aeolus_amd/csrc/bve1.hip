@@ -67,9 +67,10 @@ struct Bve1Args {
     const double2* S;         // [jp][m]; hemi: FN [m][jp], the north row's unweighted spectrum
     const double2* D;         // hemi: FS [m][jp]
     const double* wrow;       // hemi: nlat row weights (S = w (FN + FS), D = w (FN - FS))
+                              // hemi also writes the synthesis as row sums (E + O, E - O)
     int hemi, nlat;
-    double2* E;               // plane 0: [jp][m][4]
-    double2* O;
+    double2* E;               // plane 0: [jp][m][4]; hemi: the north row sums E + O
+    double2* O;               // hemi: the south row sums E - O
     double2* E1;              // plane 1 (degree half 1 of the split columns): [jp][m][4]
     double2* O1;
     const double2* ys;        // stage input state, internal [m][l]
@@ -543,8 +544,15 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                     const int o = (jp * n + m) * 4;
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        Ep[o + f] = cadd(accE[q][f], slot(0, 0, f, q));
-                        Op[o + f] = cadd(accO[q][f], slot(0, 1, f, q));
+                        const double2 ev = cadd(accE[q][f], slot(0, 0, f, q));
+                        const double2 od = cadd(accO[q][f], slot(0, 1, f, q));
+                        if (a.hemi) {            // the row values' spectra N = E + O, S = E - O
+                            Ep[o + f] = cadd(ev, od);
+                            Op[o + f] = csub(ev, od);
+                        } else {
+                            Ep[o + f] = ev;
+                            Op[o + f] = od;
+                        }
                     }
                 }
             }

@@ -863,7 +863,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
 
 // Single-member Jacobian grid stage, one workgroup per latitude row (both
 // rows of a pair are separate workgroups: 2 nh workgroups fill the chip where
-// the row-pair kernel leaves half of it idle).  The four fields of the row are
+// the row-pair kernel leaves half of it idle).  Its input is the row sums the
+// single-member stage kernel writes for it: E + O (north rows) in E, E - O
+// (south rows) in O, each workgroup reading only its own.  The four fields of the row are
 // paired into two complex rows, (a_l + i a_s) and (b_l + i b_s); one inverse
 // FFT each gives the four grid rows, the Jacobian is formed per point and one
 // forward FFT of the real product row gives its Fourier coefficients F(m),
@@ -906,19 +908,21 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         // bin's), g_f = E_f +- O_f with d/dlambda (even f) or 1/R (odd f)
         constexpr int PER = 4;
         const int tot = (L + 1) * 4, tot1 = msplit * 4;
+        // this row's sums g = E + O (north, in E) or E - O (south, in O), the
+        // split columns' second plane added
+        const double2* G = hs ? O : E;
+        const double2* G1 = hs ? O1 : E1;
         for (int t0 = 0; t0 < tot || t0 < N; t0 += PER * kGridThreads) {
-            double2 ve[PER], vo[PER], vt[PER], ve1[PER], vo1[PER];
+            double2 vg[PER], vt[PER], vg1[PER];
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
                 const int t = t0 + threadIdx.x + u * kGridThreads;
                 const int te = (t < tot) ? t : tot - 1;
-                ve[u] = E[eo + te];
-                vo[u] = O[eo + te];
+                vg[u] = G[eo + te];
                 vt[u] = g.fft.tw[(t < N) ? t : N - 1];
                 if (tot1 > 0) {
                     const int t1 = (t < tot1) ? t : tot1 - 1;
-                    ve1[u] = E1[eo + t1];
-                    vo1[u] = O1[eo + t1];
+                    vg1[u] = G1[eo + t1];
                 }
             }
 #pragma unroll
@@ -926,10 +930,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
                 const int t = t0 + threadIdx.x + u * kGridThreads;
                 const int te = (t < tot) ? t : tot - 1;
                 const int m = te >> 2, f = te & 3;
-                const bool two = te < tot1;
-                const double2 e = two ? cadd(ve[u], ve1[u]) : ve[u];
-                const double2 o = two ? cadd(vo[u], vo1[u]) : vo[u];
-                double2 gv = hs ? csub(e, o) : cadd(e, o);
+                double2 gv = (te < tot1) ? cadd(vg[u], vg1[u]) : vg[u];
                 gv = (f & 1) ? cscale(gv, invR) : dlam_q(gv, m * invR);
                 // fields f and f ^ 1 of one m sit in neighbouring lanes (t0
                 // and the u offsets are multiples of 4): swap them, then the
@@ -1275,7 +1276,7 @@ int gauss_model_prime(aeo_model_s* md, const double2* y, hipStream_t s) {
     md->prof.arm(kClsSynthesis);
     if (bve1_supported(md))
         return bve1_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true,
-                           false, s);
+                           bve1_hemi(md), s);
     if (md->mb == 0)
         return gemm_stage_launch(md, y, nullptr, nullptr, nullptr, -1, 0.0, nullptr, false, true,
                                  s);
