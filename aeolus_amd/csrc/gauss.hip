@@ -984,13 +984,16 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     }
     __syncthreads();
     const double q = 1.0 / (c * c);
-    if constexpr (LGN % 3 == 0) {
-        // radix-8 stages only: the last inverse stage of lane j leaves both
-        // rows' values at the points j + N/8 q, exactly the inputs of the
-        // first forward stage's lane j, so the last inverse stage, the
-        // Jacobian at those points and the first forward stage run in
-        // registers (the arithmetic of the staged path, no LDS round trip)
-        static_assert(LGN == 6 || LGN == 9, "radix-8 rows of 64 or 512 points");
+    if constexpr (LGN >= 6) {
+        // inverse stages radix 8, then 2^(LGN - 6) (none for 64 points, 2, 4
+        // or 8 for 128, 256, 512), then a last radix-8 stage: that stage
+        // leaves lane j both rows' values at the points j + N/8 q, exactly
+        // the inputs of the first (radix-8) forward stage's lane j, so the
+        // last inverse stage, the Jacobian at those points and the first
+        // forward stage run in registers (no LDS round trip)
+        static_assert(LGN <= 9, "rows of at most 512 points");
+        if constexpr (LGN == 7) fft_stage_c<+1, 2, 1, LGN, 3, 2, false>(lds, twl);
+        if constexpr (LGN == 8) fft_stage_c<+1, 4, 2, LGN, 3, 2, false>(lds, twl);
         if constexpr (LGN == 9) fft_stage_c<+1, 8, 3, LGN, 3, 2, false>(lds, twl);
         constexpr int LS = LGN - 3;                  // the last stage's LGNS (N/8 = 2^LS lanes)
         const int jl = threadIdx.x;
