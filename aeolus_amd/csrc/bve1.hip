@@ -20,9 +20,11 @@
 //     1; the grid stage adds the planes in the fixed order plane 0 + plane 1).
 //     No analysis-table row is loaded twice and no workgroup waits on another;
 //   * analysis lanes cover degrees l and l + 64 (equal parity, so one LDS
-//     read of S or D serves both), synthesis lanes two latitudes j and j + 64
-//     (one broadcast read of a coefficient serves both): the per-degree LDS
-//     traffic, which bounds both loops, is halved.
+//     read of S or D serves both); in the synthesis each half-wave takes two
+//     of the four fields and each lane four latitudes j + 32 q (one LDS read
+//     of a coefficient serves four recurrences), and the recurrence pairs are
+//     scalar loads: the per-degree LDS traffic, which bounds both loops, is
+//     cut from five wave-wide reads to two.
 // Per workgroup: analysis of column m (lanes over degree, waves over
 // latitude, reduced over waves in LDS), tendency -J + nu lap' zeta (+ forcing)
 // with row 0 pinned and the RK4 stage update, then the four synthesis fields
@@ -94,9 +96,12 @@ __host__ __device__ inline int bve1_split_point(int K, int q8) {
     return s < 2 ? 2 : (s > hi ? hi : s);
 }
 
-// kB1R: latitudes per synthesis lane (j + 64 q, q < kB1R): 2 when nh > 64.
-template <int kB1R>
-__global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
+// kR: latitudes per synthesis lane (g + 32 q, q < kR, g = lane % 32): 4 when
+// nh > 64.  recg: the [m][l] recurrence pairs (a.rec) as a restrict pointer,
+// so the synthesis reads them with wave-uniform scalar loads.
+template <int kR>
+__global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
+                                                                const double2* __restrict__ recg) {
     extern __shared__ double2 lds[];
     const int L = a.L, n = L + 1, nh = a.nh;
     // wave index as a wave-uniform (scalar) value: block bounds and loop
@@ -133,8 +138,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     // analysis (S/D, analysed column, wave partials) and later by the
     // synthesis tree
     double2* st = lds;                           // [n] state column (stage input, then next state)
-    double2* rcc = st + n;                       // [n] recurrence pairs of m
-    double2* cols = rcc + n;                     // [4][K] synthesis fields
+    double2* cols = st + n;                      // [4][K] synthesis fields
     double* cpc = reinterpret_cast<double*>(cols + 4 * n);   // [n] C+ of m
     double* cmc = cpc + n;                       // [n] C- of m
     double* lapc = cmc + n;                      // [n] lap
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     double2* X = scratch;                        // [2][nh + 1]
     double2* accl = X + 2 * nhx;                 // [n] analysed column
     double2* red = accl + n;                     // [8][2][64] analysis wave partials
-    double2* sred = scratch;                     // [NB/2][2 parity][4][kB1R][64] synthesis tree
+    double2* sred = scratch;                     // [NB/2][2 parity][2][kR][64] synthesis tree
 
     // ---- prologue: all independent global loads first
     double pa[2][kB1Pre];
@@ -182,11 +186,12 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     const int Kb = ((Kd + kB1SynWaves - 1) / kB1SynWaves + 1) & ~1;
     const int i0 = d0 + kb * Kb, i1 = (i0 + Kb < d1) ? i0 + Kb : d1;
     const bool syn_wave = a.syn && wave < kB1SynWaves;
-    double sx[kB1R], sp0[kB1R], sp1[kB1R];
+    const int g = lane & 31;                     // synthesis latitude group of the lane
+    double sx[kR], sp0[kR], sp1[kR];
 #pragma unroll
-    for (int q = 0; q < kB1R; ++q) {
+    for (int q = 0; q < kR; ++q) {
         sx[q] = sp0[q] = sp1[q] = 0.0;
-        const int jp = lane + 64 * q;
+        const int jp = g + 32 * q;
         if (syn_wave && jp < nh && i0 < i1) {
             const double* P = a.Ps + boff + jp;
             sx[q] = a.xn[jp];
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         }
     }
     // column data (thread t <-> degree l = t) and S/D (thread t <-> latitude t)
-    double2 vs = c2(0.0, 0.0), vr = c2(0.0, 0.0), vy0 = c2(0.0, 0.0), vks = c2(0.0, 0.0);
+    double2 vs = c2(0.0, 0.0), vy0 = c2(0.0, 0.0), vks = c2(0.0, 0.0);
     double2 vf = c2(0.0, 0.0), vS = c2(0.0, 0.0), vD = c2(0.0, 0.0);
     double vcp = 0.0, vcm = 0.0, vlap = 0.0, vnl = 0.0;
     const int tl = (tid < n) ? tid : n - 1;
@@ -203,7 +208,6 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     vcp = a.CpT[col + tl];
     vcm = a.CmT[col + tl];
     vlap = a.lap[tl];
-    if (a.syn) vr = a.rec[col + tl];
     if (a.ana) {
         vnl = a.nulap[tl];
         if (a.stage >= 0) vy0 = a.y0[col + tl];
@@ -221,7 +225,6 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     // stores
     if (tid < n) {
         st[tid] = vs;
-        rcc[tid] = vr;
         cpc[tid] = vcp;
         cmc[tid] = vcm;
         lapc[tid] = vlap;
@@ -238,7 +241,6 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     for (int t = tid + kB1Threads; t < n; t += kB1Threads) {    // n > 512 (L > 511)
         st[t] = a.ys[col + t];
-        rcc[t] = a.syn ? a.rec[col + t] : c2(0.0, 0.0);
         cpc[t] = a.CpT[col + t];
         cmc[t] = a.CmT[col + t];
         lapc[t] = a.lap[t];
@@ -398,25 +400,29 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
     }
     __syncthreads();
 
-    // ---- Pbar recurrence synthesis: wave = degree block kb of [d0, d1),
-    // lane = latitudes jbase + lane + 64 q (q < R)
+    // ---- Pbar recurrence synthesis: wave = degree block kb of [d0, d1);
+    // half-wave hf = lane / 32 takes fields 2 hf, 2 hf + 1 ((psi, S psi) or
+    // (eta, S eta)), lane latitudes jbase + g + 32 q (q < kR)
     auto slot = [&](int g2, int par2, int f, int q) -> double2& {
-        return sred[(((g2 * 2 + par2) * 4 + f) * kB1R + q) * 64 + lane];
+        return sred[(((g2 * 2 + par2) * 2 + f) * kR + q) * 64 + lane];
     };
     double2* Ep = (h == 1) ? a.E1 : a.E;
     double2* Op = (h == 1) ? a.O1 : a.O;
-    for (int jbase = 0; jbase < nh; jbase += 64 * kB1R) {
-        double2 accE[kB1R][4], accO[kB1R][4];
+    const int hf = lane >> 5;
+    const double2* cfh = cols + 2 * hf * K;      // [2][K] this half-wave's fields
+    const double2* rg = recg + col + m;          // recurrence pair of degree i at rg[i]
+    for (int jbase = 0; jbase < nh; jbase += 32 * kR) {
+        double2 accE[kR][2], accO[kR][2];
 #pragma unroll
-        for (int q = 0; q < kB1R; ++q)
+        for (int q = 0; q < kR; ++q)
 #pragma unroll
-            for (int f = 0; f < 4; ++f) accE[q][f] = accO[q][f] = c2(0.0, 0.0);
+            for (int f = 0; f < 2; ++f) accE[q][f] = accO[q][f] = c2(0.0, 0.0);
         if (syn_wave && i0 < i1) {
             const bool first = (jbase == 0);
-            double x[kB1R], p1[kB1R], p2[kB1R];
+            double x[kR], p1[kR], p2[kR];
 #pragma unroll
-            for (int q = 0; q < kB1R; ++q) {
-                const int jp = jbase + lane + 64 * q;
+            for (int q = 0; q < kR; ++q) {
+                const int jp = jbase + g + 32 * q;
                 if (first) {
                     x[q] = sx[q];
                     p2[q] = sp0[q];
@@ -430,71 +436,83 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
                 }
             }
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const double2* cf = cols + f * K;
+            for (int f = 0; f < 2; ++f) {
+                const double2* cf = cfh + f * K;
                 const double2 ce = cf[i0];
                 const double2 co = cf[(i0 + 1 < i1) ? i0 + 1 : i0];
 #pragma unroll
-                for (int q = 0; q < kB1R; ++q) {
+                for (int q = 0; q < kR; ++q) {
                     accE[q][f] = c2(pe_mul(p2[q], ce.x), pe_mul(p2[q], ce.y));
                     accO[q][f] = c2(pe_mul(p1[q], co.x), pe_mul(p1[q], co.y));
                 }
             }
-            // two degrees per step, software-pipelined: the next pair's
-            // coefficients are read while this pair's products run (the
-            // broadcast LDS reads, not the FMAs, bound this loop)
-            int i = i0 + 2;
-            double2 abn[2], cvn[2][4];
-            auto load2 = [&](int ii) {
+            // two degrees per step, double-buffered: the next pair's
+            // coefficients are read while this pair's products run (the LDS
+            // reads, not the FMAs, bound this loop); the recurrence pairs
+            // come through the idle vector-memory path (buffer loads of a
+            // wave-uniform address, counted apart from the LDS reads)
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<double2*>(rg), (short)0, K * (int)sizeof(double2), 0x00020000);
+            struct Pair {
+                double2 ab[2], cv[2][2];
+            };
+            auto load2 = [&](Pair& b, int ii) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    abn[u] = rcc[m + ii + u];
+                    b.ab[u] = __builtin_bit_cast(
+                        double2, __builtin_amdgcn_raw_buffer_load_b128(rr, (ii + u) * 16, 0, 0));
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) cvn[u][f] = cols[f * K + ii + u];
+                    for (int f = 0; f < 2; ++f) b.cv[u][f] = cfh[f * K + ii + u];
                 }
             };
-            if (i + 2 <= i1) load2(i);
-            for (; i + 2 <= i1; i += 2) {
-                double2 ab[2], cv[2][4];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    ab[u] = abn[u];
-#pragma unroll
-                    for (int f = 0; f < 4; ++f) cv[u][f] = cvn[u][f];
-                }
-                if (i + 4 <= i1) load2(i + 2);
+            auto step2 = [&](const Pair& b) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
 #pragma unroll
-                    for (int q = 0; q < kB1R; ++q) {
-                        const double p = rec_step(ab[u], x[q], p1[q], p2[q]);
+                    for (int q = 0; q < kR; ++q) {
+                        const double p = rec_step(b.ab[u], x[q], p1[q], p2[q]);
                         p2[q] = p1[q];
                         p1[q] = p;
 #pragma unroll
-                        for (int f = 0; f < 4; ++f) {
+                        for (int f = 0; f < 2; ++f) {
                             double2& acc = (u & 1) ? accO[q][f] : accE[q][f];
-                            acc.x = fma(p, cv[u][f].x, acc.x);
-                            acc.y = fma(p, cv[u][f].y, acc.y);
+                            acc.x = fma(p, b.cv[u][f].x, acc.x);
+                            acc.y = fma(p, b.cv[u][f].y, acc.y);
                         }
                     }
                 }
+            };
+            // (the reads run at most two degrees past the block: in-bounds
+            // LDS, buffer reads past the column return 0; the values go
+            // unused); the scheduling barriers keep each buffer's reads after
+            // its last use, so the two buffers stay two register sets
+            int i = i0 + 2;
+            Pair pA, pB;
+            load2(pA, i);
+            for (; i + 4 <= i1; i += 4) {
+                load2(pB, i + 2);
+                step2(pA);
+                __builtin_amdgcn_sched_barrier(0);
+                load2(pA, i + 4);
+                step2(pB);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            for (; i < i1; ++i) {       // the block's last degree (odd count)
-                const double2 ab = rcc[m + i];
-                const bool odd = (i - i0) & 1;
-                double2 c[4];
+            if (i + 2 <= i1) {
+                step2(pA);
+                i += 2;
+            }
+            if (i < i1) {               // the block's last degree (odd count)
+                const double2 ab = rg[i];
+                double2 c[2];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) c[f] = cols[f * K + i];
+                for (int f = 0; f < 2; ++f) c[f] = cfh[f * K + i];
 #pragma unroll
-                for (int q = 0; q < kB1R; ++q) {
+                for (int q = 0; q < kR; ++q) {
                     const double p = rec_step(ab, x[q], p1[q], p2[q]);
-                    p2[q] = p1[q];
-                    p1[q] = p;
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
-                        double2& acc = odd ? accO[q][f] : accE[q][f];
-                        acc.x = fma(p, c[f].x, acc.x);
-                        acc.y = fma(p, c[f].y, acc.y);
+                    for (int f = 0; f < 2; ++f) {
+                        accE[q][f].x = fma(p, c[f].x, accE[q][f].x);
+                        accE[q][f].y = fma(p, c[f].y, accE[q][f].y);
                     }
                 }
             }
@@ -506,9 +524,9 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         for (int half = kB1SynWaves / 2; half >= 2; half /= 2) {
             if (syn_wave && kb >= half && kb < 2 * half) {
 #pragma unroll
-                for (int q = 0; q < kB1R; ++q)
+                for (int q = 0; q < kR; ++q)
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
+                    for (int f = 0; f < 2; ++f) {
                         slot(kb - half, 0, f, q) = accE[q][f];
                         slot(kb - half, 1, f, q) = accO[q][f];
                     }
@@ -516,9 +534,9 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
             __syncthreads();
             if (syn_wave && kb < half) {
 #pragma unroll
-                for (int q = 0; q < kB1R; ++q)
+                for (int q = 0; q < kR; ++q)
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
+                    for (int f = 0; f < 2; ++f) {
                         accE[q][f] = cadd(accE[q][f], slot(kb, 0, f, q));
                         accO[q][f] = cadd(accO[q][f], slot(kb, 1, f, q));
                     }
@@ -527,10 +545,10 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         }
         if (syn_wave && kb < 2) {
 #pragma unroll
-            for (int q = 0; q < kB1R; ++q)
+            for (int q = 0; q < kR; ++q)
                 if ((q & 1) != kb)
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
+                    for (int f = 0; f < 2; ++f) {
                         slot(0, 0, f, q) = accE[q][f];
                         slot(0, 1, f, q) = accO[q][f];
                     }
@@ -538,12 +556,12 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
         __syncthreads();
         if (syn_wave && kb < 2) {
 #pragma unroll
-            for (int q = 0; q < kB1R; ++q) {
-                const int jp = jbase + lane + 64 * q;
+            for (int q = 0; q < kR; ++q) {
+                const int jp = jbase + g + 32 * q;
                 if ((q & 1) == kb && jp < nh) {
-                    const int o = (jp * n + m) * 4;
+                    const int o = (jp * n + m) * 4 + 2 * hf;
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
+                    for (int f = 0; f < 2; ++f) {
                         const double2 ev = cadd(accE[q][f], slot(0, 0, f, q));
                         const double2 od = cadd(accO[q][f], slot(0, 1, f, q));
                         if (a.hemi) {            // the row values' spectra N = E + O, S = E - O
@@ -562,10 +580,10 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a) {
 }
 
 static size_t bve1_lds(int L, int nh) {
-    const size_t n = L + 1, R = (nh > 64) ? 2 : 1;
+    const size_t n = L + 1, R = (nh > 64) ? 4 : 2;
     const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * 2 * 64,
-                 syn = (kB1SynWaves / 2) * 2 * 4 * R * 64;
-    return sizeof(double2) * (6 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
+                 syn = (kB1SynWaves / 2) * 2 * 2 * R * 64;
+    return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 
 // Whether the single-member path applies to this model and plan.
@@ -640,15 +658,16 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.cst = (stage == 2) ? dt : 0.5 * dt;
     a.dt6 = dt / 6.0;
     const size_t lds = bve1_lds(L, p->nh);
-    const void* kern = (p->nh > 64) ? (const void*)bve1_stage_kernel<2>
-                                    : (const void*)bve1_stage_kernel<1>;
+    const void* kern = (p->nh > 64) ? (const void*)bve1_stage_kernel<4>
+                                    : (const void*)bve1_stage_kernel<2>;
     if (lds > 65536)
         AEO_CHECK_HIP(
             hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const double2* recg = p->d_rec;
     if (p->nh > 64)
-        launch_k(bve1_stage_kernel<2>, dim3(n + ms), dim3(kB1Threads), lds, s, a);
+        launch_k(bve1_stage_kernel<4>, dim3(n + ms), dim3(kB1Threads), lds, s, a, recg);
     else
-        launch_k(bve1_stage_kernel<1>, dim3(n + ms), dim3(kB1Threads), lds, s, a);
+        launch_k(bve1_stage_kernel<2>, dim3(n + ms), dim3(kB1Threads), lds, s, a, recg);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }
