@@ -711,6 +711,72 @@ __device__ inline void fft_rest_c(double2* buf, const double2* tw) {
     }
 }
 
+// One radix-R Stockham stage from rows `src` to rows `dst` (two LDS
+// buffers): no barrier between the reads and the writes, one after the
+// writes (the in-place fft_stage_c needs both).  Full twiddle table.
+template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS>
+__device__ inline void fft_stage_pp(const double2* src, double2* dst, const double2* tw) {
+    constexpr int N = 1 << LGN, LGNB = LGN - LGR, NB = 1 << LGNB, TOT = NROWS * NB;
+    constexpr int RS = N + (N >> 3), LGSPAN = LGN - LGNS - LGR;
+    constexpr int MAXB = (TOT + kGridThreads - 1) / kGridThreads;
+    double2 v[MAXB][R], w[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < TOT) {
+            const int row = t >> LGNB, j = t & (NB - 1), k = j & ((1 << LGNS) - 1);
+            const double2* rb = src + row * RS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = rb[lp(j + r * NB)];
+            if (LGNS > 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[b][r] = tw[(r * k) << LGSPAN];
+            }
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int t = threadIdx.x + b * kGridThreads;
+        if (t < TOT) {
+            const int row = t >> LGNB, j = t & (NB - 1), k = j & ((1 << LGNS) - 1);
+            if (LGNS > 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    double2 wr = w[b][r];
+                    if (DIR > 0) wr.y = -wr.y;
+                    const double2 x = v[b][r];
+                    v[b][r] = c2(fma(x.x, wr.x, -(x.y * wr.y)), fma(x.x, wr.y, x.y * wr.x));
+                }
+            }
+            butterfly<DIR, R>(v[b]);
+            double2* rb = dst + row * RS;
+            const int base = ((j >> LGNS) << (LGNS + LGR)) + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) rb[lp(base + (q << LGNS))] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// fft_rest_c between two buffers: stage s reads a and writes b, the next
+// reads b and writes a; returns the buffer that holds the result.
+template <int DIR, int LGN, int LGNS, int NROWS>
+__device__ inline double2* fft_rest_pp(double2* a, double2* b, const double2* tw) {
+    if constexpr (LGN - LGNS >= 3) {
+        fft_stage_pp<DIR, 8, 3, LGN, LGNS, NROWS>(a, b, tw);
+        return fft_rest_pp<DIR, LGN, LGNS + 3, NROWS>(b, a, tw);
+    } else if constexpr (LGN - LGNS == 2) {
+        fft_stage_pp<DIR, 4, 2, LGN, LGNS, NROWS>(a, b, tw);
+        return b;
+    } else if constexpr (LGN - LGNS == 1) {
+        fft_stage_pp<DIR, 2, 1, LGN, LGNS, NROWS>(a, b, tw);
+        return b;
+    } else {
+        return a;
+    }
+}
+
 // Packed spectrum bin b of one row (pack_spectrum, case n > 2L), in two
 // parts so a caller can issue the LDS reads of many bins before combining:
 // packed_bin_load gathers E/O at m1 = b and m2 = n - b, packed_bin_combine
@@ -896,9 +962,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     const double invR = 1.0 / R;
     const double c = clamp_coslat(g.coslat[j]);
     const size_t eo = (size_t)jp * (L + 1) * 4;
-    // LDS: rows [2][RS] | twiddle table [N] | Z1, Z2 [2][L+1] each
-    double2* twl = lds + 2 * RS;
-    double2* sZ1 = twl + N;
+    // LDS: rows A [2][RS] | rows B [2][RS] | twiddle table [N]; the FFT
+    // stages alternate between A and B (one barrier per stage); Z1, Z2
+    // [2][L+1] each (4 (L + 1) <= 2 RS as N > 2L) sit in B until the first
+    // stage has read them into A
+    double2* rowA = lds;
+    double2* rowB = lds + 2 * RS;
+    double2* twl = lds + 4 * RS;
+    double2* sZ1 = rowB;
     double2* sZ2 = sZ1 + 2 * (L + 1);
     {
         // prologue: every load first (E/O of the 4 fields, plane 1 of the
@@ -977,13 +1048,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
                 v[r] = z;
             }
             butterfly<+1, 8>(v);
-            double2* rb = lds + row * RS;
+            double2* rb = rowA + row * RS;
 #pragma unroll
             for (int q = 0; q < 8; ++q) rb[lp(8 * jj + q)] = v[q];
         }
     }
     __syncthreads();
     const double q = 1.0 / (c * c);
+    double2* fin = rowA;                             // the rows holding the result
     if constexpr (LGN >= 6) {
         // inverse stages radix 8, then 2^(LGN - 6) (none for 64 points, 2, 4
         // or 8 for 128, 256, 512), then a last radix-8 stage: that stage
@@ -992,9 +1064,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         // last inverse stage, the Jacobian at those points and the first
         // forward stage run in registers (no LDS round trip)
         static_assert(LGN <= 9, "rows of at most 512 points");
-        if constexpr (LGN == 7) fft_stage_c<+1, 2, 1, LGN, 3, 2, false>(lds, twl);
-        if constexpr (LGN == 8) fft_stage_c<+1, 4, 2, LGN, 3, 2, false>(lds, twl);
-        if constexpr (LGN == 9) fft_stage_c<+1, 8, 3, LGN, 3, 2, false>(lds, twl);
+        if constexpr (LGN == 7) fft_stage_pp<+1, 2, 1, LGN, 3, 2>(rowA, rowB, twl);
+        if constexpr (LGN == 8) fft_stage_pp<+1, 4, 2, LGN, 3, 2>(rowA, rowB, twl);
+        if constexpr (LGN == 9) fft_stage_pp<+1, 8, 3, LGN, 3, 2>(rowA, rowB, twl);
+        double2* src = (LGN == 6) ? rowA : rowB;     // the last inverse stage's input rows
+        double2* dst = (LGN == 6) ? rowB : rowA;
         constexpr int LS = LGN - 3;                  // the last stage's LGNS (N/8 = 2^LS lanes)
         const int jl = threadIdx.x;
         double2 v[8];
@@ -1002,8 +1076,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
             double2 xa[8], xb[8], wt[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                xa[r] = lds[lp(jl + r * NB)];
-                xb[r] = lds[RS + lp(jl + r * NB)];
+                xa[r] = src[lp(jl + r * NB)];
+                xb[r] = src[RS + lp(jl + r * NB)];
                 if (r > 0) wt[r] = twl[r * jl];      // W^(r k), k = jl (span 1)
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -1021,13 +1095,12 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
             butterfly<-1, 8>(v);
         }
         static_assert(LS >= 3, "fused stages need N >= 64");
-        __syncthreads();                             // every row read before row 0 is rewritten
         if (jl < NB) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) lds[lp(8 * jl + r)] = v[r];
+            for (int r = 0; r < 8; ++r) dst[lp(8 * jl + r)] = v[r];
         }
         __syncthreads();
-        fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
+        fin = fft_rest_pp<-1, LGN, 3, 1>(dst, src, twl);
     } else {
         fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
         // the Jacobian row (real) into row 0 as a complex row with zero
@@ -1046,7 +1119,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     // column-major [m][jp]: the stage kernel of column m reads a contiguous
     // column (the row-major layout cost it one 128-byte line per 16 bytes)
     double2* F = (hs ? FS : FN) + jp;
-    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[(size_t)m * g.nh] = lds[lp(m)];
+    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[(size_t)m * g.nh] = fin[lp(m)];
 }
 
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).
@@ -1147,7 +1220,7 @@ static HemiKernel hemi_kernel_for(const aeo_plan_s* p) {
     }
 }
 static size_t hemi_lds(const aeo_plan_s* p) {
-    return sizeof(double2) * (2 * (size_t)lds_row_stride(p->nlon) + p->nlon + 4 * (size_t)(p->L + 1));
+    return sizeof(double2) * (4 * (size_t)lds_row_stride(p->nlon) + p->nlon);
 }
 
 // Rows per workgroup of the plain transforms' grid kernels for a batch of nf
