@@ -963,88 +963,73 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     const double c = clamp_coslat(g.coslat[j]);
     const size_t eo = (size_t)jp * (L + 1) * 4;
     // LDS: rows A [2][RS] | rows B [2][RS] | twiddle table [N]; the FFT
-    // stages alternate between A and B (one barrier per stage); Z1, Z2
-    // [2][L+1] each (4 (L + 1) <= 2 RS as N > 2L) sit in B until the first
-    // stage has read them into A
+    // stages alternate between A and B (one barrier per stage)
     double2* rowA = lds;
     double2* rowB = lds + 2 * RS;
     double2* twl = lds + 4 * RS;
-    double2* sZ1 = rowB;
-    double2* sZ2 = sZ1 + 2 * (L + 1);
     {
-        // prologue: every load first (E/O of the 4 fields, plane 1 of the
-        // split columns, the twiddle table), then the packed half-spectra
-        // Z1[r][m] = g_2r(m) + i g_2r+1(m) (the bin-m term, real at m = 0)
-        // and Z2[r][m] = conj(g_2r(m)) + i conj(g_2r+1(m)) (the mirrored
-        // bin's), g_f = E_f +- O_f with d/dlambda (even f) or 1/R (odd f)
-        constexpr int PER = 4;
-        const int tot = (L + 1) * 4, tot1 = msplit * 4;
-        // this row's sums g = E + O (north, in E) or E - O (south, in O), the
-        // split columns' second plane added
+        // prologue and inverse stage 1 (radix 8, no twiddles) in registers:
+        // lane (row, jj) loads, for each of its bins b = jj + r N/8, the two
+        // fields 2 row, 2 row + 1 of the one m the bin takes (N > 2L: the
+        // bin-m term m = b when r < 4, the mirrored bin's m = N - b when
+        // r >= 4) and forms the packed bin: Z1 = g_2r(m) + i g_2r+1(m) (real
+        // at m = 0) or Z2 = conj(g_2r(m)) + i conj(g_2r+1(m)), g_f = this
+        // row's sum E + O (north, in E) or E - O (south, in O), the split
+        // columns' second plane added, with d/dlambda (even f) or 1/R (odd
+        // f); every thread also loads its share of the twiddle table
         const double2* G = hs ? O : E;
         const double2* G1 = hs ? O1 : E1;
-        for (int t0 = 0; t0 < tot || t0 < N; t0 += PER * kGridThreads) {
-            double2 vg[PER], vt[PER], vg1[PER];
+        constexpr int TWPER = (N + kGridThreads - 1) / kGridThreads;
+        double2 vt[TWPER];
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int t = t0 + threadIdx.x + u * kGridThreads;
-                const int te = (t < tot) ? t : tot - 1;
-                vg[u] = G[eo + te];
-                vt[u] = g.fft.tw[(t < N) ? t : N - 1];
-                if (tot1 > 0) {
-                    const int t1 = (t < tot1) ? t : tot1 - 1;
-                    vg1[u] = G1[eo + t1];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int t = t0 + threadIdx.x + u * kGridThreads;
-                const int te = (t < tot) ? t : tot - 1;
-                const int m = te >> 2, f = te & 3;
-                double2 gv = (te < tot1) ? cadd(vg[u], vg1[u]) : vg[u];
-                gv = (f & 1) ? cscale(gv, invR) : dlam_q(gv, m * invR);
-                // fields f and f ^ 1 of one m sit in neighbouring lanes (t0
-                // and the u offsets are multiples of 4): swap them, then the
-                // even lane writes Z1 and the odd one Z2
-                const double2 gp = c2(__shfl_xor(gv.x, 1), __shfl_xor(gv.y, 1));
-                if (t < tot) {
-                    double2 ge = (f & 1) ? gp : gv, go = (f & 1) ? gv : gp;
-                    if (f & 1) {
-                        sZ2[(f >> 1) * (L + 1) + m] = c2(ge.x + go.y, -ge.y + go.x);
-                    } else {
-                        if (m == 0) { ge.y = 0.0; go.y = 0.0; }
-                        sZ1[(f >> 1) * (L + 1) + m] = c2(ge.x - go.y, ge.y + go.x);
-                    }
-                }
-                if (t < N) twl[t] = vt[u];
-            }
+        for (int u = 0; u < TWPER; ++u) {
+            const int tt = threadIdx.x + u * kGridThreads;
+            vt[u] = g.fft.tw[(tt < N) ? tt : N - 1];
         }
-    }
-    __syncthreads();
-    // inverse stage 1 (radix 8, no twiddles) fed by the packed bins
-    {
-        double2 v[8], z1[8], z2[8];
         const int t = threadIdx.x;
-        if (t < 2 * NB) {
-            const int row = t / NB, jj = t % NB;
+        const bool first = t < 2 * NB;
+        const int row = first ? t / NB : 0, jj = t % NB;
+        double2 ve[8], vo[8], ve1[8], vo1[8];
+        if (first) {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int b = jj + r * NB;
-                const int m1 = (b <= L) ? b : L;
-                const int m2 = (b > 0 && N - b <= L) ? N - b : L;
-                z1[r] = sZ1[row * (L + 1) + m1];
-                z2[r] = sZ2[row * (L + 1) + m2];
+                const int b = jj + r * NB, mm = (r < 4) ? b : N - b;
+                const int mc = (mm <= L) ? mm : L;
+                const size_t e0 = eo + (size_t)mc * 4 + 2 * row;
+                ve[r] = vo[r] = ve1[r] = vo1[r] = c2(0.0, 0.0);
+                if (mm <= L) {                       // bins past the spectrum load nothing
+                    ve[r] = G[e0];
+                    vo[r] = G[e0 + 1];
+                    if (mc < msplit) {
+                        ve1[r] = G1[e0];
+                        vo1[r] = G1[e0 + 1];
+                    }
+                }
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (t < 2 * NB) {
-            const int row = t / NB, jj = t % NB;
+#pragma unroll
+        for (int u = 0; u < TWPER; ++u) {
+            const int tt = threadIdx.x + u * kGridThreads;
+            if (tt < N) twl[tt] = vt[u];
+        }
+        if (first) {
+            double2 v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int b = jj + r * NB;
+                const int b = jj + r * NB, mm = (r < 4) ? b : N - b;
+                const int mc = (mm <= L) ? mm : L;
+                double2 ge = (mc < msplit) ? cadd(ve[r], ve1[r]) : ve[r];
+                double2 go = (mc < msplit) ? cadd(vo[r], vo1[r]) : vo[r];
+                ge = dlam_q(ge, mc * invR);
+                go = cscale(go, invR);
                 double2 z = c2(0.0, 0.0);
-                if (b <= L) z = z1[r];
-                if (b > 0 && N - b <= L) z = cadd(z, z2[r]);
+                if (r < 4) {
+                    if (mc == 0) { ge.y = 0.0; go.y = 0.0; }
+                    if (mm <= L) z = c2(ge.x - go.y, ge.y + go.x);
+                } else if (mm <= L) {
+                    z = cadd(z, c2(ge.x + go.y, -ge.y + go.x));
+                }
                 v[r] = z;
             }
             butterfly<+1, 8>(v);
