@@ -777,6 +777,60 @@ __device__ inline double2* fft_rest_pp(double2* a, double2* b, const double2* tw
     }
 }
 
+// The last Stockham stage of one row (NROWS 1), its outputs m <= mmax
+// stored straight to global memory, out[m * ostride], instead of LDS.
+template <int DIR, int R, int LGR, int LGN, int LGNS>
+__device__ inline void fft_stage_out(const double2* src, const double2* tw, double2* out,
+                                     int ostride, int mmax) {
+    static_assert(LGNS + LGR == LGN, "the last stage");
+    constexpr int N = 1 << LGN, NB = 1 << (LGN - LGR), LGSPAN = LGN - LGNS - LGR;
+    static_assert(NB <= kGridThreads, "one butterfly per thread");
+    const int j = threadIdx.x;
+    if (j < NB) {
+        double2 v[R], w[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[lp(j + r * NB)];
+        if (LGNS > 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[r] = tw[(r * j) << LGSPAN];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (LGNS > 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                double2 wr = w[r];
+                if (DIR > 0) wr.y = -wr.y;
+                const double2 x = v[r];
+                v[r] = c2(fma(x.x, wr.x, -(x.y * wr.y)), fma(x.x, wr.y, x.y * wr.x));
+            }
+        }
+        butterfly<DIR, R>(v);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int m = j + (q << LGNS);           // j < 2^LGNS: base = j
+            if (m <= mmax) out[(size_t)m * ostride] = v[q];
+        }
+    }
+    (void)N;
+}
+
+// fft_rest_pp of one row whose last stage stores outputs m <= mmax to
+// global memory (fft_stage_out).
+template <int DIR, int LGN, int LGNS>
+__device__ inline void fft_rest_pp_out(double2* a, double2* b, const double2* tw, double2* out,
+                                       int ostride, int mmax) {
+    if constexpr (LGN - LGNS > 3) {
+        fft_stage_pp<DIR, 8, 3, LGN, LGNS, 1>(a, b, tw);
+        fft_rest_pp_out<DIR, LGN, LGNS + 3>(b, a, tw, out, ostride, mmax);
+    } else if constexpr (LGN - LGNS == 3) {
+        fft_stage_out<DIR, 8, 3, LGN, LGNS>(a, tw, out, ostride, mmax);
+    } else if constexpr (LGN - LGNS == 2) {
+        fft_stage_out<DIR, 4, 2, LGN, LGNS>(a, tw, out, ostride, mmax);
+    } else if constexpr (LGN - LGNS == 1) {
+        fft_stage_out<DIR, 2, 1, LGN, LGNS>(a, tw, out, ostride, mmax);
+    }
+}
+
 // Packed spectrum bin b of one row (pack_spectrum, case n > 2L), in two
 // parts so a caller can issue the LDS reads of many bins before combining:
 // packed_bin_load gathers E/O at m1 = b and m2 = n - b, packed_bin_combine
@@ -1085,7 +1139,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
             for (int r = 0; r < 8; ++r) dst[lp(8 * jl + r)] = v[r];
         }
         __syncthreads();
-        fin = fft_rest_pp<-1, LGN, 3, 1>(dst, src, twl);
+        // column-major [m][jp]: the stage kernel of column m reads a
+        // contiguous column (the row-major layout cost it one 128-byte line
+        // per 16 bytes)
+        fft_rest_pp_out<-1, LGN, 3>(dst, src, twl, (hs ? FS : FN) + jp, g.nh, mmax);
+        return;
     } else {
         fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
         // the Jacobian row (real) into row 0 as a complex row with zero
