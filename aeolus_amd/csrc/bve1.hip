@@ -586,11 +586,30 @@ static size_t bve1_lds(int L, int nh) {
     return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 
+// The most degrees any workgroup of a launch analyses: the kernel's two
+// 64-lane analysis slots hold 128 (degrees past them would be left out).
+static int bve1_max_analysed(const aeo_plan_s* p) {
+    const int n = p->L + 1, top = (2 * p->L) / 3, ms = bve1_msplit(p);
+    int mx = 0;
+    for (int m = 0; m <= top && m < n; ++m) {
+        const int K = n - m;
+        int cnt = top - m + 1;                       // unsplit: degrees 0 .. top - m
+        if (m < ms) {
+            const int s = bve1_split_point(K, kB1SplitQ8);
+            const int c0 = ((s < top - m) ? s : top - m) + 1;   // first half: 0 .. min(s, top - m)
+            const int c1 = (top - m) - (s - 1) + 1;             // second: s - 1 .. top - m
+            cnt = (c0 > c1) ? c0 : c1;
+        }
+        if (cnt > mx) mx = cnt;
+    }
+    return mx;
+}
+
 // Whether the single-member path applies to this model and plan.
 bool bve1_supported(const aeo_model_s* md) {
     const aeo_plan_s* p = md->plan;
     return md->kind == kBVE && md->nmem == 1 && md->nslices == 0 && p->kind == kGauss &&
-           bve1_lds(p->L, p->nh) <= 163840;
+           bve1_lds(p->L, p->nh) <= 163840 && bve1_max_analysed(p) <= 128;
 }
 
 static int bve1_cus() {
@@ -625,7 +644,10 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.L = L;
     a.nh = p->nh;
     a.top = (2 * L) / 3;
-    const int ms = syn ? bve1_msplit(p) : 0;
+    // the heavy columns run as two degree halves in every launch (in one
+    // without synthesis too: a whole heavy column can analyse more degrees
+    // than the two slots hold)
+    const int ms = bve1_msplit(p);
     a.msplit = ms;
     a.split_q8 = kB1SplitQ8;
     a.ana = ana ? 1 : 0;

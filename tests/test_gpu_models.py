@@ -131,13 +131,18 @@ def test_swe_rk4_matches_oracle_short():
 
 @pytest.mark.parametrize("L,nlat,nlon,nmem", [(42, 64, 128, 1), (42, 64, 128, 3),
                                                (42, 64, 128, 4), (42, 64, 128, 9),
-                                               (85, 128, 256, 2), (170, 256, 512, 3)])
+                                               (85, 128, 256, 2), (170, 256, 512, 3),
+                                               (200, 320, 640, 2), (255, 384, 768, 2)])
 def test_bve_batch_members_match_single_runs(L, nlat, nlon, nmem):
     """Every member of a batched step (the generic stage kernel) equals the
     single-member step (the dedicated single-member kernel, whose heavy
     columns run as two degree halves whose Legendre sums the grid stage adds)
     to rounding: the per-degree analysis sums are the same, the synthesis sums
-    of a split column are associated per degree half."""
+    of a split column are associated per degree half.  L = 200 takes the
+    single-member kernel's second analysis slot (a half with more than 64
+    analysed degrees) and more than one synthesis latitude pass (nh > 128);
+    L = 255 has columns with more analysed degrees than the single-member
+    kernel's two slots hold."""
     import torch
     from aeolus_amd.physics import BarotropicVorticity
     from aeolus_amd.physics.barotropic import BarotropicBatch
