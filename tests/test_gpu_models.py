@@ -40,7 +40,8 @@ def _planets(kind, L, nlat=0, nlon=0, res=0, R=R_C1, Om=OMEGA):
 @pytest.mark.parametrize("kind,L,nlat,nlon,res,nu", [
     ("latlon", 8, 12, 24, 0, 0.0), ("latlon", 21, 32, 64, 0, 1e5),
     ("latlon", 85, 128, 256, 0, 0.0), ("geodesic", 10, 0, 0, 3, 0.0),
-    ("latlon", 200, 320, 640, 0, 1e5), ("latlon", 230, 352, 704, 0, 0.0)])
+    ("latlon", 200, 320, 640, 0, 1e5), ("latlon", 230, 352, 704, 0, 0.0),
+    ("latlon", 42, 65, 128, 0, 0.0), ("latlon", 85, 129, 200, 0, 0.0)])
 def test_bve_tendency_matches_oracle(kind, L, nlat, nlon, res, nu):
     import torch
     from aeolus_amd.physics import BarotropicState, BarotropicVorticity
@@ -77,6 +78,26 @@ def test_bve_c1_trajectory_matches_oracle():
     assert rel_l2(_np(state.coeffs), y_ref) < 1e-12
     fused = model.rk4(torch.as_tensor(z0, device="cuda"), 600.0, 144)
     assert rel_l2(_np(fused), y_ref) < 1e-12
+
+
+@pytest.mark.parametrize("L,nlat,nlon", [(42, 65, 128), (85, 129, 200), (200, 320, 640)])
+def test_bve_fused_rk4_matches_oracle_short(L, nlat, nlon):
+    """The fused RK4 step (single-member kernel, split heavy columns, the
+    per-row grid kernel on power-of-two rows; an equator row on odd grids)
+    against the oracle's RK4, 6 steps of 300 s, within 1e-12."""
+    import torch
+    from aeolus_amd.physics import BarotropicVorticity
+    from aeolus_amd.run.initial_conditions import spectrum_l15
+    from oracle.models import bve_tendency, rk4_step as o_rk4
+    print(f"case bve fused rk4 L={L} {nlat}x{nlon}", flush=True)
+    pl, op = _planets("latlon", L, nlat, nlon)
+    model = BarotropicVorticity(pl, viscosity=1e5)
+    z0 = spectrum_l15(L, np.random.default_rng(7), rms=5e-5)
+    y_ref = z0.copy()
+    for _ in range(6):
+        y_ref = o_rk4(lambda y: bve_tendency(op, y, nu=1e5), y_ref, 300.0)
+    got = model.rk4(torch.as_tensor(z0, device="cuda"), 300.0, 6)
+    assert rel_l2(_np(got), y_ref) < 1e-12
 
 
 @pytest.mark.parametrize("kind,L,nlat,nlon,res,nu4,topo", [
