@@ -125,7 +125,8 @@ def _tl_step(model, y, d, dt):
             d + (dt / 6.0) * (d1 + 2 * d2 + 2 * d3 + d4))
 
 
-@pytest.mark.parametrize("L,nlat,nlon", [(42, 64, 128), (85, 128, 256), (170, 256, 512)])
+@pytest.mark.parametrize("L,nlat,nlon", [(42, 64, 128), (85, 128, 256), (170, 256, 512),
+                                         (200, 320, 640)])
 def test_dot_product_tendency_and_step(L, nlat, nlon):
     import torch
     from aeolus_amd.physics import BarotropicState, BarotropicVorticity

@@ -30,7 +30,8 @@ def _planet(L, nlat, nlon):
                                   product_quadrature="fine")
 
 
-@pytest.mark.parametrize("L,nlat,nlon", [(21, 32, 64), (85, 128, 256), (170, 256, 512)])
+@pytest.mark.parametrize("L,nlat,nlon", [(21, 32, 64), (85, 128, 256), (170, 256, 512),
+                                         (200, 320, 640)])
 def test_split_bve_tendency_matches_fp64(L, nlat, nlon):
     import torch
     from aeolus_amd.physics import BarotropicState, BarotropicVorticity
@@ -72,7 +73,7 @@ def test_split_batched_members_match_single():
 
 
 @pytest.mark.parametrize("L,nlat,nlon,nmem,ns", [(42, 64, 128, 3, 7), (85, 128, 256, 40, 6),
-                                                 (170, 256, 512, 20, 5)])
+                                                 (170, 256, 512, 20, 5), (200, 320, 640, 5, 6)])
 def test_split_gemm_form_matches_fused_bitwise(L, nlat, nlon, nmem, ns):
     """The member-batched GEMM form (tiles 0: analysis, epilogue and synthesis
     kernels spanning the batch) against the fused stage kernel at the same
