@@ -114,42 +114,50 @@ struct PwSwe {  // 12 synthesised fields -> w * (zeta_dot, curl, kinetic, phi_do
 // each wave sums its quarter of k, the four partials meet in LDS in a fixed
 // order, and the first wave applies the pointwise stage.
 template <class Loader, class Pw>
-__global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld0, Pw pw0) {
+// The coefficient values sit in LDS tile by tile, `tile` basis entries at a
+// time: one tile holds all B of them when they fit (then every slice sums one
+// contiguous k range), several when NF B values would pass the LDS.
+__global__ __launch_bounds__(kPtsThreads) void points_syn_kernel(PointsDev p, Loader ld0, Pw pw0,
+                                                                 int tile) {
     constexpr int NF = Loader::NF;
     const Loader ld = at_group(ld0, (int)blockIdx.y);
     const Pw pw = at_group(pw0, (int)blockIdx.y);
-    extern __shared__ double2 lds[];  // [NF][B] coefficient values times c_k | partials
+    extern __shared__ double2 lds[];  // [NF][tile] coefficient values times c_k | partials
     const int B = p.B;
-    for (int t = threadIdx.x; t < NF * B; t += kPtsThreads) {
-        const int f = t / B, k = t - f * B;
-        int l, m;
-        k_to_lm(k, l, m);
-        const double2 v = ld.value(f, l, m);
-        lds[t] = (m == 0) ? v : cscale(v, 2.0);
-    }
-    __syncthreads();
     const int lane = threadIdx.x % kPtsSynPts, sl = threadIdx.x / kPtsSynPts;
     const int i = blockIdx.x * kPtsSynPts + lane;
     const int ic = (i < p.npts) ? i : p.npts - 1;   // clamped: every load stays in range
-    const int kq = (B + kPtsSynSlices - 1) / kPtsSynSlices;
-    const int k0 = sl * kq, k1 = (k0 + kq < B) ? k0 + kq : B;
     double acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0;
     const double* yr = p.Yk + ic;
     const double* yi = p.Yk + (size_t)B * p.npts + ic;
+    for (int kt0 = 0; kt0 < B; kt0 += tile) {
+        const int nt = (B - kt0 < tile) ? B - kt0 : tile;
+        if (kt0 > 0) __syncthreads();               // every slice is done with the last tile
+        for (int t = threadIdx.x; t < NF * nt; t += kPtsThreads) {
+            const int f = t / nt, kk = t - f * nt;
+            int l, m;
+            k_to_lm(kt0 + kk, l, m);
+            const double2 v = ld.value(f, l, m);
+            lds[f * tile + kk] = (m == 0) ? v : cscale(v, 2.0);
+        }
+        __syncthreads();
+        const int kq = (nt + kPtsSynSlices - 1) / kPtsSynSlices;
+        const int k0 = kt0 + sl * kq, k1 = (k0 + kq < kt0 + nt) ? k0 + kq : kt0 + nt;
 #pragma unroll 8
-    for (int k = k0; k < k1; ++k) {
-        const double a = yr[(size_t)k * p.npts];
-        const double b = yi[(size_t)k * p.npts];
+        for (int k = k0; k < k1; ++k) {
+            const double a = yr[(size_t)k * p.npts];
+            const double b = yi[(size_t)k * p.npts];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const double2 c = lds[f * B + k];
-            acc[f] = fma(a, c.x, acc[f]);
-            acc[f] = fma(-b, c.y, acc[f]);
+            for (int f = 0; f < NF; ++f) {
+                const double2 c = lds[f * tile + (k - kt0)];
+                acc[f] = fma(a, c.x, acc[f]);
+                acc[f] = fma(-b, c.y, acc[f]);
+            }
         }
     }
-    double* part = reinterpret_cast<double*>(lds + (size_t)NF * B);   // [slices][NF][64]
+    double* part = reinterpret_cast<double*>(lds + (size_t)NF * tile);   // [slices][NF][64]
 #pragma unroll
     for (int f = 0; f < NF; ++f) part[(sl * NF + f) * kPtsSynPts + lane] = acc[f];
     __syncthreads();
@@ -475,17 +483,24 @@ __global__ __launch_bounds__(kPtsThreads) void points_partial_split_kernel(
 template <class Loader, class Pw>
 static int launch_psyn(const aeo_plan_s* p, const Loader& ld, const Pw& pw, hipStream_t s,
                        int groups = 1) {
-    const size_t lds = sizeof(double2) * Loader::NF * p->B +
-                       sizeof(double) * kPtsSynSlices * Loader::NF * kPtsSynPts;
-    AEO_REQUIRE(lds <= 163840, AEO_ERR_UNSUPPORTED,
-                "point synthesis: LDS demand exceeds 160 KiB at this l_max");
+    // all B coefficient values of the NF fields in LDS when they fit, else
+    // tiles of a multiple of 64 entries
+    const size_t fixed = sizeof(double) * kPtsSynSlices * Loader::NF * kPtsSynPts;
+    const size_t per = sizeof(double2) * Loader::NF;
+    int tile = p->B;
+    if (per * p->B + fixed > 163840) {
+        AEO_REQUIRE(fixed + 64 * per <= 163840, AEO_ERR_UNSUPPORTED,
+                    "point synthesis: LDS demand exceeds 160 KiB for this field batch");
+        tile = (int)(((163840 - fixed) / per) & ~(size_t)63);
+    }
+    const size_t lds = per * tile + fixed;
     dim3 grid((p->npts + kPtsSynPts - 1) / kPtsSynPts, groups);
     if (lds > 65536) {
         AEO_CHECK_HIP(hipFuncSetAttribute((const void*)points_syn_kernel<Loader, Pw>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     hipLaunchKernelGGL((points_syn_kernel<Loader, Pw>), grid, dim3(kPtsThreads), lds, s,
-                       p->pdev(), ld, pw);
+                       p->pdev(), ld, pw, tile);
     AEO_LAUNCH_CHECK();
     return AEO_OK;
 }

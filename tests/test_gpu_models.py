@@ -41,7 +41,8 @@ def _planets(kind, L, nlat=0, nlon=0, res=0, R=R_C1, Om=OMEGA):
     ("latlon", 8, 12, 24, 0, 0.0), ("latlon", 21, 32, 64, 0, 1e5),
     ("latlon", 85, 128, 256, 0, 0.0), ("geodesic", 10, 0, 0, 3, 0.0),
     ("latlon", 200, 320, 640, 0, 1e5), ("latlon", 230, 352, 704, 0, 0.0),
-    ("latlon", 42, 65, 128, 0, 0.0), ("latlon", 85, 129, 200, 0, 0.0)])
+    ("latlon", 42, 65, 128, 0, 0.0), ("latlon", 85, 129, 200, 0, 0.0),
+    ("geodesic", 42, 0, 0, 5, 1e5)])
 def test_bve_tendency_matches_oracle(kind, L, nlat, nlon, res, nu):
     import torch
     from aeolus_amd.physics import BarotropicState, BarotropicVorticity
@@ -103,7 +104,7 @@ def test_bve_fused_rk4_matches_oracle_short(L, nlat, nlon):
 @pytest.mark.parametrize("kind,L,nlat,nlon,res,nu4,topo", [
     ("latlon", 15, 32, 64, 0, 0.0, False), ("latlon", 21, 32, 64, 0, 1e14, True),
     ("latlon", 85, 128, 256, 0, 0.0, True), ("geodesic", 10, 0, 0, 3, 0.0, True),
-    ("latlon", 170, 256, 512, 0, 0.0, False)])
+    ("latlon", 170, 256, 512, 0, 0.0, False), ("geodesic", 42, 0, 0, 5, 0.0, True)])
 def test_swe_tendency_matches_oracle(kind, L, nlat, nlon, res, nu4, topo):
     import torch
     from aeolus_amd.physics import ShallowWaterModel, Topography
