@@ -999,14 +999,16 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     const double2* __restrict__ O1, int msplit) {
     constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
     extern __shared__ double2 lds[];
-    // workgroup -> (row pair jp, hemisphere hs): the two rows of a pair on
-    // one XCD (ids dealt to the 8 XCDs round-robin), sharing the E/O loads
+    // workgroup -> (row pair jp, hemisphere hs): ids are dealt to the 8 XCDs
+    // round-robin; XCD x takes the nh/8 consecutive row pairs from x nh/8
+    // (both rows of each), so the 8 rows whose spectra share a line of
+    // FN/FS[m][jp] write it through one L2 (full-line write-backs)
     int jp, hs;
     const int id = blockIdx.x;
     if ((g.nh & 7) == 0) {
         const int x = id & 7, k = id >> 3;
         hs = k & 1;
-        jp = (k >> 1) * 8 + x;
+        jp = x * (g.nh >> 3) + (k >> 1);
     } else {
         jp = id >> 1;
         hs = id & 1;
