@@ -3,9 +3,9 @@
 // all the parallelism the Legendre stage has).
 //
 // It computes what leg_stage_kernel<KindBve, EpiBveS, 1, ANA, SYN, 0>
-// computes (per-degree analysis sums in the same order; the synthesis sums
-// of a split column associated per degree half), written for a short
-// critical path:
+// computes, to rounding (the per-degree analysis sums associated per
+// half-wave and wave, the synthesis sums of a split column per degree
+// half), written for a short critical path:
 //   * a small argument block and 32-bit indexing (no scalar reloads of a
 //     large kernel-argument struct, no 64-bit address arithmetic per load);
 //   * every global load that does not depend on this launch's results --
@@ -19,12 +19,13 @@
 //     over all latitudes into its own E/O plane (the second half into plane
 //     1; the grid stage adds the planes in the fixed order plane 0 + plane 1).
 //     No analysis-table row is loaded twice and no workgroup waits on another;
-//   * analysis lanes cover degrees l and l + 64 (equal parity, so one LDS
-//     read of S or D serves both); in the synthesis each half-wave takes two
-//     of the four fields and each lane four latitudes j + 32 q (one LDS read
-//     of a coefficient serves four recurrences), and the recurrence pairs are
-//     scalar loads: the per-degree LDS traffic, which bounds both loops, is
-//     cut from five wave-wide reads to two.
+//   * analysis lanes cover degrees l, l + 32, l + 64, l + 96 (equal parity,
+//     so one LDS read of S or D serves all four) and the two half-waves
+//     split the latitudes; in the synthesis each half-wave takes two of the
+//     four fields and each lane four latitudes j + 32 q (one LDS read of a
+//     coefficient serves four recurrences), and the recurrence pairs come
+//     through vector buffer loads: the per-degree LDS traffic, which bounds
+//     both loops, is cut from five wave-wide reads to two.
 // Per workgroup: analysis of column m (lanes over degree, waves over
 // latitude, reduced over waves in LDS), tendency -J + nu lap' zeta (+ forcing)
 // with row 0 pinned and the RK4 stage update, then the four synthesis fields
@@ -38,7 +39,8 @@ namespace aeo {
 
 constexpr int kB1Threads = 512;
 constexpr int kB1Waves = kB1Threads / 64;
-constexpr int kB1Pre = 16;       // analysis-table rows prefetched per lane and slot
+constexpr int kB1Slots = 4;      // analysis degree slots of 32 (128 degrees at most)
+constexpr int kB1Pre = 8;        // analysis-table rows prefetched per lane and slot
 constexpr int kB1SynWaves = 4;   // synthesis degree blocks, one wave each
 constexpr int kB1SplitQ8 = 93;   // split point of a heavy column: s ~ K q8 / 256
 
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int e0 = (d0 > 0) ? d0 - 1 : 0, e1 = (d1 < K) ? d1 + 1 : K;
     const int a1 = (e1 - 1 < top - m) ? e1 - 1 : top - m;
     const bool active = a.ana && e0 <= a1;
-    const bool slot2 = active && e0 + 64 <= a1;      // degrees past e0 + 63 (uniform)
+    const int nsl = active ? (a1 - e0) / 32 + 1 : 0;  // degree slots in use (uniform, <= 4)
     const long long boff = (long long)nh * ((long long)m * n - (long long)m * (m - 1) / 2);
     const int col = m * n;                       // internal [m][l] column offset
 
@@ -148,12 +150,17 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int nhx = nh + 1;
     double2* X = scratch;                        // [2][nh + 1]
     double2* accl = X + 2 * nhx;                 // [n] analysed column
-    double2* red = accl + n;                     // [8][2][64] analysis wave partials
+    double2* red = accl + n;                     // [8][4][32] analysis wave partials
     double2* sred = scratch;                     // [NB/2][2 parity][2][kR][64] synthesis tree
 
     // ---- prologue: all independent global loads first
-    double pa[2][kB1Pre];
-    const int par = (e0 + lane) & 1;             // parity of both of this lane's degrees
+    // analysis lanes: degree slot d = lane % 32 (degrees e0 + d + 32 s, s <
+    // nsl, all of one parity), half-wave ah = lane / 32 (latitude rows
+    // wave + 16 u + 8 ah of each pass of 128): one LDS read of S or D serves
+    // up to four degrees of a lane
+    const int ad = lane & 31, ah = lane >> 5;
+    double pa[kB1Slots][kB1Pre];
+    const int par = (e0 + ad) & 1;               // parity of every degree of this lane
     if (active) {
         const double* P = a.Pa + boff;
         // buffer loads over block m: lanes past the range get an offset past
@@ -162,22 +169,19 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         // the wave)
         const __amdgpu_buffer_rsrc_t rs = table_rsrc(P, (size_t)K * nh);
 #pragma unroll
-        for (int u = 0; u < kB1Pre; ++u) {
-            const int i = e0 + lane, jp = wave + kB1Waves * u;
-            const unsigned off = (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
-            pa[0][u] = load_f64(rs, off);
-        }
-        if (slot2) {
+        for (int s = 0; s < kB1Slots; ++s) {
+            if (s < nsl) {
 #pragma unroll
-            for (int u = 0; u < kB1Pre; ++u) {
-                const int i = e0 + 64 + lane, jp = wave + kB1Waves * u;
-                const unsigned off =
-                    (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
-                pa[1][u] = load_f64(rs, off);
+                for (int u = 0; u < kB1Pre; ++u) {
+                    const int i = e0 + ad + 32 * s, jp = wave + 16 * u + 8 * ah;
+                    const unsigned off =
+                        (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
+                    pa[s][u] = load_f64(rs, off);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kB1Pre; ++u) pa[s][u] = 0.0;
             }
-        } else {
-#pragma unroll
-            for (int u = 0; u < kB1Pre; ++u) pa[1][u] = 0.0;
         }
     }
     // synthesis seeds of this lane's first latitude pass
@@ -261,27 +265,29 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         }
     __syncthreads();
 
-    // ---- analysis: lanes over degrees e0 + lane and e0 + 64 + lane, waves
-    // over latitude (rows wave + 8u), reduced over waves in order
+    // ---- analysis: lanes over degrees e0 + d + 32 s, half-waves and waves
+    // over latitude; the two half-waves' sums meet by a lane swap, then the
+    // waves' sums in wave order
     if (active) {
         const double* P = a.Pa + boff;
-        double2 acc0 = c2(0.0, 0.0), acc1 = c2(0.0, 0.0);
-        for (int j0 = wave; j0 < nh; j0 += kB1Waves * kB1Pre) {
-            double pv[2][kB1Pre];
-            if (j0 == wave) {
+        double2 acc[kB1Slots];
 #pragma unroll
-                for (int u = 0; u < kB1Pre; ++u) {
-                    pv[0][u] = pa[0][u];
-                    pv[1][u] = pa[1][u];
-                }
+        for (int s = 0; s < kB1Slots; ++s) acc[s] = c2(0.0, 0.0);
+        for (int j0 = 0; j0 < nh; j0 += kB1Waves * 2 * kB1Pre) {
+            double pv[kB1Slots][kB1Pre];
+            if (j0 == 0) {
+#pragma unroll
+                for (int s = 0; s < kB1Slots; ++s)
+#pragma unroll
+                    for (int u = 0; u < kB1Pre; ++u) pv[s][u] = pa[s][u];
             } else {                                      // nh > 128 (L > 170)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const int i = e0 + 64 * q + lane;
+                for (int s = 0; s < kB1Slots; ++s) {
+                    const int i = e0 + ad + 32 * s;
 #pragma unroll
                     for (int u = 0; u < kB1Pre; ++u) {
-                        const int jp = j0 + kB1Waves * u;
-                        pv[q][u] = (jp < nh && i <= a1) ? P[jp * K + i] : 0.0;
+                        const int jp = j0 + wave + 16 * u + 8 * ah;
+                        pv[s][u] = (s < nsl && jp < nh && i <= a1) ? P[jp * K + i] : 0.0;
                     }
                 }
             }
@@ -292,29 +298,34 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
                 double2 x[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int jq = j0 + kB1Waves * (u0 + u);
+                    const int jq = j0 + wave + 16 * (u0 + u) + 8 * ah;
                     x[u] = X[par * nhx + ((jq < nh) ? jq : nh - 1)];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    acc0.x = fma(pv[0][u0 + u], x[u].x, acc0.x);
-                    acc0.y = fma(pv[0][u0 + u], x[u].y, acc0.y);
-                    if (slot2) {
-                        acc1.x = fma(pv[1][u0 + u], x[u].x, acc1.x);
-                        acc1.y = fma(pv[1][u0 + u], x[u].y, acc1.y);
+                for (int s = 0; s < kB1Slots; ++s) {
+                    if (s < nsl) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            acc[s].x = fma(pv[s][u0 + u], x[u].x, acc[s].x);
+                            acc[s].y = fma(pv[s][u0 + u], x[u].y, acc[s].y);
+                        }
                     }
                 }
             }
         }
-        red[(wave * 2 + 0) * 64 + lane] = acc0;
-        red[(wave * 2 + 1) * 64 + lane] = acc1;
-        __syncthreads();
-        if (tid < 128) {
-            const int q = tid >> 6, i = e0 + 64 * q + lane;
-            if (i <= a1) {
-                double2 sv = red[q * 64 + lane];
 #pragma unroll
-                for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[(w * 2 + q) * 64 + lane]);
+        for (int s = 0; s < kB1Slots; ++s) {
+            acc[s].x += __shfl_xor(acc[s].x, 32);
+            acc[s].y += __shfl_xor(acc[s].y, 32);
+            if (ah == 0) red[(wave * kB1Slots + s) * 32 + ad] = acc[s];
+        }
+        __syncthreads();
+        if (tid < 32 * kB1Slots) {
+            const int s = tid >> 5, d = tid & 31, i = e0 + d + 32 * s;
+            if (i <= a1) {
+                double2 sv = red[s * 32 + d];
+#pragma unroll
+                for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[(w * kB1Slots + s) * 32 + d]);
                 accl[m + i] = sv;
             }
         }
@@ -581,7 +592,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
 
 static size_t bve1_lds(int L, int nh) {
     const size_t n = L + 1, R = (nh > 64) ? 4 : 2;
-    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * 2 * 64,
+    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * kB1Slots * 32,
                  syn = (kB1SynWaves / 2) * 2 * 2 * R * 64;
     return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
