@@ -19,9 +19,9 @@
 //     over all latitudes into its own E/O plane (the second half into plane
 //     1; the grid stage adds the planes in the fixed order plane 0 + plane 1).
 //     No analysis-table row is loaded twice and no workgroup waits on another;
-//   * analysis lanes cover degrees l, l + 32, l + 64, l + 96 (equal parity,
-//     so one LDS read of S or D serves all four) and the two half-waves
-//     split the latitudes; in the synthesis each half-wave takes two of the
+//   * analysis lanes cover degrees l + 32 s (equal parity, so one LDS read
+//     of S or D serves all of them) and the two half-waves split the
+//     latitudes; in the synthesis each half-wave takes two of the
 //     four fields and each lane four latitudes j + 32 q (one LDS read of a
 //     coefficient serves four recurrences), and the recurrence pairs come
 //     through vector buffer loads: the per-degree LDS traffic, which bounds
@@ -39,8 +39,10 @@ namespace aeo {
 
 constexpr int kB1Threads = 512;
 constexpr int kB1Waves = kB1Threads / 64;
-constexpr int kB1Slots = 4;      // analysis degree slots of 32 (128 degrees at most)
-constexpr int kB1Pre = 8;        // analysis-table rows prefetched per lane and slot
+constexpr int kB1AG = 2;                  // analysis latitude groups per wave
+constexpr int kB1AW = 64 / kB1AG;         // degrees per analysis slot (lanes per group)
+constexpr int kB1Slots = 128 / kB1AW;     // analysis degree slots (128 degrees at most)
+constexpr int kB1Pre = 16 / kB1AG;        // analysis-table rows prefetched per lane and slot
 constexpr int kB1SynWaves = 4;   // synthesis degree blocks, one wave each
 constexpr int kB1SplitQ8 = 93;   // split point of a heavy column: s ~ K q8 / 256
 
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int e0 = (d0 > 0) ? d0 - 1 : 0, e1 = (d1 < K) ? d1 + 1 : K;
     const int a1 = (e1 - 1 < top - m) ? e1 - 1 : top - m;
     const bool active = a.ana && e0 <= a1;
-    const int nsl = active ? (a1 - e0) / 32 + 1 : 0;  // degree slots in use (uniform, <= 4)
+    const int nsl = active ? (a1 - e0) / kB1AW + 1 : 0;   // degree slots in use (uniform)
     const long long boff = (long long)nh * ((long long)m * n - (long long)m * (m - 1) / 2);
     const int col = m * n;                       // internal [m][l] column offset
 
@@ -150,15 +152,15 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int nhx = nh + 1;
     double2* X = scratch;                        // [2][nh + 1]
     double2* accl = X + 2 * nhx;                 // [n] analysed column
-    double2* red = accl + n;                     // [8][4][32] analysis wave partials
+    double2* red = accl + n;                     // [8][slots][AW] analysis wave partials
     double2* sred = scratch;                     // [NB/2][2 parity][2][kR][64] synthesis tree
 
     // ---- prologue: all independent global loads first
-    // analysis lanes: degree slot d = lane % 32 (degrees e0 + d + 32 s, s <
-    // nsl, all of one parity), half-wave ah = lane / 32 (latitude rows
-    // wave + 16 u + 8 ah of each pass of 128): one LDS read of S or D serves
-    // up to four degrees of a lane
-    const int ad = lane & 31, ah = lane >> 5;
+    // analysis lanes: degree slot d = lane % AW (degrees e0 + d + AW s, s <
+    // nsl, all of one parity), latitude group ag = lane / AW (rows
+    // wave + 8 (AG u + ag) of each pass of 128): one LDS read of S or D
+    // serves up to 128 / AW degrees of a lane
+    const int ad = lane % kB1AW, ag = lane / kB1AW;
     double pa[kB1Slots][kB1Pre];
     const int par = (e0 + ad) & 1;               // parity of every degree of this lane
     if (active) {
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
             if (s < nsl) {
 #pragma unroll
                 for (int u = 0; u < kB1Pre; ++u) {
-                    const int i = e0 + ad + 32 * s, jp = wave + 16 * u + 8 * ah;
+                    const int i = e0 + ad + kB1AW * s, jp = wave + 8 * (kB1AG * u + ag);
                     const unsigned off =
                         (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
                     pa[s][u] = load_f64(rs, off);
@@ -265,15 +267,15 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         }
     __syncthreads();
 
-    // ---- analysis: lanes over degrees e0 + d + 32 s, half-waves and waves
-    // over latitude; the two half-waves' sums meet by a lane swap, then the
-    // waves' sums in wave order
+    // ---- analysis: lanes over degrees e0 + d + AW s, lane groups and waves
+    // over latitude; the groups' sums meet by lane swaps, then the waves'
+    // sums in wave order
     if (active) {
         const double* P = a.Pa + boff;
         double2 acc[kB1Slots];
 #pragma unroll
         for (int s = 0; s < kB1Slots; ++s) acc[s] = c2(0.0, 0.0);
-        for (int j0 = 0; j0 < nh; j0 += kB1Waves * 2 * kB1Pre) {
+        for (int j0 = 0; j0 < nh; j0 += kB1Waves * kB1AG * kB1Pre) {
             double pv[kB1Slots][kB1Pre];
             if (j0 == 0) {
 #pragma unroll
@@ -283,10 +285,10 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
             } else {                                      // nh > 128 (L > 170)
 #pragma unroll
                 for (int s = 0; s < kB1Slots; ++s) {
-                    const int i = e0 + ad + 32 * s;
+                    const int i = e0 + ad + kB1AW * s;
 #pragma unroll
                     for (int u = 0; u < kB1Pre; ++u) {
-                        const int jp = j0 + wave + 16 * u + 8 * ah;
+                        const int jp = j0 + wave + 8 * (kB1AG * u + ag);
                         pv[s][u] = (s < nsl && jp < nh && i <= a1) ? P[jp * K + i] : 0.0;
                     }
                 }
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
                 double2 x[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int jq = j0 + wave + 16 * (u0 + u) + 8 * ah;
+                    const int jq = j0 + wave + 8 * (kB1AG * (u0 + u) + ag);
                     x[u] = X[par * nhx + ((jq < nh) ? jq : nh - 1)];
                 }
 #pragma unroll
@@ -315,17 +317,21 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         }
 #pragma unroll
         for (int s = 0; s < kB1Slots; ++s) {
-            acc[s].x += __shfl_xor(acc[s].x, 32);
-            acc[s].y += __shfl_xor(acc[s].y, 32);
-            if (ah == 0) red[(wave * kB1Slots + s) * 32 + ad] = acc[s];
+#pragma unroll
+            for (int o = 32; o >= kB1AW; o >>= 1) {
+                acc[s].x += __shfl_xor(acc[s].x, o);
+                acc[s].y += __shfl_xor(acc[s].y, o);
+            }
+            if (ag == 0) red[(wave * kB1Slots + s) * kB1AW + ad] = acc[s];
         }
         __syncthreads();
-        if (tid < 32 * kB1Slots) {
-            const int s = tid >> 5, d = tid & 31, i = e0 + d + 32 * s;
+        if (tid < kB1AW * kB1Slots) {
+            const int s = tid / kB1AW, d = tid % kB1AW, i = e0 + d + kB1AW * s;
             if (i <= a1) {
-                double2 sv = red[s * 32 + d];
+                double2 sv = red[s * kB1AW + d];
 #pragma unroll
-                for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[(w * kB1Slots + s) * 32 + d]);
+                for (int w = 1; w < kB1Waves; ++w)
+                    sv = cadd(sv, red[(w * kB1Slots + s) * kB1AW + d]);
                 accl[m + i] = sv;
             }
         }
@@ -592,7 +598,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
 
 static size_t bve1_lds(int L, int nh) {
     const size_t n = L + 1, R = (nh > 64) ? 4 : 2;
-    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * kB1Slots * 32,
+    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * kB1Slots * kB1AW,
                  syn = (kB1SynWaves / 2) * 2 * 2 * R * 64;
     return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
