@@ -706,6 +706,8 @@ def main():
                                              ensemble_modes(48, ns_split).items()}
         _progress("SWE and ensemble done")
         secondary["adjoint_L170"] = adjoint_cost(h["model"], 7)
+        # the same with FP64 contractions (the forward on the single-member kernels)
+        secondary["adjoint_L170_fp64"] = adjoint_cost(h["model"], 0)
         secondary["geodesic_res4_L21_modes"] = geodesic_modes(args.steps, args.warmup)
         secondary["primitive_equations_K8"] = pe_modes(min(args.steps, 100))
         secondary["c1_bve_gauss12x24_L8_fp64_ms_per_step"] = c1_device(144)
