@@ -200,6 +200,18 @@ int aeo_plan_create_gauss(int nlat, int nlon, int l_max, const double* x_nodes,
         }
         tw[t] = make_double2(x, y);
     }
+    // the per-row Jacobian kernel's stage-major copy (hemi_tw) after it
+    if ((nlon & (nlon - 1)) == 0 && nlon >= 64 && nlon <= 512) {
+        int lgn = 0;
+        while ((1 << lgn) < nlon) ++lgn;
+        const HemiTw ht = hemi_tw(lgn);
+        for (int b = 0; b < ht.n; ++b)
+            for (int r = 1; r < (1 << ht.lgr[b]); ++r)
+                for (int k = 0; k < (1 << ht.lgns[b]); ++k) {
+                    const double2 w = tw[(r * k) << (lgn - ht.lgns[b] - ht.lgr[b])];
+                    tw.push_back(w);
+                }
+    }
     std::vector<double> xn(x_nodes, x_nodes + nh);
     // Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}), stored m-major as
     // (a_lm, a_lm b_lm) for the one-fma step of rec_step

@@ -35,6 +35,45 @@ struct FftDev {
     int lgn;            // log2(n) when n is a power of two (shift/mask stages), else -1
 };
 
+// Twiddle table of the per-row Jacobian kernel (grid_jac_hemi_kernel),
+// stage-major so that the lanes of a stage read consecutive entries: read
+// from the full table W^t, t < N, the span-8 radix-8 stage took W^(8 r k) at
+// a stride of 8 r entries (eight lanes on one bank) and the span-1 stages
+// W^(r k) at a stride of r.  Block (LGR, LGNS), the radix-2^LGR stage whose
+// spans are 2^LGNS, holds W^((r k) 2^(LGN - LGNS - LGR)) at (r - 1) 2^LGNS + k,
+// 0 < r < 2^LGR, k < 2^LGNS.  The blocks are those of the stages the kernel
+// runs after the untwiddled first one, each once: the inverse middle stage
+// (radix 2^(LGN - 6), spans 8), the last inverse stage (radix 8, spans
+// N/8), the forward stage of spans 8 (radix 8) and the forward last stage
+// (radix 2^(LGN - 6), spans 64); N = 512 and N = 64 share theirs between
+// the two directions.  A plan of 64 to 512 (power-of-two) longitudes keeps
+// it after its full table, at tw + N.
+struct HemiTw {
+    int n, total;
+    int lgr[4], lgns[4], off[4];
+};
+__host__ __device__ constexpr HemiTw hemi_tw(int lgn) {
+    HemiTw t{};
+    const int cr[4] = {lgn - 6, 3, 3, lgn - 6}, cs[4] = {3, lgn - 3, 3, 6};
+    for (int i = 0; i < 4; ++i) {
+        bool have = cr[i] <= 0;
+        for (int k = 0; k < t.n; ++k) have = have || (t.lgr[k] == cr[i] && t.lgns[k] == cs[i]);
+        if (have) continue;
+        t.lgr[t.n] = cr[i];
+        t.lgns[t.n] = cs[i];
+        t.off[t.n] = t.total;
+        t.total += ((1 << cr[i]) - 1) << cs[i];
+        ++t.n;
+    }
+    return t;
+}
+__host__ __device__ constexpr int hemi_tw_off(int lgn, int lgr, int lgns) {
+    const HemiTw t = hemi_tw(lgn);
+    for (int k = 0; k < t.n; ++k)
+        if (t.lgr[k] == lgr && t.lgns[k] == lgns) return t.off[k];
+    return -1;
+}
+
 template <int DIR>
 __device__ inline double2 twiddle(const double2* tw, int t) {
     double2 w = tw[t];

@@ -713,12 +713,15 @@ __device__ inline void fft_rest_c(double2* buf, const double2* tw) {
 
 // One radix-R Stockham stage from rows `src` to rows `dst` (two LDS
 // buffers): no barrier between the reads and the writes, one after the
-// writes (the in-place fft_stage_c needs both).  Full twiddle table.
+// writes (the in-place fft_stage_c needs both).  `tw` is the per-row
+// kernel's stage-major table (hemi_tw).
 template <int DIR, int R, int LGR, int LGN, int LGNS, int NROWS>
 __device__ inline void fft_stage_pp(const double2* src, double2* dst, const double2* tw) {
     constexpr int N = 1 << LGN, LGNB = LGN - LGR, NB = 1 << LGNB, TOT = NROWS * NB;
-    constexpr int RS = N + (N >> 3), LGSPAN = LGN - LGNS - LGR;
+    constexpr int RS = N + (N >> 3);
     constexpr int MAXB = (TOT + kGridThreads - 1) / kGridThreads;
+    constexpr int TWO = LGNS > 0 ? hemi_tw_off(LGN, LGR, LGNS) : 0;
+    static_assert(TWO >= 0, "a stage with a twiddle block");
     double2 v[MAXB][R], w[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
@@ -730,7 +733,7 @@ __device__ inline void fft_stage_pp(const double2* src, double2* dst, const doub
             for (int r = 0; r < R; ++r) v[b][r] = rb[lp(j + r * NB)];
             if (LGNS > 0) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) w[b][r] = tw[(r * k) << LGSPAN];
+                for (int r = 1; r < R; ++r) w[b][r] = tw[TWO + ((r - 1) << LGNS) + k];
             }
         }
     }
@@ -759,31 +762,16 @@ __device__ inline void fft_stage_pp(const double2* src, double2* dst, const doub
     __syncthreads();
 }
 
-// fft_rest_c between two buffers: stage s reads a and writes b, the next
-// reads b and writes a; returns the buffer that holds the result.
-template <int DIR, int LGN, int LGNS, int NROWS>
-__device__ inline double2* fft_rest_pp(double2* a, double2* b, const double2* tw) {
-    if constexpr (LGN - LGNS >= 3) {
-        fft_stage_pp<DIR, 8, 3, LGN, LGNS, NROWS>(a, b, tw);
-        return fft_rest_pp<DIR, LGN, LGNS + 3, NROWS>(b, a, tw);
-    } else if constexpr (LGN - LGNS == 2) {
-        fft_stage_pp<DIR, 4, 2, LGN, LGNS, NROWS>(a, b, tw);
-        return b;
-    } else if constexpr (LGN - LGNS == 1) {
-        fft_stage_pp<DIR, 2, 1, LGN, LGNS, NROWS>(a, b, tw);
-        return b;
-    } else {
-        return a;
-    }
-}
-
 // The last Stockham stage of one row (NROWS 1), its outputs m <= mmax
-// stored straight to global memory, out[m * ostride], instead of LDS.
+// stored straight to global memory, out[m * ostride], instead of LDS;
+// `tw` is the stage-major table (hemi_tw).
 template <int DIR, int R, int LGR, int LGN, int LGNS>
 __device__ inline void fft_stage_out(const double2* src, const double2* tw, double2* out,
                                      int ostride, int mmax) {
     static_assert(LGNS + LGR == LGN, "the last stage");
-    constexpr int N = 1 << LGN, NB = 1 << (LGN - LGR), LGSPAN = LGN - LGNS - LGR;
+    constexpr int N = 1 << LGN, NB = 1 << (LGN - LGR);
+    constexpr int TWO = LGNS > 0 ? hemi_tw_off(LGN, LGR, LGNS) : 0;
+    static_assert(TWO >= 0, "a stage with a twiddle block");
     static_assert(NB <= kGridThreads, "one butterfly per thread");
     const int j = threadIdx.x;
     if (j < NB) {
@@ -792,7 +780,7 @@ __device__ inline void fft_stage_out(const double2* src, const double2* tw, doub
         for (int r = 0; r < R; ++r) v[r] = src[lp(j + r * NB)];
         if (LGNS > 0) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) w[r] = tw[(r * j) << LGSPAN];
+            for (int r = 1; r < R; ++r) w[r] = tw[TWO + ((r - 1) << LGNS) + j];
         }
         __builtin_amdgcn_sched_barrier(0);
         if (LGNS > 0) {
@@ -1018,8 +1006,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     const double invR = 1.0 / R;
     const double c = clamp_coslat(g.coslat[j]);
     const size_t eo = (size_t)jp * (L + 1) * 4;
-    // LDS: rows A [2][RS] | rows B [2][RS] | twiddle table [N]; the FFT
-    // stages alternate between A and B (one barrier per stage)
+    // LDS: rows A [2][RS] | rows B [2][RS] | stage-major twiddle table
+    // (hemi_tw); the FFT stages alternate between A and B (one barrier per
+    // stage)
     double2* rowA = lds;
     double2* rowB = lds + 2 * RS;
     double2* twl = lds + 4 * RS;
@@ -1032,16 +1021,19 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         // at m = 0) or Z2 = conj(g_2r(m)) + i conj(g_2r+1(m)), g_f = this
         // row's sum E + O (north, in E) or E - O (south, in O), the split
         // columns' second plane added, with d/dlambda (even f) or 1/R (odd
-        // f); every thread also loads its share of the twiddle table
+        // f); every thread also gathers its share of the stage-major
+        // twiddle table from the full one
         const double2* G = hs ? O : E;
         const double2* G1 = hs ? O1 : E1;
-        constexpr int TWPER = (N + kGridThreads - 1) / kGridThreads;
-        double2 vt[TWPER];
-#pragma unroll
-        for (int u = 0; u < TWPER; ++u) {
-            const int tt = threadIdx.x + u * kGridThreads;
-            vt[u] = g.fft.tw[(tt < N) ? tt : N - 1];
-        }
+        constexpr int TWN = hemi_tw(LGN).total;
+        static_assert(TWN <= 2 * kGridThreads, "two table entries per thread");
+        // two named values, not an array: the array was kept in scratch
+        // memory, whose store waited for the table loads before the
+        // coefficient loads below were issued
+        const double2* twh = g.fft.tw + N;           // the plan's copy, after the full table
+        const int t0 = threadIdx.x, t1 = threadIdx.x + kGridThreads;
+        const double2 vt0 = twh[t0 < TWN ? t0 : TWN - 1];
+        const double2 vt1 = twh[t1 < TWN ? t1 : TWN - 1];
         const int t = threadIdx.x;
         const bool first = t < 2 * NB;
         const int row = first ? t / NB : 0, jj = t % NB;
@@ -1064,11 +1056,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < TWPER; ++u) {
-            const int tt = threadIdx.x + u * kGridThreads;
-            if (tt < N) twl[tt] = vt[u];
-        }
+        if (t0 < TWN) twl[t0] = vt0;
+        if (t1 < TWN) twl[t1] = vt1;
         if (first) {
             double2 v[8];
 #pragma unroll
@@ -1096,21 +1085,21 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     }
     __syncthreads();
     const double q = 1.0 / (c * c);
-    double2* fin = rowA;                             // the rows holding the result
-    if constexpr (LGN >= 6) {
+    {
         // inverse stages radix 8, then 2^(LGN - 6) (none for 64 points, 2, 4
         // or 8 for 128, 256, 512), then a last radix-8 stage: that stage
         // leaves lane j both rows' values at the points j + N/8 q, exactly
         // the inputs of the first (radix-8) forward stage's lane j, so the
         // last inverse stage, the Jacobian at those points and the first
         // forward stage run in registers (no LDS round trip)
-        static_assert(LGN <= 9, "rows of at most 512 points");
+        static_assert(LGN >= 6 && LGN <= 9, "rows of 64 to 512 points");
         if constexpr (LGN == 7) fft_stage_pp<+1, 2, 1, LGN, 3, 2>(rowA, rowB, twl);
         if constexpr (LGN == 8) fft_stage_pp<+1, 4, 2, LGN, 3, 2>(rowA, rowB, twl);
         if constexpr (LGN == 9) fft_stage_pp<+1, 8, 3, LGN, 3, 2>(rowA, rowB, twl);
         double2* src = (LGN == 6) ? rowA : rowB;     // the last inverse stage's input rows
         double2* dst = (LGN == 6) ? rowB : rowA;
         constexpr int LS = LGN - 3;                  // the last stage's LGNS (N/8 = 2^LS lanes)
+        constexpr int TWF = hemi_tw_off(LGN, 3, LS);   // its twiddles W^(r jl)
         const int jl = threadIdx.x;
         double2 v[8];
         if (jl < NB) {
@@ -1119,7 +1108,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
             for (int r = 0; r < 8; ++r) {
                 xa[r] = src[lp(jl + r * NB)];
                 xb[r] = src[RS + lp(jl + r * NB)];
-                if (r > 0) wt[r] = twl[r * jl];      // W^(r k), k = jl (span 1)
+                if (r > 0) wt[r] = twl[TWF + (r - 1) * NB + jl];
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1145,26 +1134,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         // contiguous column (the row-major layout cost it one 128-byte line
         // per 16 bytes)
         fft_rest_pp_out<-1, LGN, 3>(dst, src, twl, (hs ? FS : FN) + jp, g.nh, mmax);
-        return;
-    } else {
-        fft_rest_c<+1, LGN, 3, 2, false>(lds, twl);
-        // the Jacobian row (real) into row 0 as a complex row with zero
-        // imaginary part
-#pragma unroll
-        for (int k = threadIdx.x; k < N; k += kGridThreads) {
-            const int e = lp(k);
-            const double2 za = lds[e], zb = lds[RS + e];
-            const double jv = jac_point_q(za.x, za.y, zb.x, zb.y, q);
-            lds[e] = c2(jv, 0.0);
-        }
-        __syncthreads();
-        fft_stage_c<-1, 8, 3, LGN, 0, 1, false>(lds, twl);
-        fft_rest_c<-1, LGN, 3, 1, false>(lds, twl);
     }
-    // column-major [m][jp]: the stage kernel of column m reads a contiguous
-    // column (the row-major layout cost it one 128-byte line per 16 bytes)
-    double2* F = (hs ? FS : FN) + jp;
-    for (int m = threadIdx.x; m <= mmax; m += kGridThreads) F[(size_t)m * g.nh] = fin[lp(m)];
 }
 
 // Kernel instance for the plan's FFT length (power-of-two stages or general radices).
@@ -1265,7 +1235,8 @@ static HemiKernel hemi_kernel_for(const aeo_plan_s* p) {
     }
 }
 static size_t hemi_lds(const aeo_plan_s* p) {
-    return sizeof(double2) * (4 * (size_t)lds_row_stride(p->nlon) + p->nlon);
+    const int lgn = p->fft.lgn < 6 ? 6 : (p->fft.lgn > 9 ? 9 : p->fft.lgn);
+    return sizeof(double2) * (4 * (size_t)lds_row_stride(p->nlon) + hemi_tw(lgn).total);
 }
 
 // Rows per workgroup of the plain transforms' grid kernels for a batch of nf
