@@ -1014,15 +1014,16 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     double2* twl = lds + 4 * RS;
     {
         // prologue and inverse stage 1 (radix 8, no twiddles) in registers:
-        // lane (row, jj) loads, for each of its bins b = jj + r N/8, the two
+        // lane (row, jj) reads, for each of its bins b = jj + r N/8, the two
         // fields 2 row, 2 row + 1 of the one m the bin takes (N > 2L: the
         // bin-m term m = b when r < 4, the mirrored bin's m = N - b when
         // r >= 4) and forms the packed bin: Z1 = g_2r(m) + i g_2r+1(m) (real
         // at m = 0) or Z2 = conj(g_2r(m)) + i conj(g_2r+1(m)), g_f = this
         // row's sum E + O (north, in E) or E - O (south, in O), the split
         // columns' second plane added, with d/dlambda (even f) or 1/R (odd
-        // f); every thread also gathers its share of the stage-major
-        // twiddle table from the full one
+        // f); every thread also loads its share of the stage-major twiddle
+        // table
+        static_assert(2 * N + 16 <= 2 * RS, "the staged coefficients fit in rows B");
         const double2* G = hs ? O : E;
         const double2* G1 = hs ? O1 : E1;
         constexpr int TWN = hemi_tw(LGN).total;
@@ -1034,38 +1035,46 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
         const int t0 = threadIdx.x, t1 = threadIdx.x + kGridThreads;
         const double2 vt0 = twh[t0 < TWN ? t0 : TWN - 1];
         const double2 vt1 = twh[t1 < TWN ? t1 : TWN - 1];
-        const int t = threadIdx.x;
-        const bool first = t < 2 * NB;
-        const int row = first ? t / NB : 0, jj = t % NB;
-        double2 ve[8], vo[8], ve1[8], vo1[8];
-        if (first) {
+        // the row's coefficients [m][4 fields] in coalesced loads (a lane
+        // per 16 bytes: the per-bin gathers took 16 bytes of each 64 and
+        // one cache line per lane), the split columns' second plane added,
+        // staged field-major in rows B, cs[f RC + m]: RC = 4 mod 16 keeps
+        // the 16 lanes of a write pass (4 m x 4 f) and the bin reads
+        // (consecutive m) on distinct banks
+        const int NE = 4 * (L + 1), RC = ((L + 16) & ~15) + 4;
+        constexpr int EPT = (4 * (N / 2) + kGridThreads - 1) / kGridThreads;  // 4 (L + 1) <= 2 N
+        double2* cs = rowB;
+        double2 ce[EPT], ce1[EPT];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int b = jj + r * NB, mm = (r < 4) ? b : N - b;
-                const int mc = (mm <= L) ? mm : L;
-                const size_t e0 = eo + (size_t)mc * 4 + 2 * row;
-                ve[r] = vo[r] = ve1[r] = vo1[r] = c2(0.0, 0.0);
-                if (mm <= L) {                       // bins past the spectrum load nothing
-                    ve[r] = G[e0];
-                    vo[r] = G[e0 + 1];
-                    if (mc < msplit) {
-                        ve1[r] = G1[e0];
-                        vo1[r] = G1[e0 + 1];
-                    }
-                }
-            }
+        for (int u = 0; u < EPT; ++u) {
+            const int e = threadIdx.x + u * kGridThreads;
+            ce[u] = ce1[u] = c2(0.0, 0.0);
+            if (e < NE) ce[u] = G[eo + e];
+            if (e < NE && (e >> 2) < msplit) ce1[u] = G1[eo + e];
         }
         __builtin_amdgcn_sched_barrier(0);
         if (t0 < TWN) twl[t0] = vt0;
         if (t1 < TWN) twl[t1] = vt1;
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int e = threadIdx.x + u * kGridThreads, m = e >> 2;
+            if (e < NE) cs[(e & 3) * RC + m] = (m < msplit) ? cadd(ce[u], ce1[u]) : ce[u];
+        }
+        __syncthreads();
+        const int t = threadIdx.x;
+        const bool first = t < 2 * NB;
+        const int row = first ? t / NB : 0, jj = t % NB;
         if (first) {
             double2 v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 const int b = jj + r * NB, mm = (r < 4) ? b : N - b;
                 const int mc = (mm <= L) ? mm : L;
-                double2 ge = (mc < msplit) ? cadd(ve[r], ve1[r]) : ve[r];
-                double2 go = (mc < msplit) ? cadd(vo[r], vo1[r]) : vo[r];
+                double2 ge = c2(0.0, 0.0), go = c2(0.0, 0.0);
+                if (mm <= L) {                       // bins past the spectrum read nothing
+                    ge = cs[2 * row * RC + mc];
+                    go = cs[(2 * row + 1) * RC + mc];
+                }
                 ge = dlam_q(ge, mc * invR);
                 go = cscale(go, invR);
                 double2 z = c2(0.0, 0.0);
