@@ -56,7 +56,8 @@ static void dev_free(void* p) {
 static void free_plan(aeo_plan_s* p) {
     if (!p) return;
     for (void* q : {(void*)p->d_Ps, (void*)p->d_Pa, (void*)p->d_offs, (void*)p->d_w_rows,
-                    (void*)p->d_coslat, (void*)p->d_tw, (void*)p->d_E, (void*)p->d_O,
+                    (void*)p->d_coslat, (void*)p->d_tw, (void*)p->d_b1pa, (void*)p->d_E,
+                    (void*)p->d_O,
                     (void*)p->d_S, (void*)p->d_D, (void*)p->d_Yk, (void*)p->d_Yp, (void*)p->d_w,
                     (void*)p->d_pcoslat, (void*)p->d_gw, (void*)p->d_part, (void*)p->d_Cp,
                     (void*)p->d_Cm, (void*)p->d_spec, (void*)p->d_Psl, (void*)p->d_offs_ss,
@@ -542,6 +543,7 @@ static int model_common(aeo_plan_t prod, double radius, aeo_model_s* m) {
         if (m->kind == kBVE && nm == 1) {
             if ((rc = dev_alloc(&m->d_E1, plane * nsyn))) return rc;
             if ((rc = dev_alloc(&m->d_O1, plane * nsyn))) return rc;
+            if ((rc = bve1_prepare(prod, nullptr))) return rc;
         }
         if ((rc = dev_alloc(&m->d_S, plane * nana * nm))) return rc;
         if ((rc = dev_alloc(&m->d_D, plane * nana * nm))) return rc;

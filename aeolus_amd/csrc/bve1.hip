@@ -43,25 +43,20 @@ constexpr int kB1AG = 2;                  // analysis latitude groups per wave
 constexpr int kB1AW = 64 / kB1AG;         // degrees per analysis slot (lanes per group)
 constexpr int kB1Slots = 128 / kB1AW;     // analysis degree slots (128 degrees at most)
 constexpr int kB1Pre = 16 / kB1AG;        // analysis-table rows prefetched per lane and slot
+constexpr int kB1PalQ = kB1Pre / 2;       // 16-byte lane-order entries per lane and slot
+constexpr int kB1PassRows = kB1Waves * kB1AG * kB1Pre;   // latitudes per analysis pass
+static_assert(kB1AW == 32 && kB1PalQ == 4 && kB1PassRows == 128, "analysis lane layout");
 constexpr int kB1SynWaves = 4;   // synthesis degree blocks, one wave each
 constexpr int kB1SplitQ8 = 93;   // split point of a heavy column: s ~ K q8 / 256
 
-// Raw buffer over `count` doubles (the gfx9 raw-buffer descriptor word 3).
-__device__ inline __amdgpu_buffer_rsrc_t table_rsrc(const double* p, size_t count) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0,
-                                             (int)(count * sizeof(double)), 0x00020000);
-}
-constexpr unsigned kOutOfRange = 0x80000000u;   // byte offset past every table block
-__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
 
 struct Bve1Args {
     int L, nh, top;           // top: analysed degrees l <= top (2L/3)
     int msplit;               // columns m < msplit run as two degree halves
     int split_q8;             // their split point: i = l - m < s = (K q8 / 256) & ~1
     int ana, syn;             // analysis + epilogue; synthesis of the (new) state
-    const double* Pa;         // analysis table, block m: [jp][l - m]
+    const double2* Pal;       // analysis-table blocks in lane order (bve1_pal_kernel)
+    int npass;                // analysis passes of kB1PassRows latitudes
     const double* Ps;         // synthesis table, block m: [l - m][jp]
     const double* xn;         // nh northern nodes
     const double2* rec;       // [m][l] recurrence pairs
@@ -100,6 +95,67 @@ __host__ __device__ inline int bve1_split_point(int K, int q8) {
     return s < 2 ? 2 : (s > hi ? hi : s);
 }
 
+// Workgroup b of a launch: column m and degree half h (-1: whole column;
+// the split columns first, heaviest first); its degrees i = l - m in [d0, d1)
+// (RK4 outputs and synthesis), the state needed on [e0, e1) (the S
+// operator's neighbours) and the analysed degrees [e0, a1].
+struct Bve1Wg {
+    int m, h, K, d0, d1, e0, e1, a1;
+};
+__host__ __device__ inline Bve1Wg bve1_wg(int b, int L, int msplit, int q8, int top) {
+    Bve1Wg w;
+    if (b < 2 * msplit) {
+        w.m = b >> 1;
+        w.h = b & 1;
+    } else {
+        w.m = b - msplit;
+        w.h = -1;
+    }
+    w.K = L - w.m + 1;
+    w.d0 = 0;
+    w.d1 = w.K;
+    if (w.h >= 0) {
+        const int s = bve1_split_point(w.K, q8);
+        if (w.h == 0) w.d1 = s; else w.d0 = s;
+    }
+    w.e0 = (w.d0 > 0) ? w.d0 - 1 : 0;
+    w.e1 = (w.d1 < w.K) ? w.d1 + 1 : w.K;
+    w.a1 = (w.e1 - 1 < top - w.m) ? w.e1 - 1 : top - w.m;
+    return w;
+}
+
+// The analysis-table rows of each workgroup in the order its lanes read
+// them: entry ((p kB1Slots + s) kB1PalQ + q) kB1Threads + t of workgroup b
+// holds, for lane t % 64 of wave t / 64 (degree slot ad = lane % 32,
+// latitude group ag = lane / 32), degree i = e0 + ad + 32 s and the rows
+// jp = 128 p + wave + 8 (2 u + ag) of u = 2 q (x) and 2 q + 1 (y); zero past
+// a1 or nh.  Every load of the stage kernel's analysis is then one
+// contiguous kilobyte per wave (16 bytes a lane) where the [jp][i] blocks
+// gave two 256-byte runs of 8-byte lanes: a 32 MiB read of the two
+// patterns took 2.8 and 1.2 us.
+__global__ __launch_bounds__(kB1Threads) void bve1_pal_kernel(const double* __restrict__ Pa,
+                                                               double2* __restrict__ out, int L,
+                                                               int nh, int msplit, int q8, int top,
+                                                               int npass) {
+    const int n = L + 1, b = blockIdx.x, t = threadIdx.x;
+    const Bve1Wg w = bve1_wg(b, L, msplit, q8, top);
+    const long long boff = (long long)nh * ((long long)w.m * n - (long long)w.m * (w.m - 1) / 2);
+    const int lane = t & 63, wave = t >> 6, ad = lane % kB1AW, ag = lane / kB1AW;
+    double2* o = out + (size_t)b * npass * kB1Slots * kB1PalQ * kB1Threads + t;
+    for (int p = 0; p < npass; ++p)
+        for (int s = 0; s < kB1Slots; ++s) {
+            const int i = w.e0 + ad + kB1AW * s;
+            for (int q = 0; q < kB1PalQ; ++q) {
+                double v[2];
+                for (int e = 0; e < 2; ++e) {
+                    const int u = 2 * q + e, jp = kB1PassRows * p + wave + 8 * (kB1AG * u + ag);
+                    v[e] = (i <= w.a1 && jp < nh) ? Pa[boff + (long long)jp * w.K + i] : 0.0;
+                }
+                o[((p * kB1Slots + s) * kB1PalQ + q) * kB1Threads] = c2(v[0], v[1]);
+            }
+        }
+}
+
 // kR: latitudes per synthesis lane (g + 32 q, q < kR, g = lane % 32): 4 when
 // nh > 64.  recg: the [m][l] recurrence pairs (a.rec) as a restrict pointer,
 // so the synthesis reads them with wave-uniform scalar loads.
@@ -112,27 +168,10 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     // counters derived from it stay in scalar registers
     const int tid = threadIdx.x, lane = tid & 63,
               wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // workgroup -> (m, degree half): the split columns first (heaviest first)
-    int m, h;
-    if ((int)blockIdx.x < 2 * a.msplit) {
-        m = blockIdx.x >> 1;
-        h = blockIdx.x & 1;
-    } else {
-        m = blockIdx.x - a.msplit;
-        h = -1;
-    }
-    const int K = L - m + 1;
-    const int top = a.top;
-    // degrees i = l - m in [d0, d1) are this workgroup's (RK4 outputs and
-    // synthesis); the state is needed on [e0, e1) (the S operator's
-    // neighbours) and analysed on [e0, a1]
-    int d0 = 0, d1 = K;
-    if (h >= 0) {
-        const int s = bve1_split_point(K, a.split_q8);
-        if (h == 0) d1 = s; else d0 = s;
-    }
-    const int e0 = (d0 > 0) ? d0 - 1 : 0, e1 = (d1 < K) ? d1 + 1 : K;
-    const int a1 = (e1 - 1 < top - m) ? e1 - 1 : top - m;
+    // workgroup -> (m, degree half) and its degree ranges (bve1_wg)
+    const Bve1Wg wg = bve1_wg(blockIdx.x, L, a.msplit, a.split_q8, a.top);
+    const int m = wg.m, h = wg.h, K = wg.K, d0 = wg.d0, d1 = wg.d1, e0 = wg.e0, e1 = wg.e1,
+              a1 = wg.a1;
     const bool active = a.ana && e0 <= a1;
     const int nsl = active ? (a1 - e0) / kB1AW + 1 : 0;   // degree slots in use (uniform)
     const long long boff = (long long)nh * ((long long)m * n - (long long)m * (m - 1) / 2);
@@ -163,26 +202,19 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int ad = lane % kB1AW, ag = lane / kB1AW;
     double pa[kB1Slots][kB1Pre];
     const int par = (e0 + ad) & 1;               // parity of every degree of this lane
+    // this workgroup's table rows in lane order (bve1_pal_kernel): load
+    // (s, q) of a lane is one 16-byte entry, rows u = 2 q and 2 q + 1
+    const double2* PL =
+        a.Pal + (size_t)blockIdx.x * a.npass * kB1Slots * kB1PalQ * kB1Threads + tid;
     if (active) {
-        const double* P = a.Pa + boff;
-        // buffer loads over block m: lanes past the range get an offset past
-        // the buffer's range, which reads 0 without a memory request (no
-        // branch per load; clamping them onto one address instead serialises
-        // the wave)
-        const __amdgpu_buffer_rsrc_t rs = table_rsrc(P, (size_t)K * nh);
 #pragma unroll
         for (int s = 0; s < kB1Slots; ++s) {
-            if (s < nsl) {
 #pragma unroll
-                for (int u = 0; u < kB1Pre; ++u) {
-                    const int i = e0 + ad + kB1AW * s, jp = wave + 8 * (kB1AG * u + ag);
-                    const unsigned off =
-                        (i <= a1 && jp < nh) ? (unsigned)(jp * K + i) * 8u : kOutOfRange;
-                    pa[s][u] = load_f64(rs, off);
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < kB1Pre; ++u) pa[s][u] = 0.0;
+            for (int q = 0; q < kB1PalQ; ++q) {
+                double2 v = c2(0.0, 0.0);
+                if (s < nsl && 32 * q < nh) v = PL[(s * kB1PalQ + q) * kB1Threads];
+                pa[s][2 * q] = v.x;
+                pa[s][2 * q + 1] = v.y;
             }
         }
     }
@@ -271,11 +303,10 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     // over latitude; the groups' sums meet by lane swaps, then the waves'
     // sums in wave order
     if (active) {
-        const double* P = a.Pa + boff;
         double2 acc[kB1Slots];
 #pragma unroll
         for (int s = 0; s < kB1Slots; ++s) acc[s] = c2(0.0, 0.0);
-        for (int j0 = 0; j0 < nh; j0 += kB1Waves * kB1AG * kB1Pre) {
+        for (int j0 = 0; j0 < nh; j0 += kB1PassRows) {
             double pv[kB1Slots][kB1Pre];
             if (j0 == 0) {
 #pragma unroll
@@ -283,13 +314,15 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
 #pragma unroll
                     for (int u = 0; u < kB1Pre; ++u) pv[s][u] = pa[s][u];
             } else {                                      // nh > 128 (L > 170)
+                const double2* PP = PL + (j0 / kB1PassRows) * kB1Slots * kB1PalQ * kB1Threads;
 #pragma unroll
                 for (int s = 0; s < kB1Slots; ++s) {
-                    const int i = e0 + ad + kB1AW * s;
 #pragma unroll
-                    for (int u = 0; u < kB1Pre; ++u) {
-                        const int jp = j0 + wave + 8 * (kB1AG * u + ag);
-                        pv[s][u] = (s < nsl && jp < nh && i <= a1) ? P[jp * K + i] : 0.0;
+                    for (int q = 0; q < kB1PalQ; ++q) {
+                        double2 v = c2(0.0, 0.0);
+                        if (s < nsl && j0 + 32 * q < nh) v = PP[(s * kB1PalQ + q) * kB1Threads];
+                        pv[s][2 * q] = v.x;
+                        pv[s][2 * q + 1] = v.y;
                     }
                 }
             }
@@ -622,11 +655,16 @@ static int bve1_max_analysed(const aeo_plan_s* p) {
     return mx;
 }
 
-// Whether the single-member path applies to this model and plan.
+static bool bve1_fits(const aeo_plan_s* p) {
+    return p->kind == kGauss && bve1_lds(p->L, p->nh) <= 163840 && bve1_max_analysed(p) <= 128;
+}
+
+// Whether the single-member path applies to this model and plan (its
+// lane-order table is built with the model: bve1_prepare).
 bool bve1_supported(const aeo_model_s* md) {
     const aeo_plan_s* p = md->plan;
-    return md->kind == kBVE && md->nmem == 1 && md->nslices == 0 && p->kind == kGauss &&
-           bve1_lds(p->L, p->nh) <= 163840 && bve1_max_analysed(p) <= 128;
+    return md->kind == kBVE && md->nmem == 1 && md->nslices == 0 && bve1_fits(p) &&
+           p->d_b1pa != nullptr;
 }
 
 static int bve1_cus() {
@@ -652,6 +690,27 @@ int bve1_msplit(const aeo_plan_s* p) {
     return ms < 0 ? 0 : ms;
 }
 
+static int bve1_npass(const aeo_plan_s* p) { return (p->nh + kB1PassRows - 1) / kB1PassRows; }
+
+int bve1_prepare(aeo_plan_s* p, hipStream_t s) {
+    if (p->d_b1pa != nullptr || !bve1_fits(p)) return AEO_OK;
+    const int L = p->L, ms = bve1_msplit(p), np = bve1_npass(p), nwg = L + 1 + ms;
+    const size_t count = (size_t)nwg * np * kB1Slots * kB1PalQ * kB1Threads;
+    double2* d = nullptr;
+    AEO_CHECK_HIP(hipMalloc((void**)&d, count * sizeof(double2)));
+    launch_k(bve1_pal_kernel, dim3(nwg), dim3(kB1Threads), 0, s, (const double*)p->d_Pa, d, L,
+             p->nh, ms, kB1SplitQ8, (2 * L) / 3, np);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        set_error(std::string("bve1_prepare: ") + hipGetErrorString(e));
+        return AEO_ERR_HIP;
+    }
+    p->d_b1pa = d;
+    return AEO_OK;
+}
+
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
                 double2* kout, int stage, double dt, double2* ynext, bool ana, bool syn,
                 bool hemi, hipStream_t s) {
@@ -669,7 +728,8 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.split_q8 = kB1SplitQ8;
     a.ana = ana ? 1 : 0;
     a.syn = syn ? 1 : 0;
-    a.Pa = p->d_Pa;
+    a.Pal = p->d_b1pa;
+    a.npass = bve1_npass(p);
     a.Ps = p->d_Ps;
     a.xn = p->d_xn;
     a.rec = p->d_rec;

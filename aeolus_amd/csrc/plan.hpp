@@ -63,6 +63,7 @@ struct aeo_plan_s {
     int nlat = 0, nlon = 0, nh = 0;
     double* d_Ps = nullptr;
     double* d_Pa = nullptr;
+    double2* d_b1pa = nullptr;     // single-member BVE stage: its analysis-table rows in lane order
     long long* d_offs = nullptr;
     double* d_w_rows = nullptr;
     double* d_coslat = nullptr;
@@ -252,6 +253,9 @@ bool bve1_supported(const aeo_model_s* md);
 // Heavy columns a synthesising launch splits into two degree halves (the
 // second half's E/O in d_E1/d_O1; the Jacobian grid stage adds the planes).
 int bve1_msplit(const aeo_plan_s* p);
+// The single-member stage's lane-order copy of the analysis table
+// (p->d_b1pa), built once per plan where the single-member path fits it.
+int bve1_prepare(aeo_plan_s* p, hipStream_t s);
 // hemi: S/D hold the unweighted row spectra FN/FS ([m][jp]) of the per-row
 // Jacobian grid kernel (the stage forms S = w (FN + FS), D = w (FN - FS)).
 int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const double2* forcing,
