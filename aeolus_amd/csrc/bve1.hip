@@ -163,6 +163,15 @@ template <int kR>
 __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
                                                                 const double2* __restrict__ recg) {
     extern __shared__ double2 lds[];
+    // the arguments into scalar registers at entry, in one batch: loaded
+    // where first used, they came in four rounds, each waited for before the
+    // next (the analysis-table loads waited for three, the column loads for
+    // four)
+    asm volatile("" ::"s"(a.L), "s"(a.nh), "s"(a.top), "s"(a.msplit), "s"(a.split_q8),
+                 "s"(a.ana), "s"(a.syn), "s"(a.Pal), "s"(a.npass), "s"(a.Ps), "s"(a.xn),
+                 "s"(a.CpT), "s"(a.CmT), "s"(a.lap), "s"(a.nulap));
+    asm volatile("" ::"s"(a.S), "s"(a.D), "s"(a.wrow), "s"(a.hemi), "s"(a.nlat), "s"(a.ys),
+                 "s"(a.y0), "s"(a.ksum), "s"(a.forcing), "s"(a.stage), "s"(recg));
     const int L = a.L, n = L + 1, nh = a.nh;
     // wave index as a wave-uniform (scalar) value: block bounds and loop
     // counters derived from it stay in scalar registers

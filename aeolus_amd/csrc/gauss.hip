@@ -987,6 +987,11 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_hemi_kernel(
     const double2* __restrict__ O1, int msplit) {
     constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
     extern __shared__ double2 lds[];
+    // the arguments into scalar registers at entry, in one batch (loaded
+    // where first used, msplit came in a second round waited for between
+    // the coefficient loads)
+    asm volatile("" ::"s"(g.nh), "s"(g.L), "s"(g.nlat), "s"(g.coslat), "s"(g.fft.tw), "s"(E),
+                 "s"(O), "s"(R), "s"(mmax), "s"(FN), "s"(FS), "s"(E1), "s"(O1), "s"(msplit));
     // workgroup -> (row pair jp, hemisphere hs): ids are dealt to the 8 XCDs
     // round-robin; XCD x takes the nh/8 consecutive row pairs from x nh/8
     // (both rows of each), so the 8 rows whose spectra share a line of
