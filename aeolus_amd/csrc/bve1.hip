@@ -95,18 +95,6 @@ __host__ __device__ inline int bve1_split_point(int K, int q8) {
     return s < 2 ? 2 : (s > hi ? hi : s);
 }
 
-// x(l) + x(l ^ 32) in lane l (in that order for l < 32): v_permlane32_swap
-// exchanges the half-waves in a VALU op, where __shfl_xor(x, 32) takes two
-// trips through the LDS crossbar (ds_bpermute) per double.
-__device__ inline double sum_xor32(double x) {
-    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
-    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-    // [0]: lanes 0-31 keep x(l), lanes 32-63 get x(l - 32); [1]: lanes 0-31
-    // get x(l + 32), lanes 32-63 keep x(l)
-    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
-}
-
 // Workgroup b of a launch: column m and degree half h (-1: whole column;
 // the split columns first, heaviest first); its degrees i = l - m in [d0, d1)
 // (RK4 outputs and synthesis), the state needed on [e0, e1) (the S

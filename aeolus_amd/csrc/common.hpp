@@ -75,6 +75,27 @@ __host__ __device__ inline double2 cfma(double2 a, double s, double2 c) {
     return c2(fma(a.x, s, c.x), fma(a.y, s, c.y));
 }
 
+// x(l) + x(l ^ 32) in lane l of a wave (in that order for l < 32):
+// v_permlane32_swap exchanges the half-waves in a VALU op, where
+// __shfl_xor(x, 32) takes two trips through the LDS crossbar (ds_bpermute)
+// per double.  [0] keeps x(l) in lanes 0-31 and takes x(l - 32) in lanes
+// 32-63; [1] takes x(l + 32) in lanes 0-31 and keeps x(l) in lanes 32-63.
+__device__ inline double sum_xor32(double x) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+// x(l) + x(l ^ 16) (in that order where l & 16 is 0): v_permlane16_swap
+// exchanges the odd 16-lane rows of its first operand with the even rows of
+// its second.
+__device__ inline double sum_xor16(double x) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
 inline int n_basis(int L) { return (L + 1) * (L + 2) / 2; }
 
 // Block size used by the grid (FFT) kernels and the maximum number of

@@ -1325,10 +1325,10 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
                         // (b0 + b2) + (b1 + b3): the association of the LDS tree,
                         // so batched and single-member results agree bitwise
                         double2 e = accE[bb][f], o = accO[bb][f];
-                        e.x += __shfl_xor(e.x, 32); e.y += __shfl_xor(e.y, 32);
-                        o.x += __shfl_xor(o.x, 32); o.y += __shfl_xor(o.y, 32);
-                        e.x += __shfl_xor(e.x, 16); e.y += __shfl_xor(e.y, 16);
-                        o.x += __shfl_xor(o.x, 16); o.y += __shfl_xor(o.y, 16);
+                        e.x = sum_xor32(e.x); e.y = sum_xor32(e.y);
+                        o.x = sum_xor32(o.x); o.y = sum_xor32(o.y);
+                        e.x = sum_xor16(e.x); e.y = sum_xor16(e.y);
+                        o.x = sum_xor16(o.x); o.y = sum_xor16(o.y);
                         accE[bb][f] = e;
                         accO[bb][f] = o;
                     }
