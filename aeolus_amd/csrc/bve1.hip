@@ -187,8 +187,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     const int col = m * n;                       // internal [m][l] column offset
 
     // LDS carve-up: persistent columns, then one scratch region used by the
-    // analysis (S/D, analysed column, wave partials) and later by the
-    // synthesis tree
+    // analysis (S/D, wave partials) and later by the synthesis block sums
     double2* st = lds;                           // [n] state column (stage input, then next state)
     double2* cols = st + n;                      // [4][K] synthesis fields
     double* cpc = reinterpret_cast<double*>(cols + 4 * n);   // [n] C+ of m
@@ -199,9 +198,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
     // of both parities in one instruction) fall in different LDS banks
     const int nhx = nh + 1;
     double2* X = scratch;                        // [2][nh + 1]
-    double2* accl = X + 2 * nhx;                 // [n] analysed column
-    double2* red = accl + n;                     // [8][slots][AW] analysis wave partials
-    double2* sred = scratch;                     // [NB/2][2 parity][2][kR][64] synthesis tree
+    double2* red = X + 2 * nhx;                  // [8][slots][AW] analysis wave partials
+    double2* sred = scratch;                     // [4 blocks][2 parity][2][kR][64] block sums
 
     // ---- prologue: all independent global loads first
     // analysis lanes: degree slot d = lane % AW (degrees e0 + d + AW s, s <
@@ -275,7 +273,6 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         cpc[tid] = vcp;
         cmc[tid] = vcm;
         lapc[tid] = vlap;
-        accl[tid] = c2(0.0, 0.0);
     }
     if (active && tid < nh) {
         if (a.hemi) {
@@ -291,7 +288,6 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
         cpc[t] = a.CpT[col + t];
         cmc[t] = a.CmT[col + t];
         lapc[t] = a.lap[t];
-        accl[t] = c2(0.0, 0.0);
     }
     if (active)
         for (int t = tid + kB1Threads; t < nh; t += kB1Threads) {
@@ -364,18 +360,20 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
             if (ag == 0) red[(wave * kB1Slots + s) * kB1AW + ad] = acc[s];
         }
         __syncthreads();
-        if (tid < kB1AW * kB1Slots) {
-            const int s = tid / kB1AW, d = tid % kB1AW, i = e0 + d + kB1AW * s;
-            if (i <= a1) {
-                double2 sv = red[s * kB1AW + d];
-#pragma unroll
-                for (int w = 1; w < kB1Waves; ++w)
-                    sv = cadd(sv, red[(w * kB1Slots + s) * kB1AW + d]);
-                accl[m + i] = sv;
-            }
-        }
-        __syncthreads();
     }
+    // the analysed value of degree l: the waves' partial sums in wave order,
+    // summed by the epilogue thread of l (0 for degrees not analysed here)
+    auto analysed = [&](int l) {
+        const int i = l - m - e0;
+        double2 sv = c2(0.0, 0.0);
+        if (active && i >= 0 && l - m <= a1) {
+            const int s = i / kB1AW, d = i % kB1AW;
+            sv = red[s * kB1AW + d];
+#pragma unroll
+            for (int w = 1; w < kB1Waves; ++w) sv = cadd(sv, red[(w * kB1Slots + s) * kB1AW + d]);
+        }
+        return sv;
+    };
 
     // synthesis fields psi = zeta / lap (0 at l = 0) and eta = zeta (+ f at
     // (1, 0)) of degree l on [e0, e1) ([f][i], i = l - m; S psi and S eta
@@ -421,7 +419,7 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
             const bool writer = (h < 0) || (h == 0 ? i < d1 : i >= d0);
             double2 k = c2(0.0, 0.0);
             if (l > 0) {
-                const double2 acc = accl[l];
+                const double2 acc = analysed(l);
                 k = cfma(st[l], nl, c2(-acc.x, -acc.y));
                 if (a.forcing) k = cadd(k, fv);
             }
@@ -573,60 +571,45 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
                 }
             }
         }
-        // blocks meet pairwise, b and b + half for half = NB/2, ..., 2, then
-        // wave 0 + wave 1 (four blocks: (b0 + b2) + (b1 + b3)); the last sum
-        // and store of latitude slot q runs on wave q % 2 (a + b == b + a)
-#pragma unroll
-        for (int half = kB1SynWaves / 2; half >= 2; half /= 2) {
-            if (syn_wave && kb >= half && kb < 2 * half) {
-#pragma unroll
-                for (int q = 0; q < kR; ++q)
-#pragma unroll
-                    for (int f = 0; f < 2; ++f) {
-                        slot(kb - half, 0, f, q) = accE[q][f];
-                        slot(kb - half, 1, f, q) = accO[q][f];
-                    }
-            }
-            __syncthreads();
-            if (syn_wave && kb < half) {
-#pragma unroll
-                for (int q = 0; q < kR; ++q)
-#pragma unroll
-                    for (int f = 0; f < 2; ++f) {
-                        accE[q][f] = cadd(accE[q][f], slot(kb, 0, f, q));
-                        accO[q][f] = cadd(accO[q][f], slot(kb, 1, f, q));
-                    }
-            }
-            __syncthreads();
-        }
-        if (syn_wave && kb < 2) {
+        // the four blocks' sums meet in one round: each wave writes the
+        // slots of the (latitude slot q, field f) pairs other waves finish,
+        // then wave w finishes its own pairs (q = w for kR = 4; q = w % 2,
+        // f = w / 2 for kR = 2) as (b0 + b2) + (b1 + b3) and stores them
+        static_assert(kB1SynWaves == 4, "four synthesis blocks");
+        auto owner = [](int q, int f) { return kR == 4 ? q : (q | (f << 1)); };
+        if (syn_wave) {
 #pragma unroll
             for (int q = 0; q < kR; ++q)
-                if ((q & 1) != kb)
 #pragma unroll
-                    for (int f = 0; f < 2; ++f) {
-                        slot(0, 0, f, q) = accE[q][f];
-                        slot(0, 1, f, q) = accO[q][f];
+                for (int f = 0; f < 2; ++f)
+                    if (owner(q, f) != kb) {
+                        slot(kb, 0, f, q) = accE[q][f];
+                        slot(kb, 1, f, q) = accO[q][f];
                     }
         }
         __syncthreads();
-        if (syn_wave && kb < 2) {
+        if (syn_wave) {
 #pragma unroll
             for (int q = 0; q < kR; ++q) {
                 const int jp = jbase + g + 32 * q;
-                if ((q & 1) == kb && jp < nh) {
-                    const int o = (jp * n + m) * 4 + 2 * hf;
 #pragma unroll
-                    for (int f = 0; f < 2; ++f) {
-                        const double2 ev = cadd(accE[q][f], slot(0, 0, f, q));
-                        const double2 od = cadd(accO[q][f], slot(0, 1, f, q));
-                        if (a.hemi) {            // the row values' spectra N = E + O, S = E - O
-                            Ep[o + f] = cadd(ev, od);
-                            Op[o + f] = csub(ev, od);
-                        } else {
-                            Ep[o + f] = ev;
-                            Op[o + f] = od;
-                        }
+                for (int f = 0; f < 2; ++f) {
+                    if (owner(q, f) != kb || jp >= nh) continue;
+                    double2 e4[4], o4[4];
+#pragma unroll
+                    for (int bb = 0; bb < 4; ++bb) {
+                        e4[bb] = (bb == kb) ? accE[q][f] : slot(bb, 0, f, q);
+                        o4[bb] = (bb == kb) ? accO[q][f] : slot(bb, 1, f, q);
+                    }
+                    const double2 ev = cadd(cadd(e4[0], e4[2]), cadd(e4[1], e4[3]));
+                    const double2 od = cadd(cadd(o4[0], o4[2]), cadd(o4[1], o4[3]));
+                    const int o = (jp * n + m) * 4 + 2 * hf + f;
+                    if (a.hemi) {                // the row values' spectra N = E + O, S = E - O
+                        Ep[o] = cadd(ev, od);
+                        Op[o] = csub(ev, od);
+                    } else {
+                        Ep[o] = ev;
+                        Op[o] = od;
                     }
                 }
             }
@@ -637,8 +620,8 @@ __global__ __launch_bounds__(kB1Threads) void bve1_stage_kernel(Bve1Args a,
 
 static size_t bve1_lds(int L, int nh) {
     const size_t n = L + 1, R = (nh > 64) ? 4 : 2;
-    const size_t ana = 2 * ((size_t)nh + 1) + n + kB1Waves * kB1Slots * kB1AW,
-                 syn = (kB1SynWaves / 2) * 2 * 2 * R * 64;
+    const size_t ana = 2 * ((size_t)nh + 1) + kB1Waves * kB1Slots * kB1AW,
+                 syn = kB1SynWaves * 2 * 2 * R * 64;
     return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 
