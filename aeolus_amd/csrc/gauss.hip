@@ -223,6 +223,24 @@ __device__ inline void grid_wg_index(int nh, int nmem, int& jp, int& b) {
     }
 }
 
+// Row constants of a north/south row pair (clamped cos(latitude) of both
+// rows, the row weight; halved for an equator row) through the vector-memory
+// path: as scalar loads through a pointer argument they came in a round of
+// their own, waited for (with the argument loads) before the kernel's first
+// global load.
+__device__ inline double vload_f64(const double* p, int i) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double*>(p), (short)0, 0x7fffffff, 0x00020000);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+}
+__device__ inline void row_pair_consts(const GaussDev& g, int jN, int jS, double& cN, double& cS,
+                                       double& w) {
+    cN = clamp_coslat(vload_f64(g.coslat, jN));
+    cS = clamp_coslat(vload_f64(g.coslat, jS));
+    const double wr = vload_f64(g.w_rows, jN);
+    w = (jN == jS) ? 0.5 * wr : wr;
+}
+
 // Prologue of the fused grid kernels: the twiddle table (n entries at twl)
 // and the NF even/odd columns of one member of a row pair into LDS, field
 // major (sE/sO[f * (L+1) + m], m <= L), so the per-row spectrum packing reads
@@ -459,6 +477,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
                                                                int mmax, int nmem, double2* S,
                                                                double2* D) {
     extern __shared__ double2 lds[];
+    // the arguments into scalar registers at entry, in one batch
+    asm volatile("" ::"s"(g.nh), "s"(g.nlat), "s"(g.L), "s"(g.nlon), "s"(g.coslat),
+                 "s"(g.w_rows), "s"(g.fft.tw), "s"(E), "s"(O), "s"(R), "s"(mmax), "s"(nmem),
+                 "s"(S), "s"(D));
     int jp, b;
     grid_wg_index(g.nh, nmem, jp, b);
     // two workgroups per (row pair, member): half 0 forms the products
@@ -471,9 +493,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_swe_kernel(GaussDev g, cons
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 10;
     const size_t ms = (size_t)nmem * 10;
-    const double cN = clamp_coslat(g.coslat[jN]);
-    const double cS = clamp_coslat(g.coslat[jS]);
-    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
+    double cN, cS, w;
+    row_pair_consts(g, jN, jS, cN, cS, w);
     // grid row r <- (Legendre field, spectral factor)
     constexpr int src[12] = {0, 1, 2, 3, 4, 5, 4, 6, 7, 6, 8, 9};
     constexpr int mode[12] = {kSpecDlam, kSpecScale, kSpecDlam, kSpecScale, kSpecDlam,
@@ -860,6 +881,10 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
                                                                   const double2* O1, int msplit) {
     constexpr int N = 1 << LGN, RS = N + (N >> 3), NB = N >> 3;
     extern __shared__ double2 lds[];
+    // the arguments into scalar registers at entry, in one batch
+    asm volatile("" ::"s"(g.nh), "s"(g.nlat), "s"(g.L), "s"(g.coslat), "s"(g.w_rows),
+                 "s"(g.fft.tw), "s"(E), "s"(O), "s"(R), "s"(mmax), "s"(nmem), "s"(S), "s"(D),
+                 "s"(E1), "s"(O1), "s"(msplit));
     int jp, b;
     grid_wg_index(g.nh, nmem, jp, b);
     const int L = g.L;
@@ -867,9 +892,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_jac_p2_kernel(GaussDev g, c
     const double invR = 1.0 / R;
     const size_t eo = ((size_t)jp * (L + 1) * nmem + b) * 4;
     const size_t ms = (size_t)nmem * 4;
-    const double cN = clamp_coslat(g.coslat[jN]);
-    const double cS = clamp_coslat(g.coslat[jS]);
-    const double w = (jN == jS) ? 0.5 * g.w_rows[jN] : g.w_rows[jN];
+    double cN, cS, w;
+    row_pair_consts(g, jN, jS, cN, cS, w);
     // LDS: rows [4][RS] | twiddle table [N] | E, O columns [4][L+1] each.
     // COMPACT (member batches: more workgroups per CU): a quarter twiddle
     // table [N/4], and the E/O columns (8 (L+1) <= 4 RS as N > 2L) alias the
