@@ -566,6 +566,13 @@ __global__ __launch_bounds__(kStageThreads) void leg_stage_kernel(GaussDev g, St
     constexpr int kPre = Prefetch<MB, NA, NF>::PA;
     constexpr int kPC = Prefetch<MB, NA, NF>::PC;
     extern __shared__ double2 lds[];
+    // the plan and stage arguments the prologue reads into scalar registers
+    // at entry, in one batch (loaded where first used, they came in rounds
+    // waited for between the prologue's global loads)
+    asm volatile("" ::"s"(g.L), "s"(g.nh), "s"(g.Pa), "s"(g.Ps), "s"(g.xn), "s"(g.rec),
+                 "s"(a.ys), "s"(a.ys_stride), "s"(a.ys_aux), "s"(a.ys_mmajor), "s"(a.nmem),
+                 "s"(a.nstate_valid), "s"(a.S), "s"(a.D), "s"(a.lmax_a), "s"(a.mmax_a));
+    asm volatile("" ::"s"(a.E), "s"(a.O), "s"(a.k.Cp), "s"(a.k.Cm), "s"(a.k.lap));
     const int m = a.ys_mmajor ? (int)blockIdx.x : stage_column(blockIdx.x, blockIdx.z);
     const int b0 = blockIdx.z * MB;
     const int L = g.L, nh = g.nh, n = L + 1;
