@@ -85,9 +85,11 @@ struct Bve1Args {
 };
 
 // Degree split point of a heavy column with K degrees: even, 2 <= s <= K - 2,
-// about K q8 / 256 but at most 62, so the first half's analysis (degrees
-// 0..s, the neighbour included) fits one 64-lane slot (a second slot doubles
-// its table loads: measured 0.0803 -> 0.0821 ms per L = 170 step).
+// about K q8 / 256 but at most 62, so the first half analyses degrees 0..s
+// (the neighbour included) in two 32-degree slots (first chosen for one
+// 64-lane slot: a second then doubled its table loads, 0.0803 -> 0.0821 ms
+// per L = 170 step; re-swept with 32-degree slots and the lane-order table,
+// 93 / 62 stayed the fastest).
 __host__ __device__ inline int bve1_split_point(int K, int q8) {
     int s = ((K * q8) >> 8) & ~1;
     if (s > 62) s = 62;
@@ -625,8 +627,8 @@ static size_t bve1_lds(int L, int nh) {
     return sizeof(double2) * (5 * n + (ana > syn ? ana : syn)) + sizeof(double) * (3 * n + (n & 1));
 }
 
-// The most degrees any workgroup of a launch analyses: the kernel's two
-// 64-lane analysis slots hold 128 (degrees past them would be left out).
+// The most degrees any workgroup of a launch analyses: the kernel's four
+// 32-degree analysis slots hold 128 (degrees past them would be left out).
 static int bve1_max_analysed(const aeo_plan_s* p) {
     const int n = p->L + 1, top = (2 * p->L) / 3, ms = bve1_msplit(p);
     int mx = 0;
@@ -711,7 +713,7 @@ int bve1_launch(aeo_model_s* md, const double2* ys, const double2* y0, const dou
     a.top = (2 * L) / 3;
     // the heavy columns run as two degree halves in every launch (in one
     // without synthesis too: a whole heavy column can analyse more degrees
-    // than the two slots hold)
+    // than the four slots hold)
     const int ms = bve1_msplit(p);
     a.msplit = ms;
     a.split_q8 = kB1SplitQ8;
